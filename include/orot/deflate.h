@@ -1,0 +1,147 @@
+/*
+ * orot/deflate.h -- drop-in C API of orot's DEFLATE compressor, served by the
+ * MI355X (gfx950) implementation in liborot_amd.so.
+ *
+ * The types, values and entry points below keep the meaning of orot's
+ * include/orot/deflate.h; each declaration cites the orot line it replaces.
+ * Entry points of orot that are not part of the GPU compress path
+ * (deflate_decompress, the streaming API, deflate_set_allocator) are not
+ * exported by this library.
+ *
+ * Deviations from orot, all deliberate:
+ *   - gzip trailers carry the IEEE CRC-32 on every host (orot's x86 build
+ *     writes CRC-32C, src/simd/x86/match_sse42.cpp:76-99);
+ *   - deflate_parallel_compress returns 0 instead of writing past
+ *     out_capacity when the compressed stream does not fit;
+ *   - num_threads is accepted and ignored (the GPU decides its parallelism);
+ *   - levels 1-3 accept block sizes up to 256 KiB (L1) / 128 KiB (L2, L3).
+ */
+#ifndef OROT_DEFLATE_H
+#define OROT_DEFLATE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Result codes (orot include/orot/deflate.h:15-25). */
+typedef enum deflate_result {
+    DEFLATE_OK = 0,
+    DEFLATE_STREAM_END = 1,
+    DEFLATE_NEED_INPUT = 2,
+    DEFLATE_NEED_OUTPUT = 3,
+    DEFLATE_DATA_ERROR = -1,
+    DEFLATE_MEM_ERROR = -2,
+    DEFLATE_BUF_ERROR = -3,
+    DEFLATE_VERSION_ERROR = -4,
+    DEFLATE_PARAM_ERROR = -5
+} deflate_result;
+
+/* Framing (orot include/orot/deflate.h:28-32). */
+typedef enum deflate_format {
+    DEFLATE_FORMAT_RAW = 0,  /* RFC 1951 */
+    DEFLATE_FORMAT_ZLIB = 1, /* RFC 1950, Adler-32 trailer */
+    DEFLATE_FORMAT_GZIP = 2  /* RFC 1952, CRC-32 + ISIZE trailer */
+} deflate_format;
+
+/* Levels (orot include/orot/deflate.h:35-39); values are clamped to 0..12. */
+#define DEFLATE_LEVEL_STORE 0
+#define DEFLATE_LEVEL_FAST 1
+#define DEFLATE_LEVEL_DEFAULT 6
+#define DEFLATE_LEVEL_BETTER 9
+#define DEFLATE_LEVEL_MAX 12
+
+/* Upper bound of the compressed size (orot include/orot/deflate.h:66,
+ * src/api/deflate_api.cpp:81-98): n + n/4096 + n/16384 + n/2^25 + 13, plus 6
+ * (zlib) or 18 (gzip). */
+size_t deflate_compress_bound(size_t in_size, deflate_format format);
+
+/* Parallel compressor (orot include/orot/deflate.h:120-145,
+ * src/api/stream_api.cpp:384-406).  block_size 0 selects 128 KiB for levels
+ * <= 3, 512 KiB for levels <= 6 and 1 MiB above.  Returns NULL when no GPU is
+ * usable or on allocation failure. */
+typedef struct deflate_parallel_ctx deflate_parallel_ctx;
+
+deflate_parallel_ctx* deflate_parallel_new(int level, deflate_format format, int num_threads, size_t block_size);
+void deflate_parallel_free(deflate_parallel_ctx* ctx);
+
+/* Compress in[0..in_size) (host memory) into out[0..out_capacity).  Blocks
+ * until done.  Returns the compressed size or 0 on failure. */
+size_t deflate_parallel_compress(deflate_parallel_ctx* ctx, const void* in, size_t in_size, void* out,
+                                 size_t out_capacity);
+
+/* Checksums (orot include/orot/deflate.h:149-150). */
+uint32_t deflate_adler32(uint32_t initial, const void* data, size_t len);
+uint32_t deflate_crc32(uint32_t initial, const void* data, size_t len);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+#include <span>
+#include <stdexcept>
+#include <vector>
+
+namespace orot {
+namespace deflate {
+
+/* C++ wrappers with the names and behaviour of orot's header-only wrappers
+ * (orot include/orot/deflate.h:188-211, 344-387). */
+enum class Format : int { Raw = DEFLATE_FORMAT_RAW, Zlib = DEFLATE_FORMAT_ZLIB, Gzip = DEFLATE_FORMAT_GZIP };
+
+struct Level {
+    static constexpr int Store = DEFLATE_LEVEL_STORE;
+    static constexpr int Fast = DEFLATE_LEVEL_FAST;
+    static constexpr int Default = DEFLATE_LEVEL_DEFAULT;
+    static constexpr int Better = DEFLATE_LEVEL_BETTER;
+    static constexpr int Max = DEFLATE_LEVEL_MAX;
+};
+
+class Error : public std::runtime_error {
+public:
+    Error(deflate_result code, const char* what) : std::runtime_error(what), code_(code) {}
+    deflate_result code() const noexcept { return code_; }
+
+private:
+    deflate_result code_;
+};
+
+class ParallelCompressor {
+public:
+    explicit ParallelCompressor(int level = Level::Default, Format fmt = Format::Gzip, int threads = 0,
+                                size_t block_size = 0)
+        : ctx_(deflate_parallel_new(level, static_cast<deflate_format>(fmt), threads, block_size)) {
+        if (!ctx_) throw Error(DEFLATE_MEM_ERROR, "ParallelCompressor: allocation failed");
+    }
+    ~ParallelCompressor() { deflate_parallel_free(ctx_); }
+    ParallelCompressor(const ParallelCompressor&) = delete;
+    ParallelCompressor& operator=(const ParallelCompressor&) = delete;
+
+    /* Output sized with the gzip bound whatever the format, as orot does. */
+    std::vector<uint8_t> compress(std::span<const uint8_t> input) {
+        std::vector<uint8_t> out(deflate_compress_bound(input.size(), DEFLATE_FORMAT_GZIP));
+        const size_t n = deflate_parallel_compress(ctx_, input.data(), input.size(), out.data(), out.size());
+        if (n == 0) throw Error(DEFLATE_BUF_ERROR, "ParallelCompressor::compress: buffer overflow");
+        out.resize(n);
+        return out;
+    }
+
+private:
+    deflate_parallel_ctx* ctx_;
+};
+
+inline uint32_t adler32(std::span<const uint8_t> data, uint32_t init = 1) {
+    return deflate_adler32(init, data.data(), data.size());
+}
+inline uint32_t crc32(std::span<const uint8_t> data, uint32_t init = 0) {
+    return deflate_crc32(init, data.data(), data.size());
+}
+
+}  // namespace deflate
+}  // namespace orot
+#endif /* __cplusplus */
+
+#endif /* OROT_DEFLATE_H */

@@ -1,0 +1,49 @@
+/*
+ * orot/deflate_gpu.h -- device-resident extensions of liborot_amd.so.
+ *
+ * None of these has a counterpart in orot; they exist for callers whose data
+ * already lives in GPU memory (benchmarks, pipelines that produce the input on
+ * the GPU) and for test harnesses that hold no GPU runtime of their own.
+ */
+#ifndef OROT_DEFLATE_GPU_H
+#define OROT_DEFLATE_GPU_H
+
+#include "deflate.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* deflate_parallel_compress with d_in / d_out in device memory.  Same output
+ * bytes as the host entry point; blocks until the result size is known. */
+size_t deflate_parallel_compress_device(deflate_parallel_ctx* ctx, const void* d_in, size_t in_size, void* d_out,
+                                        size_t out_capacity);
+
+/* Per-stage device time of the last call on ctx, in milliseconds, measured
+ * with HIP events on the context's stream: [0] checksum, [1] LZ77 + block
+ * emission, [2] offset scan + framing, [3] assembly.  Timing is recorded
+ * only after deflate_parallel_set_timing(ctx, 1).  Returns the number of
+ * values written. */
+void deflate_parallel_set_timing(deflate_parallel_ctx* ctx, int on);
+int deflate_parallel_stage_times(const deflate_parallel_ctx* ctx, float* ms, int max_values);
+
+/* Device memory helpers (hipMalloc / hipFree / hipMemcpy / sync).
+ * kind: 0 host->device, 1 device->host, 2 device->device.  Return 0 on
+ * success. */
+void* orot_gpu_malloc(size_t bytes);
+int orot_gpu_free(void* ptr);
+int orot_gpu_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int orot_gpu_memset(void* dst, int value, size_t bytes);
+int orot_gpu_synchronize(void);
+
+/* Device name and compute-unit count; returns 0 on success. */
+int orot_gpu_device_info(char* name, size_t name_len, int* compute_units);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* orot_gpu_last_error(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* OROT_DEFLATE_GPU_H */

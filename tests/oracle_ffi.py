@@ -1,0 +1,130 @@
+"""ctypes bindings of the oracle (oracle/liboracle.so) and, when it was built
+in this checkout, of orot itself (oracle/_ref/liborot_ref.so).  Test
+infrastructure only."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "liborot_ref.so")
+
+_vp, _sz = ctypes.c_void_p, ctypes.c_size_t
+_o = None
+_r = None
+
+
+def oracle() -> ctypes.CDLL:
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_PATH):
+            raise ImportError(f"{ORACLE_PATH} missing: run `make -C oracle oracle`")
+        L = ctypes.CDLL(ORACLE_PATH)
+        L.oracle_crc32.restype = ctypes.c_uint32
+        L.oracle_crc32.argtypes = [ctypes.c_uint32, _vp, _sz]
+        L.oracle_adler32.restype = ctypes.c_uint32
+        L.oracle_adler32.argtypes = [ctypes.c_uint32, _vp, _sz]
+        L.oracle_crc32_combine.restype = ctypes.c_uint32
+        L.oracle_crc32_combine.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
+        L.oracle_adler32_combine.restype = ctypes.c_uint32
+        L.oracle_adler32_combine.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64]
+        L.oracle_compress_bound.restype = _sz
+        L.oracle_compress_bound.argtypes = [_sz, ctypes.c_int]
+        L.oracle_lz77.restype = _sz
+        L.oracle_lz77.argtypes = [_vp, _sz, ctypes.c_int, _vp]
+        L.oracle_l1_candidates.restype = None
+        L.oracle_l1_candidates.argtypes = [_vp, _sz, ctypes.c_int, _vp]
+        L.oracle_parallel_compress.restype = _sz
+        L.oracle_parallel_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz, _vp, _sz, _vp, _sz]
+        L.oracle_limited_lengths.restype = ctypes.c_int
+        L.oracle_limited_lengths.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp]
+        _o = L
+    return _o
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref() -> ctypes.CDLL:
+    global _r
+    if _r is None:
+        L = ctypes.CDLL(REF_PATH)
+        L.deflate_parallel_new.restype = _vp
+        L.deflate_parallel_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz]
+        L.deflate_parallel_compress.restype = _sz
+        L.deflate_parallel_compress.argtypes = [_vp, _vp, _sz, _vp, _sz]
+        L.deflate_parallel_free.restype = None
+        L.deflate_parallel_free.argtypes = [_vp]
+        L.ref_lz77.restype = _sz
+        L.ref_lz77.argtypes = [_vp, _sz, ctypes.c_int, _vp]
+        _r = L
+    return _r
+
+
+def _u8(data) -> np.ndarray:
+    if isinstance(data, np.ndarray):
+        return np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    return np.frombuffer(bytes(data), dtype=np.uint8)
+
+
+def crc32(data, init=0) -> int:
+    a = _u8(data)
+    return int(oracle().oracle_crc32(init, a.ctypes.data, a.size))
+
+
+def adler32(data, init=1) -> int:
+    a = _u8(data)
+    return int(oracle().oracle_adler32(init, a.ctypes.data, a.size))
+
+
+def lz77(data, level: int) -> np.ndarray:
+    a = _u8(data)
+    t = np.zeros(max(1, a.size), dtype=np.uint32)
+    n = oracle().oracle_lz77(a.ctypes.data, a.size, level, t.ctypes.data)
+    return t[:n]
+
+
+def l1_candidates(data, hash_bits: int = 12) -> np.ndarray:
+    a = _u8(data)
+    d = np.zeros(max(1, a.size), dtype=np.uint16)
+    oracle().oracle_l1_candidates(a.ctypes.data, a.size, hash_bits, d.ctypes.data)
+    return d[: a.size]
+
+
+def parallel_compress(data, level: int, fmt: int, block_size: int = 0, threads: int = 1) -> bytes:
+    a = _u8(data)
+    cap = a.size + a.size // 8 + 4096
+    out = np.zeros(cap, dtype=np.uint8)
+    n = oracle().oracle_parallel_compress(level, fmt, threads, block_size, a.ctypes.data, a.size, out.ctypes.data, cap)
+    if n == 0:
+        raise RuntimeError("oracle_parallel_compress failed")
+    return out[:n].tobytes()
+
+
+def limited_lengths(freq, max_bits: int) -> np.ndarray:
+    f = np.ascontiguousarray(freq, dtype=np.uint32)
+    lens = np.zeros(f.size, dtype=np.uint8)
+    oracle().oracle_limited_lengths(f.ctypes.data, f.size, max_bits, lens.ctypes.data)
+    return lens
+
+
+def ref_parallel_compress(data, level: int, fmt: int, block_size: int = 0, threads: int = 1) -> bytes:
+    a = _u8(data)
+    L = ref()
+    ctx = L.deflate_parallel_new(level, fmt, threads, block_size)
+    cap = a.size + a.size // 8 + 4096
+    out = np.zeros(cap, dtype=np.uint8)
+    n = L.deflate_parallel_compress(ctx, a.ctypes.data, a.size, out.ctypes.data, cap)
+    L.deflate_parallel_free(ctx)
+    return out[:n].tobytes()
+
+
+def ref_lz77(data, level: int) -> np.ndarray:
+    a = _u8(data)
+    t = np.zeros(max(1, a.size), dtype=np.uint32)
+    n = ref().ref_lz77(a.ctypes.data, a.size, level, t.ctypes.data)
+    return t[:n]

@@ -1,0 +1,49 @@
+"""GPU parity of the L0-L3 parallel path against the oracle (byte-exact),
+zlib inflate of every output, and orot's error behaviour."""
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle_ffi as orc
+from orot_amd import corpus
+
+pytestmark = pytest.mark.gpu
+
+WBITS = {0: -15, 1: 15, 2: 31}
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import orot_amd
+
+    orot_amd.lib()
+    return orot_amd
+
+
+def _check(amd, data, level, fmt, block):
+    # output sized like orot's parallel test: bound + 4096 (test_deflate_parallel.cpp:32)
+    out = np.empty(amd.compress_bound(len(data), fmt) + 4096, dtype=np.uint8)
+    with amd.ParallelCompressor(level, fmt, 0, block) as pc:
+        n = pc.compress_into(np.ascontiguousarray(data, dtype=np.uint8), out)
+    assert n > 0, amd.last_error()
+    got = out[:n].tobytes()
+    want = orc.parallel_compress(data, level, fmt, block)
+    assert got == want, (len(got), len(want))
+    assert zlib.decompress(got, WBITS[fmt]) == bytes(np.asarray(data, dtype=np.uint8))
+
+
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+@pytest.mark.parametrize("n", [0, 1, 3, 4, 5, 100, 4096, 70000])
+@pytest.mark.parametrize("fmt", [0, 1, 2])
+def test_small_l1(amd, kind, n, fmt):
+    data = corpus.generate(kind, n, 1000 + n)
+    _check(amd, data, 1, fmt, 0)
+
+
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+@pytest.mark.parametrize("level", [0, 1, 2, 3])
+@pytest.mark.parametrize("block", [0, 1024, 65536, 131072])
+def test_levels_blocks(amd, kind, level, block):
+    data = corpus.generate(kind, 300_000, 77)
+    _check(amd, data, level, 2, block)
