@@ -39,6 +39,8 @@ def oracle() -> ctypes.CDLL:
         L.oracle_l1_candidates.argtypes = [_vp, _sz, ctypes.c_int, _vp]
         L.oracle_parallel_compress.restype = _sz
         L.oracle_parallel_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz, _vp, _sz, _vp, _sz]
+        L.oracle_encode_block_fixed.restype = ctypes.c_int
+        L.oracle_encode_block_fixed.argtypes = [_vp, _sz, _vp, _sz, ctypes.c_int, _vp, _vp]
         L.oracle_limited_lengths.restype = ctypes.c_int
         L.oracle_limited_lengths.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp]
         _o = L

@@ -1,0 +1,60 @@
+"""The C ABI: liborot_amd.so loads without a GPU and exports every symbol the
+headers under include/ declare; host-only entry points behave as orot's."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import orot_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for hdr in ("deflate.h", "deflate_gpu.h"):
+        with open(os.path.join(ROOT, "include", "orot", hdr)) as f:
+            text = f.read()
+        c_part = text.split("#ifdef __cplusplus\n#include")[0]
+        for m in re.finditer(r"^\s*[A-Za-z_][\w \*]*?\b(\w+)\s*\(", c_part, re.M):
+            name = m.group(1)
+            if name.startswith(("deflate_", "orot_gpu_")):
+                names.add(name)
+    return names
+
+
+def test_library_built():
+    assert os.path.exists(orot_amd.LIB_PATH), "run python -m orot_amd.build"
+
+
+def test_every_declared_symbol_exported():
+    lib = ctypes.CDLL(orot_amd.LIB_PATH)
+    declared = _declared()
+    assert declared == set(orot_amd.EXPORTED_SYMBOLS)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+@pytest.mark.parametrize("n", [0, 1, 4096, 65535, 1 << 20, (1 << 33) + 5])
+def test_bound_matches_orot(n):
+    raw = n + (n >> 12) + (n >> 14) + (n >> 25) + 13
+    assert orot_amd.compress_bound(n, 0) == raw
+    assert orot_amd.compress_bound(n, 1) == raw + 6
+    assert orot_amd.compress_bound(n, 2) == raw + 18
+
+
+def test_no_gpu_means_no_context():
+    """Without a GPU the parallel context cannot be created: there is no CPU
+    compression path behind the ABI."""
+    import subprocess
+    import sys
+
+    code = (
+        "import orot_amd,sys\n"
+        "L=orot_amd.lib()\n"
+        "ctx=L.deflate_parallel_new(1,0,0,0)\n"
+        "sys.exit(0 if (ctx is None) == (L.orot_gpu_synchronize()!=0) else 1)\n"
+    )
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env={**os.environ, "PYTHONPATH": ROOT})
+    assert r.returncode == 0

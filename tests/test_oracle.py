@@ -1,0 +1,284 @@
+"""Pins the oracle (oracle/orot_oracle.c) before it is trusted as the checker.
+
+Each test names what fixes the expected value: orot's own KATs and table
+facts, orot's bytes (golden fixtures written by orot, tests/golden/), zlib,
+closed forms, or brute force.  CPU only.
+"""
+import hashlib
+import itertools
+import json
+import os
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle_ffi as orc
+from orot_amd import corpus
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WBITS = {0: -15, 1: 15, 2: 31}
+
+
+@pytest.fixture(scope="module")
+def fixtures():
+    with open(os.path.join(HERE, "golden", "orot_fixtures.json")) as f:
+        return json.load(f)
+
+
+# ---- checksums: orot KATs (tests/unit/test_deflate_formats.cpp:74-88) and zlib
+def test_crc32_kat():
+    assert orc.crc32(b"123456789") == 0xCBF43926
+
+
+def test_adler32_kat():
+    assert orc.adler32(b"Wikipedia") == 0x11E60398
+
+
+def test_checksums_empty_is_initial():  # test_deflate_simd.cpp: empty = initial value
+    assert orc.crc32(b"", 0) == 0
+    assert orc.crc32(b"", 0x12345678) == 0x12345678
+    assert orc.adler32(b"", 1) == 1
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 5551, 5552, 5553, 100_000, 3_000_001])
+def test_checksums_vs_zlib(n):
+    d = corpus.random_bytes(n, n)
+    assert orc.crc32(d) == zlib.crc32(d.tobytes())
+    assert orc.adler32(d) == zlib.adler32(d.tobytes())
+    assert orc.crc32(d, 0xDEADBEEF) == zlib.crc32(d.tobytes(), 0xDEADBEEF)
+    assert orc.adler32(d, 0x00450011) == zlib.adler32(d.tobytes(), 0x00450011)
+
+
+def test_checksums_long_runs_vs_zlib():  # all-0xFF stresses Adler's NMAX reduction
+    d = np.full(1_000_003, 0xFF, dtype=np.uint8)
+    assert orc.adler32(d) == zlib.adler32(d.tobytes())
+    assert orc.crc32(d) == zlib.crc32(d.tobytes())
+
+
+@pytest.mark.parametrize("split", [0, 1, 4, 1000, 65536, 99_999])
+def test_combine_vs_zlib(split):
+    d = corpus.random_bytes(100_000, 5).tobytes()
+    a, b = d[:split], d[split:]
+    O = orc.oracle()
+    assert O.oracle_crc32_combine(zlib.crc32(a), zlib.crc32(b), len(b)) == zlib.crc32(d)
+    assert O.oracle_adler32_combine(zlib.adler32(a), zlib.adler32(b), len(b)) == zlib.adler32(d)
+
+
+# ---- bound (src/formats/raw_deflate.cpp:9-12, src/api/deflate_api.cpp:81-98)
+@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 16384, 1 << 25, (1 << 32) + 7])
+def test_bound_closed_form(n):
+    raw = n + (n >> 12) + (n >> 14) + (n >> 25) + 13
+    O = orc.oracle()
+    assert O.oracle_compress_bound(n, 0) == raw
+    assert O.oracle_compress_bound(n, 1) == raw + 6
+    assert O.oracle_compress_bound(n, 2) == raw + 18
+
+
+# ---- closed-form outputs derivable without orot (empty input, one byte)
+def test_empty_and_one_byte_closed_forms():
+    e = np.zeros(0, dtype=np.uint8)
+    assert orc.parallel_compress(e, 1, 0) == bytes.fromhex("0300")
+    assert orc.parallel_compress(e, 6, 0) == bytes.fromhex("0300")
+    assert orc.parallel_compress(e, 1, 1) == bytes.fromhex("789c030000000001")
+    assert orc.parallel_compress(e, 1, 2) == bytes.fromhex("1f8b08000000000000ff0300000000000000000000")[:20]
+    assert orc.parallel_compress(e, 0, 0) == bytes.fromhex("010000ffff")
+    a = np.frombuffer(b"a", dtype=np.uint8)
+    assert orc.parallel_compress(a, 1, 0) == bytes.fromhex("4b0400")
+    assert orc.parallel_compress(a, 6, 0) == bytes.fromhex("4b0400")
+    assert orc.parallel_compress(a, 1, 1) == bytes.fromhex("789c4b0400006200 62".replace(" ", ""))
+
+
+# ---- RFC 1951 tables, through the fixed-Huffman encoding of single symbols
+# (orot test_deflate_huffman.cpp:21-76: fixed lens 8/9/7/8, LENGTH_BASE,
+# DIST_BASE, length_to_code(3)=0, (258)=28).  A stream holding one match of
+# length L at distance d inflates through zlib only if base / extra tables
+# are right.
+@pytest.mark.parametrize("L", [3, 4, 10, 11, 12, 18, 19, 34, 35, 66, 67, 130, 131, 257, 258])
+@pytest.mark.parametrize("d", [1, 2, 4, 5, 8, 9, 24577, 32768])
+def test_match_encoding_via_zlib(L, d):
+    # d distinct literal bytes prefix so a match at distance d is legal, then L bytes copying
+    rng = np.random.default_rng(L * 131 + d)
+    pre = rng.integers(0, 256, size=d, dtype=np.uint8)
+    body = np.concatenate([pre, np.resize(pre, L) if d >= L else np.resize(pre, L)])
+    toks = [int(x) for x in pre] + [0x80000000 | (L << 16) | d]
+    t = np.array(toks, dtype=np.uint32)
+    dst = np.zeros(len(toks) * 4 + 64, dtype=np.uint8)
+    bp = np.zeros(1, dtype=np.uint64)
+    O = orc.oracle()
+    O.oracle_encode_block_fixed(t.ctypes.data, t.size, body.ctypes.data, 10**9, 1, dst.ctypes.data, bp.ctypes.data)
+    stream = dst[: (int(bp[0]) + 7) // 8].tobytes()
+    want = bytearray(pre.tobytes())
+    for i in range(L):
+        want.append(want[len(want) - d])
+    assert zlib.decompress(stream, -15) == bytes(want)
+
+
+# ---- L1 candidate rule against a brute-force restatement of orot's fast path
+def _brute_candidates(buf, bits):
+    n = len(buf)
+    out = [0] * n
+    hist = {}
+    for p in range(n):
+        if p + 4 > n:
+            continue
+        v = int.from_bytes(buf[p:p + 4], "little")
+        h = ((v * 0x9E3779B1) & 0xFFFFFFFF) >> (32 - bits)
+        c = hist.get(h, 0)
+        hist[h] = p
+        if c == 0:
+            continue
+        d = (p - c) & 0x7FFF
+        if d and buf[c] == buf[p]:
+            out[p] = d
+    return out
+
+
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+def test_l1_candidates_bruteforce(kind):
+    d = corpus.generate(kind, 40_000, 3)
+    assert list(orc.l1_candidates(d, 12)) == _brute_candidates(d.tobytes(), 12)
+    assert list(orc.l1_candidates(d, 14)) == _brute_candidates(d.tobytes(), 14)
+
+
+def _greedy_from_candidates(buf, cand):
+    """orot's fast-path parse (lz77.cpp:378-406) restated over candidates."""
+    n = len(buf)
+    toks = []
+    p = 0
+    while p < n:
+        l = 0
+        if p + 4 <= n and cand[p]:
+            m = min(258, n - p)
+            d = cand[p]
+            while l < m and buf[p + l] == buf[p - d + l]:
+                l += 1
+        if l >= 3:
+            toks.append(0x80000000 | (l << 16) | cand[p])
+            p += l
+        else:
+            toks.append(buf[p])
+            p += 1
+    return toks
+
+
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+def test_l1_parse_is_greedy_over_candidates(kind):
+    d = corpus.generate(kind, 30_000, 11)
+    assert list(orc.lz77(d, 1)) == _greedy_from_candidates(d.tobytes(), [int(x) for x in orc.l1_candidates(d, 12)])
+
+
+# ---- orot's own bytes and token streams (golden fixtures written by orot)
+def test_golden_small_payloads(fixtures):
+    bad = []
+    for e in fixtures["small"]:
+        data = corpus.generate(e["kind"], e["n"], e["seed"])
+        got = orc.parallel_compress(data, e["level"], e["format"], e["block"])
+        if got.hex() != e["hex"]:
+            bad.append(e)
+    assert not bad, bad[:3]
+
+
+def test_golden_large_digests(fixtures):
+    for e in fixtures["large"]:
+        data = corpus.generate(e["kind"], e["n"], e["seed"])
+        got = orc.parallel_compress(data, e["level"], e["format"], e["block"], threads=4)
+        assert len(got) == e["size"] and hashlib.sha256(got).hexdigest() == e["sha256"], e
+
+
+def test_golden_cfg1(fixtures):
+    e = fixtures["cfg1"]
+    data = corpus.generate(e["kind"], e["n"], e["seed"])
+    got = orc.parallel_compress(data, e["level"], e["format"], e["block"], threads=8)
+    assert len(got) == e["size"] and hashlib.sha256(got).hexdigest() == e["sha256"]
+
+
+def test_golden_tokens(fixtures):
+    for e in fixtures["tokens"]:
+        data = corpus.generate(e["kind"], e["n"], e["seed"])
+        t = orc.lz77(data, e["level"])
+        assert t.size == e["count"] and hashlib.sha256(t.astype("<u4").tobytes()).hexdigest() == e["sha256"], e
+
+
+def test_tokens_reconstruct_input():
+    for kind in corpus.GENERATORS:
+        data = corpus.generate(kind, 50_000, 8)
+        for level in (1, 2, 3, 4, 6, 9):
+            out = bytearray()
+            for t in orc.lz77(data, level):
+                t = int(t)
+                if t & 0x80000000:
+                    L, d = (t >> 16) & 0x1FF, t & 0xFFFF
+                    assert 3 <= L <= 258 and 1 <= d <= 32768
+                    for _ in range(L):
+                        out.append(out[len(out) - d])
+                else:
+                    out.append(t & 0xFF)
+            assert bytes(out) == data.tobytes(), (kind, level)
+
+
+def test_l6_class_ratio_vs_orot(fixtures):
+    """The L4-9 block encoder (the HIP L6-class design) stays within 3% of
+    orot's size and is always zlib-valid, also where orot's is not (F5)."""
+    for e in fixtures["levels_l6"]:
+        data = corpus.generate(e["kind"], e["n"], e["seed"])
+        got = orc.parallel_compress(data, e["level"], 0, 0, threads=4)
+        assert zlib.decompress(got, -15) == data.tobytes()
+        assert len(got) <= 1.03 * e["size"], (e, len(got))
+
+
+# ---- optimal length-limited Huffman lengths: brute force on tiny alphabets
+def _kraft(lens):
+    return sum(2.0 ** -int(l) for l in lens if l)
+
+
+def _best_cost(freq, max_bits):
+    act = [f for f in freq if f]
+    best = None
+    for ls in itertools.product(range(1, max_bits + 1), repeat=len(act)):
+        if sum(2.0 ** -l for l in ls) <= 1.0 + 1e-12:
+            c = sum(f * l for f, l in zip(act, ls))
+            best = c if best is None or c < best else best
+    return best
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_limited_lengths_optimal(seed):
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(2, 7))
+    freq = np.zeros(k + 2, dtype=np.uint32)
+    freq[rng.choice(k + 2, size=k, replace=False)] = rng.integers(1, 1000, size=k) ** int(rng.integers(1, 3))
+    max_bits = int(rng.integers(3, 6))
+    lens = orc.limited_lengths(freq, max_bits)
+    assert all((l == 0) == (f == 0) for l, f in zip(lens, freq))
+    assert max(lens) <= max_bits
+    assert abs(_kraft(lens) - 1.0) < 1e-12
+    assert sum(int(f) * int(l) for f, l in zip(freq, lens)) == _best_cost(list(freq), max_bits)
+
+
+def test_limited_lengths_large_alphabet_bounds():
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        freq = (rng.zipf(1.2, size=288) % 100000).astype(np.uint32)
+        lens = orc.limited_lengths(freq, 15)
+        assert max(lens) <= 15 and abs(_kraft(lens) - 1.0) < 1e-9
+        lens7 = orc.limited_lengths(freq[:19], 7)
+        assert max(lens7) <= 7 and abs(_kraft(lens7) - 1.0) < 1e-9
+
+
+def test_single_symbol_gets_length_one():
+    f = np.zeros(32, dtype=np.uint32)
+    f[5] = 9
+    assert list(orc.limited_lengths(f, 15)) == [0] * 5 + [1] + [0] * 26
+
+
+# ---- restatement == orot, compared directly where orot was built here
+@pytest.mark.skipif(not orc.ref_available(), reason="orot not built in this checkout (oracle/_ref)")
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+def test_oracle_equals_orot_build(kind):
+    data = corpus.generate(kind, 200_000, 21)
+    for level in range(1, 10):
+        assert np.array_equal(orc.lz77(data, level), orc.ref_lz77(data, level)), level
+    for level in (0, 1, 2, 3):
+        for fmt in (0, 1, 2):
+            assert orc.parallel_compress(data, level, fmt) == orc.ref_parallel_compress(data, level, fmt)
