@@ -40,6 +40,22 @@ size_t default_block_size(int level) {  // src/parallel/parallel_compressor.cpp:
 
 int clamp_level(int level) { return level < 0 ? 0 : (level > 12 ? 12 : level); }
 
+// Result of the start-up lane-order check of LDS exchanges: -1 not run yet,
+// 0 failed (the L1 kernels use their ballot variant), 1 passed.
+std::mutex g_exch_mu;
+int g_exch_state = -1;
+
+bool exchange_order_ok(hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(g_exch_mu);
+    if (g_exch_state < 0) {
+        bool ok = false;
+        hipError_t e = orot_gpu::check_exchange_order(stream, &ok);
+        if (e != hipSuccess) return false;
+        g_exch_state = ok ? 1 : 0;
+    }
+    return g_exch_state == 1;
+}
+
 template <class T>
 bool grow(T*& p, size_t& cap, size_t need, const char* what) {
     if (need <= cap && p) return true;
@@ -68,10 +84,8 @@ struct deflate_parallel_ctx {
     size_t in_cap = 0;
     uint8_t* d_out = nullptr;
     size_t out_cap = 0;
-    uint16_t* d_d16 = nullptr;
-    size_t d16_cap = 0;
-    uint16_t* d_m1 = nullptr;
-    size_t m1_cap = 0;
+    uint32_t* d_tok = nullptr;
+    size_t tok_cap = 0;
     uint32_t* d_stage = nullptr;
     size_t stage_cap = 0;
     ChunkInfo* d_info = nullptr;
@@ -82,12 +96,14 @@ struct deflate_parallel_ctx {
     hipEvent_t ev[8] = {};
     float stage_ms[4] = {};
     int nstage = 0;
+    bool exch_ok = false;
+    size_t lds_max = 65536;
     std::mutex mu;
 
     ~deflate_parallel_ctx() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
-        for (void* p : {(void*)d_in, (void*)d_out, (void*)d_d16, (void*)d_m1, (void*)d_stage, (void*)d_info,
+        for (void* p : {(void*)d_in, (void*)d_out, (void*)d_tok, (void*)d_stage, (void*)d_info,
                         (void*)d_res})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -120,7 +136,7 @@ struct deflate_parallel_ctx {
         }
         const uint32_t nb = (uint32_t)nb64;
         const uint64_t sw = orot_gpu::stage_words_for_chunk(chunk);
-        if (!grow(d_d16, d16_cap, n, "hipMalloc(d16)") || !grow(d_m1, m1_cap, n, "hipMalloc(m1)") ||
+        if (!grow(d_tok, tok_cap, (size_t)nb * ((chunk + 7u) & ~7u) + 8, "hipMalloc(tok)") ||
             !grow(d_stage, stage_cap, (size_t)(sw * nb), "hipMalloc(stage)") ||
             !grow(d_info, info_cap, nb, "hipMalloc(info)") || !grow(d_res, res_cap, 1, "hipMalloc(result)"))
             return 0;
@@ -142,13 +158,14 @@ struct deflate_parallel_ctx {
         p.format = format;
         p.d_out = out;
         p.cap = cap;
-        p.d_d16 = d_d16;
-        p.d_m1 = d_m1;
+        p.d_tok = d_tok;
         p.d_stage = d_stage;
         p.stage_words = sw;
         p.d_info = d_info;
         p.d_result = d_res;
         p.stream = stream;
+        p.exch_ok = exch_ok;
+        p.lds_max = lds_max;
         for (int i = 0; i < 8; ++i) p.ev[i] = timing ? ev[i] : nullptr;
         hipError_t e = orot_gpu::launch_parallel_compress(p);
         if (e != hipSuccess) {
@@ -204,6 +221,13 @@ deflate_parallel_ctx* deflate_parallel_new(int level, deflate_format format, int
         ctx->stream = nullptr;
         delete ctx;
         return nullptr;
+    }
+    ctx->exch_ok = exchange_order_ok(ctx->stream);
+    {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess && v > 0)
+            ctx->lds_max = (size_t)v;
     }
     return ctx;
 }

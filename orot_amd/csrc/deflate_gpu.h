@@ -38,17 +38,21 @@ struct Plan {
     int format;             // 0 raw, 1 zlib, 2 gzip
     uint8_t* d_out;
     uint64_t cap;
-    uint16_t* d_d16;        // n entries
-    uint16_t* d_m1;         // n entries
+    uint32_t* d_tok;        // n entries: per-position token word
     uint32_t* d_stage;      // nchunks * stage_words
     uint64_t stage_words;   // per chunk
     ChunkInfo* d_info;      // nchunks
     Result* d_result;
     hipStream_t stream;
     hipEvent_t ev[8];       // optional timing events (nullptr = off)
+    bool exch_ok;           // LDS exchanges of one wave apply in lane order
+    size_t lds_max;         // largest dynamic LDS allocation per workgroup
 };
 
-// Largest chunk the L1-class kernel accepts for the given hash width.
+// One-time start-up check behind Plan::exch_ok.
+hipError_t check_exchange_order(hipStream_t stream, bool* ok);
+
+// Largest chunk the L1-class kernels accept for the given hash width.
 uint32_t max_chunk_for_hash_bits(int hash_bits);
 
 // Stage capacity (32-bit words) needed per chunk of `chunk` bytes.

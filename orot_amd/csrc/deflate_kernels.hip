@@ -208,26 +208,6 @@ __device__ uint32_t lcp_wave(const uint8_t* a, const uint8_t* b, uint32_t cap, i
     return cap;
 }
 
-// Greedy match length at a token start (orot match_find_fast, lz77.cpp:163-191
-// with the candidate precomputed in D).
-__device__ __forceinline__ uint32_t m1_lane(const uint8_t* s, uint32_t n, const uint16_t* D, uint32_t pos) {
-    if (pos + 4 > n) return 0;
-    const uint32_t d = D[pos];
-    if (!d) return 0;
-    const uint32_t cap = min(258u, n - pos);
-    const uint32_t l = lcp_lane(s + pos, s + pos - d, cap);
-    return l >= 3 ? l : 0;
-}
-
-__device__ __forceinline__ uint32_t m1_wave(const uint8_t* s, uint32_t n, const uint16_t* D, uint32_t pos, int lane) {
-    if (pos + 4 > n) return 0;
-    const uint32_t d = D[pos];
-    if (!d) return 0;
-    const uint32_t cap = min(258u, n - pos);
-    const uint32_t l = lcp_wave(s + pos, s + pos - d, cap, lane);
-    return l >= 3 ? l : 0;
-}
-
 __device__ __forceinline__ void clear_bits(uint32_t* bm, uint32_t x, uint32_t y, int lane) {
     if (x >= y) return;
     const uint32_t w0 = x >> 5, w1 = (y - 1) >> 5;
@@ -323,12 +303,28 @@ struct L1Cfg {
 };
 
 constexpr uint32_t L1_BM_WORDS = MAX_CHUNK_L1 / 32;
-constexpr uint32_t L1_PHASE_B_BYTES = (L1_BM_WORDS + L1_THREADS + 16) * 4;
+constexpr uint32_t L1_PHASE_B_BYTES = (L1_BM_WORDS + 2 * L1_THREADS + 16) * 4 + L1_THREADS;
+
+// Per-position token word: literal byte | match length << 8 | distance << 17.
+__device__ __forceinline__ uint32_t tok_len(uint32_t t) { return (t >> 8) & 0x1FFu; }
+__device__ __forceinline__ uint32_t tok_dist(uint32_t t) { return t >> 17; }
+__device__ __forceinline__ uint32_t tok_bits(uint32_t t) {
+    const uint32_t l = tok_len(t);
+    return l ? match_bits(l, tok_dist(t)) : lit_bits(t & 0xFFu);
+}
+
+// Byte j (0..15) of a 16-byte window held as two little-endian u64.
+__device__ __forceinline__ uint32_t win_byte(uint64_t w0, uint64_t w1, uint32_t j) {
+    return (uint32_t)((j < 8 ? (w0 >> (8 * j)) : (w1 >> (8 * (j - 8)))) & 0xFFu);
+}
+__device__ __forceinline__ uint64_t win_u64(uint64_t w0, uint64_t w1, uint32_t j) {  // bytes j..j+7, j < 8
+    return j == 0 ? w0 : ((w0 >> (8 * j)) | (w1 << (64 - 8 * j)));
+}
 
 template <int HB>
 __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
                                                              uint32_t chunk, uint32_t nchunks,
-                                                             uint16_t* __restrict__ d16, uint16_t* __restrict__ m1g,
+                                                             uint32_t* __restrict__ tokw,
                                                              uint32_t* __restrict__ stage, uint64_t stage_words,
                                                              ChunkInfo* __restrict__ info) {
     extern __shared__ uint32_t smem[];
@@ -338,12 +334,13 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
     const uint64_t base = (uint64_t)c * chunk;
     const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
     const uint8_t* s = src + base;
-    uint16_t* D = d16 + base;
-    uint16_t* M = m1g + base;
+    uint32_t* T = tokw + (uint64_t)c * ((chunk + 7u) & ~7u);  // 32-byte aligned per chunk
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
     // ---- Phase A1: latest earlier position with the same hash, within each
-    // sub-range.  D holds (local index + 1) of that position, 0 if none.
+    // sub-range (orot's single-probe head table, every position inserted:
+    // lz77.cpp:173-176, 400-403).  T[p] = (local index + 1) of that
+    // position, 0 if none.  The next step's input word is loaded ahead.
     const uint32_t nh = n >= 4 ? n - 3 : 0;  // positions p with p + 4 <= n are hashed
     const uint32_t SR = (((nh + W - 1) / W) + 63u) & ~63u;
     uint16_t* tabs = reinterpret_cast<uint16_t*>(smem);
@@ -352,16 +349,19 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
         for (uint32_t i = lane; i < TAB / 2; i += 64) reinterpret_cast<uint32_t*>(tab)[i] = 0;
         const uint32_t a = (uint32_t)wv * SR;
         const uint32_t b = min(a + SR, nh);
+        uint32_t wnext = (a + (uint32_t)lane < b) ? ld32u(s + a + lane) : 0u;
         for (uint32_t p0 = a; p0 < b; p0 += 64) {
             const uint32_t p = p0 + (uint32_t)lane;
             const bool valid = p < b;
-            const uint32_t h = valid ? hash4<HB>(ld32u(s + p)) : 0u;
+            const uint32_t wcur = wnext;
+            wnext = (p + 64 < b) ? ld32u(s + p + 64) : 0u;
+            const uint32_t h = valid ? hash4<HB>(wcur) : 0u;
             uint64_t m = ballot(valid);
 #pragma unroll
             for (int k = 0; k < HB; ++k) {
-                const bool bit = (h >> k) & 1u;
-                const uint64_t bk = ballot(bit);
-                m &= bit ? bk : ~bk;
+                const uint32_t bit = (h >> k) & 1u;
+                const uint64_t bk = ballot(bit != 0);
+                m &= bk ^ ((uint64_t)bit - 1ull);  // bit ? bk : ~bk
             }
             if (valid) {
                 const uint64_t below = m & ((1ull << lane) - 1ull);
@@ -369,68 +369,191 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
                 if (below) res = p0 + (uint32_t)(63 - __builtin_clzll(below)) - a + 1u;
                 else res = tab[h];
                 if ((m >> lane) == 1ull) tab[h] = (uint16_t)(p - a + 1u);
-                D[p] = (uint16_t)res;
+                T[p] = res;
             }
         }
     }
     __syncthreads();
 
-    // ---- Phase A2: resolve first occurrences against earlier sub-ranges,
-    // then apply orot's candidate rules: C == 0 is "empty", distance masked
-    // to 15 bits, first byte compared at C (lz77.cpp:178-184).
-    for (uint32_t p = tid; p < n; p += L1_THREADS) {
-        uint32_t out = 0;
-        if (p < nh) {
-            const uint32_t w = p / SR;
-            const uint32_t t = D[p];
-            uint32_t C = 0;
-            bool have = false;
-            if (t) {
-                C = w * SR + t - 1u;
-                have = true;
+    // ---- Phase A2: per position, the final candidate (first occurrences
+    // resolved against earlier sub-ranges; C == 0 is "empty", distance masked
+    // to 15 bits, first byte compared at C: lz77.cpp:178-184) and the greedy
+    // match length M1 (capped at 258 and the chunk end; < 3 means none).
+    // Each thread walks a contiguous run in batches of 8 positions; along a
+    // diagonal (same distance) the verified extent is carried, so long runs
+    // cost O(1) per position.
+    {
+        const uint32_t R = (((n + L1_THREADS - 1) / L1_THREADS) + 7u) & ~7u;
+        const uint32_t r0 = min(n, (uint32_t)tid * R), r1 = min(n, r0 + R);
+        const uint32_t lim = min(n, r1 + 257u);
+        uint32_t cd = 0, ce = 0;
+        bool cx = false;
+        for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
+            uint32_t loc[8];
+            if (p0 + 8 <= nh) {
+                const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
+                const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
+                loc[0] = v0.x; loc[1] = v0.y; loc[2] = v0.z; loc[3] = v0.w;
+                loc[4] = v1.x; loc[5] = v1.y; loc[6] = v1.z; loc[7] = v1.w;
             } else {
-                const uint32_t h = hash4<HB>(ld32u(s + p));
-                for (int w2 = (int)w - 1; w2 >= 0; --w2) {
-                    const uint32_t u = tabs[(uint32_t)w2 * TAB + h];
-                    if (u) {
-                        C = (uint32_t)w2 * SR + u - 1u;
-                        have = true;
-                        break;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) loc[k] = (p0 + k < nh) ? T[p0 + k] : 0u;
+            }
+            uint64_t w0, w1;
+            if (p0 + 16 <= n) {
+                w0 = ld64u(s + p0);
+                w1 = ld64u(s + p0 + 8);
+            } else {
+                w0 = w1 = 0;
+                for (uint32_t j = 0; j < 16 && p0 + j < n; ++j) {
+                    const uint64_t bv = s[p0 + j];
+                    if (j < 8) w0 |= bv << (8 * j);
+                    else w1 |= bv << (8 * (j - 8));
+                }
+            }
+            uint32_t dd[8];
+            uint64_t cw[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + k;
+                dd[k] = 0;
+                cw[k] = 0;
+                if (p >= r1 || p >= nh) continue;
+                const uint32_t w = p / SR;
+                uint32_t C = 0;
+                bool have = false;
+                if (loc[k]) {
+                    C = w * SR + loc[k] - 1u;
+                    have = true;
+                } else {
+                    const uint32_t word = (uint32_t)(win_u64(w0, w1, k) & 0xFFFFFFFFull);
+                    const uint32_t h = hash4<HB>(word);
+                    for (int w2 = (int)w - 1; w2 >= 0; --w2) {
+                        const uint32_t u = tabs[(uint32_t)w2 * TAB + h];
+                        if (u) {
+                            C = (uint32_t)w2 * SR + u - 1u;
+                            have = true;
+                            break;
+                        }
+                    }
+                }
+                if (have && C != 0) {
+                    const uint32_t d = (p - C) & 0x7FFFu;
+                    if (d) {
+                        const uint32_t first = (p - C == d) ? 0xFFFFFFFFu : s[C];
+                        if (first == 0xFFFFFFFFu || first == win_byte(w0, w1, k)) {
+                            dd[k] = d;
+                            if (p + 8 <= n) cw[k] = ld64u(s + p - d);
+                        }
                     }
                 }
             }
-            if (have && C != 0) {
-                const uint32_t d = (p - C) & 0x7FFFu;
-                if (d && s[C] == s[p]) out = d;
+            uint32_t outw[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + k;
+                const uint32_t lit = p < n ? win_byte(w0, w1, k) : 0u;
+                uint32_t m1 = 0;
+                const uint32_t d = dd[k];
+                if (d) {
+                    const uint32_t cap = min(258u, n - p);
+                    uint32_t L;
+                    if (d == cd && p <= ce && (cx || ce >= p + cap)) {
+                        L = ce - p;
+                    } else {
+                        // fresh comparison on diagonal d
+                        uint32_t e;
+                        bool x;
+                        if (p + 8 <= n) {
+                            const uint64_t diff = win_u64(w0, w1, k) ^ cw[k];
+                            if (diff) {
+                                e = p + (uint32_t)(__builtin_ctzll(diff) >> 3);
+                                x = true;
+                            } else {
+                                e = p + 8;
+                                x = e >= n;
+                            }
+                        } else {
+                            e = p;
+                            while (e < n && s[e] == s[e - d]) ++e;
+                            x = true;
+                        }
+                        // extend 64 bytes per round up to what the rest of the run can use
+                        while (!x && e < lim) {
+                            if (e + 64 <= n) {
+                                uint64_t a8[8], b8[8];
+#pragma unroll
+                                for (int q = 0; q < 8; ++q) {
+                                    a8[q] = ld64u(s + e + 8 * q);
+                                    b8[q] = ld64u(s + e - d + 8 * q);
+                                }
+                                uint32_t adv = 64;
+#pragma unroll
+                                for (int q = 7; q >= 0; --q) {
+                                    const uint64_t df = a8[q] ^ b8[q];
+                                    if (df) adv = 8u * q + (uint32_t)(__builtin_ctzll(df) >> 3);
+                                }
+                                e += adv;
+                                if (adv < 64) x = true;
+                            } else {
+                                while (e < n && s[e] == s[e - d]) ++e;
+                                x = true;
+                            }
+                        }
+                        cd = d;
+                        ce = e;
+                        cx = x;
+                        L = e - p;
+                    }
+                    L = min(L, cap);
+                    m1 = L >= 3 ? L : 0u;
+                }
+                outw[k] = lit | (m1 << 8) | ((m1 ? d : 0u) << 17);
+            }
+            if (p0 + 8 <= r1) {
+                *reinterpret_cast<uint4*>(T + p0) = make_uint4(outw[0], outw[1], outw[2], outw[3]);
+                *reinterpret_cast<uint4*>(T + p0 + 4) = make_uint4(outw[4], outw[5], outw[6], outw[7]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (p0 + k < r1) T[p0 + k] = outw[k];
             }
         }
-        D[p] = (uint16_t)out;
     }
     __syncthreads();
 
-    // ---- Phase B: speculative greedy parse per lane segment.
+    // ---- Phase B: greedy parse (lz77.cpp:378-406) of each lane's segment,
+    // speculatively entered at its start; fixed-Huffman bits counted on the way.
     uint32_t* bm = smem;
     uint32_t* exit_spec = smem + L1_BM_WORDS;
-    uint32_t* scan = exit_spec + L1_THREADS;
+    uint32_t* segbits = exit_spec + L1_THREADS;
+    uint32_t* scan = segbits + L1_THREADS;
+    uint8_t* touched = reinterpret_cast<uint8_t*>(scan + 16);
     const uint32_t S = max(32u, (((n + L1_THREADS - 1) / L1_THREADS) + 31u) & ~31u);
     const uint32_t NS = (n + S - 1) / S;
     const uint32_t nbw = (n + 31) >> 5;
     for (uint32_t i = tid; i < nbw; i += L1_THREADS) bm[i] = 0;
+    touched[tid] = 0;
     __syncthreads();
     if ((uint32_t)tid < NS) {
         uint32_t pos = (uint32_t)tid * S;
         const uint32_t end = min(pos + S, n);
+        uint32_t bits = 0;
         while (pos < end) {
             bm[pos >> 5] |= 1u << (pos & 31);
-            const uint32_t l = m1_lane(s, n, D, pos);
-            M[pos] = (uint16_t)l;
+            const uint32_t t = T[pos];
+            bits += tok_bits(t);
+            const uint32_t l = tok_len(t);
             pos += l ? l : 1u;
         }
         exit_spec[tid] = pos;
+        segbits[tid] = bits;
     }
     __syncthreads();
 
-    // ---- Serial resolve of segment entries (wave 0, lockstep).
+    // ---- Serial resolve of segment entries (wave 0, lockstep): a segment
+    // entered past its start is re-parsed until it meets a position of its
+    // speculative parse (from there on both parses agree).
     if (wv == 0 && NS > 1) {
         uint32_t e = exit_spec[0];
         uint32_t si = 1;
@@ -450,6 +573,7 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
                 si = jj + 1;
                 continue;
             }
+            if (lane == 0) touched[si] = 1;
             const uint32_t st = si * S, en = min(st + S, n);
             if (e >= en) {
                 clear_bits(bm, st, en, lane);
@@ -464,8 +588,7 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
                     break;
                 }
                 if (lane == 0) bm[pos >> 5] |= 1u << (pos & 31);
-                const uint32_t l = m1_wave(s, n, D, pos, lane);
-                if (lane == 0) M[pos] = (uint16_t)l;
+                const uint32_t l = tok_len(T[pos]);
                 const uint32_t np = pos + (l ? l : 1u);
                 clear_bits(bm, pos + 1, min(np, en), lane);
                 if (np >= en) {
@@ -480,18 +603,32 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
     }
     __syncthreads();
 
-    // ---- Phase C: bit counts, scan, block choice, emission.
+    // ---- Phase C: bit counts (re-counted only where the resolve changed the
+    // parse), block-wide exclusive scan, stored / fixed choice
+    // (deflate_block.cpp:414-421), emission.
     const uint32_t st = (uint32_t)tid * S;
     const uint32_t en = (uint32_t)tid < NS ? min(st + S, n) : st;
-    uint32_t mybits = 0;
-    for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) {
-        uint32_t x = bm[w];
-        while (x) {
-            const uint32_t p = (w << 5) + (uint32_t)__builtin_ctz(x);
-            x &= x - 1;
-            const uint32_t l = M[p];
-            mybits += l ? match_bits(l, D[p]) : lit_bits(s[p]);
+    uint32_t mybits = (uint32_t)tid < NS ? segbits[tid] : 0u;
+    if ((uint32_t)tid < NS && touched[tid]) {
+        mybits = 0;
+        uint32_t pk[8];
+        int cnt = 0;
+        for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) {
+            uint32_t x = bm[w];
+            while (x) {
+                pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
+                x &= x - 1;
+                if (cnt == 8) {
+                    uint32_t tv[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) mybits += tok_bits(tv[k]);
+                    cnt = 0;
+                }
+            }
         }
+        for (int k = 0; k < cnt; ++k) mybits += tok_bits(T[pk[k]]);
     }
     const uint32_t incl = wave_incl_scan(mybits, lane);
     if (lane == 63) scan[wv] = incl;
@@ -527,32 +664,946 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
         uint32_t nacc = bitpos & 31u;
         bool partial = nacc != 0;
         uint64_t acc = 0;
+        uint32_t pk[8];
+        int cnt = 0;
+        auto put = [&](uint32_t t) {
+            uint32_t v, len;
+            const uint32_t l = tok_len(t);
+            if (l) fixed_match(l, tok_dist(t), v, len);
+            else fixed_lit(t & 0xFFu, v, len);
+            acc |= (uint64_t)v << nacc;
+            nacc += len;
+            if (nacc >= 32) {
+                if (partial) {
+                    atomicOr(&stg[wi], (uint32_t)acc);
+                    partial = false;
+                } else {
+                    stg[wi] = (uint32_t)acc;
+                }
+                ++wi;
+                acc >>= 32;
+                nacc -= 32;
+            }
+        };
         for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) {
             uint32_t x = bm[w];
             while (x) {
-                const uint32_t p = (w << 5) + (uint32_t)__builtin_ctz(x);
+                pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
                 x &= x - 1;
-                const uint32_t l = M[p];
-                uint32_t v, len;
-                if (l) fixed_match(l, D[p], v, len);
-                else fixed_lit(s[p], v, len);
-                acc |= (uint64_t)v << nacc;
-                nacc += len;
-                if (nacc >= 32) {
-                    if (partial) {
-                        atomicOr(&stg[wi], (uint32_t)acc);
-                        partial = false;
-                    } else {
-                        stg[wi] = (uint32_t)acc;
-                    }
-                    ++wi;
-                    acc >>= 32;
-                    nacc -= 32;
+                if (cnt == 8) {
+                    uint32_t tv[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
+                    for (int k = 0; k < 8; ++k) put(tv[k]);
+                    cnt = 0;
                 }
             }
         }
+        for (int k = 0; k < cnt; ++k) put(T[pk[k]]);
         if (nacc) atomicOr(&stg[wi], (uint32_t)acc);
     }
+}
+
+// ---------------------------------------------------------------------------
+// L1-class kernel for chunks up to 128 KiB: one 1024-thread workgroup per
+// chunk, the chunk's match lengths resident in LDS.
+//   A1  latest earlier same-hash position per sub-range (8 sub-ranges for
+//       12-bit hashes, 2 for 14-bit), 8 positions per lane in flight; one LDS
+//       exchange per position (EXCH: same-address exchanges of one wave
+//       instruction take effect in lane order, checked at start-up by
+//       exchange_order_kernel) or a ballot match-any.
+//   A2a final candidate distance per position (earlier sub-ranges, aliased
+//       first-byte test).
+//   A2b match length per position with the verified extent carried along a
+//       diagonal; a u8 length code goes to LDS (0 literal, 1..253 = len - 2,
+//       254 = 258, 255 = 256/257 read back from the token word), the token
+//       word (literal | length << 8 | distance << 17) to HBM.
+//   B   speculative greedy parse of 1024 lane segments.
+//   R   serial resolve of segment entries on wave 0, all in LDS.
+//   C   bit counts, block-wide scan, stored / fixed choice, emission.
+// ---------------------------------------------------------------------------
+constexpr int LC_THREADS = 1024;
+constexpr int LC_WAVES = LC_THREADS / 64;
+constexpr uint32_t LC_MAX_CHUNK = 128u * 1024u;
+constexpr uint32_t LC_BM_WORDS = LC_MAX_CHUNK / 32;
+constexpr uint32_t LC_REGION0 = LC_MAX_CHUNK;  // bytes: tables during A1/A2a, length codes after
+constexpr uint32_t LC_LDS_BYTES = LC_REGION0 + (LC_BM_WORDS + LC_THREADS + 64) * 4;
+
+template <int HB>
+struct LCCfg {
+    static constexpr int W = HB == 12 ? 8 : 2;
+    static constexpr uint32_t TAB = 1u << HB;
+    static_assert(W * TAB * 4u <= LC_REGION0, "tables must fit region 0");
+};
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t len_code8(uint32_t m1) {
+    if (m1 == 0) return 0;
+    if (m1 == 258) return 254;
+    if (m1 >= 256) return 255;
+    return m1 - 2;
+}
+
+// parse step at pos from its length code (escape: exact length from HBM)
+__device__ __forceinline__ uint32_t step_at(const uint8_t* code, const uint32_t* T, uint32_t pos) {
+    const uint32_t b = code[pos];
+    if (b == 0) return 1;
+    if (b < 254) return b + 2;
+    if (b == 254) return 258;
+    return tok_len(T[pos]);
+}
+
+template <int HB, bool EXCH>
+__global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
+                                                              uint32_t chunk, uint32_t nchunks,
+                                                              uint32_t* __restrict__ tokw,
+                                                              uint32_t* __restrict__ stage, uint64_t stage_words,
+                                                              ChunkInfo* __restrict__ info) {
+    extern __shared__ uint32_t smem[];
+    constexpr int W = LCCfg<HB>::W;
+    constexpr uint32_t TAB = LCCfg<HB>::TAB;
+    const uint32_t c = blockIdx.x;
+    const uint64_t base = (uint64_t)c * chunk;
+    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
+    const uint8_t* s = src + base;
+    uint32_t* T = tokw + (uint64_t)c * ((chunk + 7u) & ~7u);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint32_t* tabs = smem;
+    uint8_t* code = reinterpret_cast<uint8_t*>(smem);
+    uint32_t* bm = smem + LC_REGION0 / 4;
+    uint32_t* exits = bm + LC_BM_WORDS;
+    uint32_t* misc = exits + LC_THREADS;
+
+    // ---- A1
+    const uint32_t nh = n >= 4 ? n - 3 : 0;
+    const uint32_t SR = (((nh + W - 1) / W) + 63u) & ~63u;
+    if (wv < W) {
+        uint32_t* tab = tabs + (uint32_t)wv * TAB;
+        for (uint32_t i = lane; i < TAB; i += 64) tab[i] = 0;
+        const uint32_t a = (uint32_t)wv * SR;
+        const uint32_t b = min(a + SR, nh);
+        for (uint32_t p0 = a; p0 < b; p0 += 512) {
+            uint32_t wd[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wd[j] = ld32u(s + min(p0 + 64u * j + (uint32_t)lane, b - 1));
+            uint32_t res[8];
+            if (EXCH) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                    res[j] = p < b ? atomicExch(&tab[hash4<HB>(wd[j])], p + 1u) : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t q0 = p0 + 64u * j;
+                    const uint32_t p = q0 + (uint32_t)lane;
+                    const bool valid = p < b;
+                    const uint32_t h = hash4<HB>(wd[j]);
+                    uint64_t m = ballot(valid);
+#pragma unroll
+                    for (int k = 0; k < HB; ++k) {
+                        const uint32_t bit = (h >> k) & 1u;
+                        const uint64_t bk = ballot(bit != 0);
+                        m &= bk ^ ((uint64_t)bit - 1ull);
+                    }
+                    res[j] = 0;
+                    if (valid) {
+                        const uint64_t below = m & ((1ull << lane) - 1ull);
+                        if (below) res[j] = q0 + (uint32_t)(63 - __builtin_clzll(below)) + 1u;
+                        else res[j] = tab[h];
+                        if ((m >> lane) == 1ull) tab[h] = p + 1u;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                if (p < b) T[p] = res[j];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- A2a: final candidate distance; the first-byte test of a
+    // non-aliased candidate (C = p - d) is part of the match length.
+    for (uint32_t p0 = (uint32_t)tid * 8u; p0 < nh; p0 += LC_THREADS * 8u) {
+        uint32_t loc[8];
+        uint64_t w0 = 0, w1 = 0;
+        if (p0 + 8 <= nh) {
+            const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
+            const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
+            loc[0] = v0.x; loc[1] = v0.y; loc[2] = v0.z; loc[3] = v0.w;
+            loc[4] = v1.x; loc[5] = v1.y; loc[6] = v1.z; loc[7] = v1.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) loc[k] = (p0 + k < nh) ? T[p0 + k] : 0u;
+        }
+        if (p0 + 16 <= n) {
+            w0 = ld64u(s + p0);
+            w1 = ld64u(s + p0 + 8);
+        } else {
+            for (uint32_t j = 0; j < 16 && p0 + j < n; ++j) {
+                const uint64_t bv = s[p0 + j];
+                if (j < 8) w0 |= bv << (8 * j);
+                else w1 |= bv << (8 * (j - 8));
+            }
+        }
+        uint32_t outv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t p = p0 + k;
+            uint32_t C1 = loc[k];
+            outv[k] = 0;
+            if (p >= nh) continue;
+            if (C1 == 0) {
+                const uint32_t w = p / SR;
+                if (w > 0) {
+                    const uint32_t h = hash4<HB>((uint32_t)(win_u64(w0, w1, k) & 0xFFFFFFFFull));
+                    for (int w2 = (int)w - 1; w2 >= 0; --w2) {
+                        const uint32_t u = tabs[(uint32_t)w2 * TAB + h];
+                        if (u) {
+                            C1 = u;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (C1 > 1) {
+                const uint32_t C = C1 - 1u;
+                const uint32_t d = (p - C) & 0x7FFFu;
+                if (d && (p - C == d || s[C] == win_byte(w0, w1, k))) outv[k] = d;
+            }
+        }
+        if (p0 + 8 <= nh) {
+            *reinterpret_cast<uint4*>(T + p0) = make_uint4(outv[0], outv[1], outv[2], outv[3]);
+            *reinterpret_cast<uint4*>(T + p0 + 4) = make_uint4(outv[4], outv[5], outv[6], outv[7]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (p0 + k < nh) T[p0 + k] = outv[k];
+        }
+    }
+    __syncthreads();
+
+    // ---- A2b: match lengths (orot match_find_fast + match_len,
+    // lz77.cpp:163-191) over the thread's run of positions.
+    const uint32_t S = max(32u, (((n + LC_THREADS - 1) / LC_THREADS) + 31u) & ~31u);
+    {
+        const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
+        const uint32_t lim = min(n, r1 + 257u);
+        uint32_t cd = 0, ce = 0;
+        bool cx = false;
+        for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
+            uint32_t dv[8];
+            if (p0 + 8 <= nh) {
+                const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
+                const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
+                dv[0] = v0.x; dv[1] = v0.y; dv[2] = v0.z; dv[3] = v0.w;
+                dv[4] = v1.x; dv[5] = v1.y; dv[6] = v1.z; dv[7] = v1.w;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) dv[k] = (p0 + k < nh) ? T[p0 + k] : 0u;
+            }
+            uint64_t w0 = 0, w1 = 0;
+            if (p0 + 16 <= n) {
+                w0 = ld64u(s + p0);
+                w1 = ld64u(s + p0 + 8);
+            } else {
+                for (uint32_t j = 0; j < 16 && p0 + j < n; ++j) {
+                    const uint64_t bv = s[p0 + j];
+                    if (j < 8) w0 |= bv << (8 * j);
+                    else w1 |= bv << (8 * (j - 8));
+                }
+            }
+            uint64_t cw[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) cw[k] = (dv[k] && p0 + k + 8 <= n) ? ld64u(s + p0 + k - dv[k]) : 0ull;
+            uint32_t outw[8];
+            uint32_t cod[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + k;
+                const uint32_t lit = win_byte(w0, w1, k);
+                const uint32_t d = dv[k];
+                uint32_t m1 = 0;
+                if (d && p < r1) {
+                    const uint32_t cap = min(258u, n - p);
+                    uint32_t L;
+                    if (d == cd && p <= ce && (cx || ce >= p + cap)) {
+                        L = ce - p;
+                    } else {
+                        uint32_t e;
+                        bool x;
+                        if (p + 8 <= n) {
+                            const uint64_t diff = win_u64(w0, w1, k) ^ cw[k];
+                            if (diff) {
+                                e = p + (uint32_t)(__builtin_ctzll(diff) >> 3);
+                                x = true;
+                            } else {
+                                e = p + 8;
+                                x = e >= n;
+                            }
+                        } else {
+                            e = p;
+                            while (e < n && s[e] == s[e - d]) ++e;
+                            x = true;
+                        }
+                        while (!x && e < lim) {
+                            if (e + 64 <= n) {
+                                uint64_t a8[8], b8[8];
+#pragma unroll
+                                for (int q = 0; q < 8; ++q) {
+                                    a8[q] = ld64u(s + e + 8 * q);
+                                    b8[q] = ld64u(s + e - d + 8 * q);
+                                }
+                                uint32_t adv = 64;
+#pragma unroll
+                                for (int q = 7; q >= 0; --q) {
+                                    const uint64_t df = a8[q] ^ b8[q];
+                                    if (df) adv = 8u * q + (uint32_t)(__builtin_ctzll(df) >> 3);
+                                }
+                                e += adv;
+                                if (adv < 64) x = true;
+                            } else {
+                                while (e < n && s[e] == s[e - d]) ++e;
+                                x = true;
+                            }
+                        }
+                        cd = d;
+                        ce = e;
+                        cx = x;
+                        L = e - p;
+                    }
+                    L = min(L, cap);
+                    m1 = L >= 3 ? L : 0u;
+                }
+                outw[k] = lit | (m1 << 8) | ((m1 ? d : 0u) << 17);
+                cod[k] = len_code8(m1);
+            }
+            if (p0 + 8 <= r1) {
+                *reinterpret_cast<uint4*>(T + p0) = make_uint4(outw[0], outw[1], outw[2], outw[3]);
+                *reinterpret_cast<uint4*>(T + p0 + 4) = make_uint4(outw[4], outw[5], outw[6], outw[7]);
+                reinterpret_cast<uint32_t*>(code)[p0 >> 2] = cod[0] | (cod[1] << 8) | (cod[2] << 16) | (cod[3] << 24);
+                reinterpret_cast<uint32_t*>(code)[(p0 >> 2) + 1] = cod[4] | (cod[5] << 8) | (cod[6] << 16) | (cod[7] << 24);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (p0 + k < r1) {
+                        T[p0 + k] = outw[k];
+                        code[p0 + k] = (uint8_t)cod[k];
+                    }
+            }
+        }
+    }
+    // bitmap cleared by the owners of its words (segments are 32-aligned)
+    {
+        const uint32_t st = (uint32_t)tid * S, en = min(st + S, n);
+        for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) bm[w] = 0;
+    }
+    __syncthreads();
+
+    // ---- B: speculative greedy parse (lz77.cpp:378-406) of each segment.
+    const uint32_t NS = (n + S - 1) / S;
+    if ((uint32_t)tid < NS) {
+        uint32_t pos = (uint32_t)tid * S;
+        const uint32_t end = min(pos + S, n);
+        while (pos < end) {
+            bm[pos >> 5] |= 1u << (pos & 31);
+            pos += step_at(code, T, pos);
+        }
+        exits[tid] = pos;
+    }
+    __syncthreads();
+
+    // ---- R: serial resolve of the segment entries (wave 0).
+    if (wv == 0 && NS > 1) {
+        uint32_t e = exits[0];
+        uint32_t si = 1;
+        while (si < NS) {
+            if (e == si * S) {
+                const uint32_t j = si + (uint32_t)lane;
+                const bool mm = (j >= NS) || (exits[j] != (j + 1) * S);
+                const uint64_t bmask = ballot(mm);
+                if (bmask == 0) {
+                    si += 64;
+                    e = si * S;
+                    continue;
+                }
+                const uint32_t jj = si + (uint32_t)__builtin_ctzll(bmask);
+                if (jj >= NS) break;
+                e = exits[jj];
+                si = jj + 1;
+                continue;
+            }
+            const uint32_t st = si * S, en = min(st + S, n);
+            if (e >= en) {
+                clear_bits(bm, st, en, lane);
+                ++si;
+                continue;
+            }
+            clear_bits(bm, st, e, lane);
+            uint32_t pos = e, ne;
+            for (;;) {
+                if (bm[pos >> 5] & (1u << (pos & 31))) {
+                    ne = exits[si];
+                    break;
+                }
+                if (lane == 0) bm[pos >> 5] |= 1u << (pos & 31);
+                const uint32_t np = pos + step_at(code, T, pos);
+                clear_bits(bm, pos + 1, min(np, en), lane);
+                if (np >= en) {
+                    ne = np;
+                    break;
+                }
+                pos = np;
+            }
+            e = ne;
+            ++si;
+        }
+    }
+    __syncthreads();
+
+    // ---- C: bit counts, scan, block choice (deflate_block.cpp:414-421), emission.
+    const uint32_t st = (uint32_t)tid * S;
+    const uint32_t en = (uint32_t)tid < NS ? min(st + S, n) : st;
+    uint32_t mybits = 0;
+    {
+        uint32_t pk[8];
+        int cnt = 0;
+        for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) {
+            uint32_t x = bm[w];
+            while (x) {
+                pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
+                x &= x - 1;
+                if (cnt == 8) {
+                    uint32_t tv[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) mybits += tok_bits(tv[k]);
+                    cnt = 0;
+                }
+            }
+        }
+        for (int k = 0; k < cnt; ++k) mybits += tok_bits(T[pk[k]]);
+    }
+    const uint32_t incl = wave_incl_scan(mybits, lane);
+    if (lane == 63) misc[wv] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < LC_WAVES; ++i) {
+            const uint32_t t = misc[i];
+            misc[i] = acc;
+            acc += t;
+        }
+        misc[LC_WAVES] = acc;
+    }
+    __syncthreads();
+    const uint32_t excl = misc[wv] + incl - mybits;
+    const uint64_t fixed = 3ull + misc[LC_WAVES] + 7ull;
+    const bool is_last = c == nchunks - 1;
+    const bool stored = (uint64_t)(n + 5) * 8ull <= fixed;
+    if (tid == 0) {
+        info[c].fixed_bits = fixed;
+        info[c].raw_len = n;
+        info[c].is_stored = stored ? 1u : 0u;
+    }
+    if (stored) return;
+    uint32_t* stg = stage + (uint64_t)c * stage_words;
+    const uint32_t nw = (uint32_t)((fixed + 31) >> 5) + 1u;
+    for (uint32_t i = tid; i < nw; i += LC_THREADS) stg[i] = 0;
+    __syncthreads();
+    if (tid == 0) atomicOr(&stg[0], (is_last ? 1u : 0u) | 2u);  // BFINAL, BTYPE = 01
+    if (mybits) {
+        const uint32_t bitpos = 3u + excl;
+        uint32_t wi = bitpos >> 5;
+        uint32_t nacc = bitpos & 31u;
+        bool partial = nacc != 0;
+        uint64_t acc = 0;
+        auto put = [&](uint32_t tkn) {
+            uint32_t v, len;
+            const uint32_t l = tok_len(tkn);
+            if (l) fixed_match(l, tok_dist(tkn), v, len);
+            else fixed_lit(tkn & 0xFFu, v, len);
+            acc |= (uint64_t)v << nacc;
+            nacc += len;
+            if (nacc >= 32) {
+                if (partial) {
+                    atomicOr(&stg[wi], (uint32_t)acc);
+                    partial = false;
+                } else {
+                    stg[wi] = (uint32_t)acc;
+                }
+                ++wi;
+                acc >>= 32;
+                nacc -= 32;
+            }
+        };
+        uint32_t pk[8];
+        int cnt = 0;
+        for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) {
+            uint32_t x = bm[w];
+            while (x) {
+                pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
+                x &= x - 1;
+                if (cnt == 8) {
+                    uint32_t tv[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
+                    for (int k = 0; k < 8; ++k) put(tv[k]);
+                    cnt = 0;
+                }
+            }
+        }
+        for (int k = 0; k < cnt; ++k) put(T[pk[k]]);
+        if (nacc) atomicOr(&stg[wi], (uint32_t)acc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// L1-class kernel, 12-bit hashes, chunks up to 128 KiB (orot's default L1
+// chunk): one 1024-thread workgroup per chunk.
+//   0   the chunk's bytes are staged in LDS.
+//   A1  wave 0: latest earlier same-hash position, one LDS exchange per
+//       position (EXCH: same-address exchanges of one wave instruction take
+//       effect in lane order, checked at start-up by exchange_order_kernel)
+//       or a ballot match-any; T[p] = C + 1.
+//   A2  per position: candidate distance (C == 0 empty, 15-bit masked,
+//       first byte compared at C: lz77.cpp:178-184) and greedy match length
+//       (lz77.cpp:163-191), the verified extent carried along a diagonal,
+//       all compares in LDS; T[p] = literal | length << 8 | distance << 17.
+//   K   the LDS bytes are replaced by u8 length codes (0 literal,
+//       1..253 = len - 2, 254 = 258, 255 = 256/257 read back from T).
+//   P   parse barriers: q is a barrier when no token starting before q ends
+//       after q (max over p < q of p + step(p) == q); every greedy parse
+//       visits every barrier.  Lane j parses exactly from the first barrier
+//       of its 128-position range to the first barrier of a later range
+//       (lz77.cpp:378-406), counting fixed-Huffman bits.
+//   C   block-wide scan, stored / fixed choice (deflate_block.cpp:414-421),
+//       emission into the chunk's staging bitstream.
+// ---------------------------------------------------------------------------
+constexpr uint32_t LB_BM_OFF = LC_REGION0 / 4;               // words: table 0, then the token bitmap
+constexpr uint32_t LB_T1_OFF = LB_BM_OFF + LC_BM_WORDS;       // table 1 (two sub-ranges), then fb / misc
+constexpr uint32_t LB_FB_OFF = LB_T1_OFF;                     // first barrier per range
+constexpr uint32_t LB_MISC_OFF = LB_FB_OFF + LC_THREADS;
+template <int W>
+struct LBCfg {
+    static constexpr uint32_t LDS_BYTES = W == 2 ? (LB_T1_OFF + 4096u) * 4u : (LB_MISC_OFF + 64u) * 4u;
+};
+static_assert(LB_MISC_OFF + 64 <= LB_T1_OFF + 4096, "fb / misc must fit under table 1");
+
+__device__ __forceinline__ uint32_t lds_u32(const uint32_t* L, uint32_t off) {
+    const uint32_t i = off >> 2, k = off & 3u;
+    return __builtin_amdgcn_alignbyte(L[i + 1], L[i], k);
+}
+__device__ __forceinline__ uint64_t lds_u64(const uint32_t* L, uint32_t off) {
+    const uint32_t i = off >> 2, k = off & 3u;
+    const uint32_t w0 = L[i], w1 = L[i + 1], w2 = L[i + 2];
+    return (uint64_t)__builtin_amdgcn_alignbyte(w1, w0, k) | ((uint64_t)__builtin_amdgcn_alignbyte(w2, w1, k) << 32);
+}
+__device__ __forceinline__ uint32_t lds_u8(const uint32_t* L, uint32_t off) {
+    return (L[off >> 2] >> (8 * (off & 3u))) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t code_step(uint32_t b, const uint32_t* T, uint32_t pos) {
+    if (b == 0) return 1;
+    if (b < 254) return b + 2;
+    if (b == 254) return 258;
+    return tok_len(T[pos]);
+}
+
+// Fixed-Huffman bit count of the tokens whose start bit is set in
+// bm over [a, b); T read 8 tokens at a time.
+__device__ __forceinline__ uint32_t range_bits(const uint32_t* bm, const uint32_t* T, uint32_t a, uint32_t b) {
+    uint32_t bits = 0;
+    uint32_t pk[8];
+    int cnt = 0;
+    for (uint32_t w = a >> 5; w < ((b + 31) >> 5); ++w) {
+        uint32_t x = bm[w];
+        while (x) {
+            pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
+            x &= x - 1;
+            if (cnt == 8) {
+                uint32_t tv[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) bits += tok_bits(tv[k]);
+                cnt = 0;
+            }
+        }
+    }
+    for (int k = 0; k < cnt; ++k) bits += tok_bits(T[pk[k]]);
+    return bits;
+}
+
+template <int W, bool EXCH>
+__global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
+                                                                uint32_t chunk, uint32_t nchunks,
+                                                                uint32_t* __restrict__ tokw,
+                                                                uint32_t* __restrict__ stage, uint64_t stage_words,
+                                                                ChunkInfo* __restrict__ info) {
+    extern __shared__ uint32_t smem[];
+    constexpr int HB = 12;
+    constexpr uint32_t TAB = 1u << HB;
+    const uint32_t c = blockIdx.x;
+    const uint64_t base = (uint64_t)c * chunk;
+    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
+    const uint8_t* s = src + base;
+    uint32_t* T = tokw + (uint64_t)c * ((chunk + 7u) & ~7u);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    uint32_t* D = smem;                         // chunk bytes, then length codes
+    uint8_t* code = reinterpret_cast<uint8_t*>(smem);
+    uint32_t* bm = smem + LB_BM_OFF;
+    uint32_t* fb = smem + LB_FB_OFF;
+    uint32_t* misc = smem + LB_MISC_OFF;
+
+    // ---- 0: stage the chunk in LDS (plus zeroed padding words), clear tables
+    for (uint32_t i = (uint32_t)tid * 16u; i < n; i += LC_THREADS * 16u) {
+        if (i + 16 <= n) {
+            const uint64_t a = ld64u(s + i), b = ld64u(s + i + 8);
+            D[(i >> 2) + 0] = (uint32_t)a;
+            D[(i >> 2) + 1] = (uint32_t)(a >> 32);
+            D[(i >> 2) + 2] = (uint32_t)b;
+            D[(i >> 2) + 3] = (uint32_t)(b >> 32);
+        } else {
+            for (uint32_t w = i >> 2; w < ((n + 3) >> 2); ++w) {
+                uint32_t v = 0;
+                for (uint32_t k = 0; k < 4; ++k)
+                    if (4 * w + k < n) v |= (uint32_t)s[4 * w + k] << (8 * k);
+                D[w] = v;
+            }
+        }
+    }
+    for (uint32_t i = tid; i < W * TAB; i += LC_THREADS) smem[LB_BM_OFF + i] = 0;
+    __syncthreads();
+    if (tid < 4) D[((n + 3) >> 2) + tid] = 0;  // after the table clear: may overlap table 0
+    __syncthreads();
+
+    // ---- A1 (waves 0..W-1, one sub-range each): T[p] = C + 1 for the latest
+    // earlier position C of the sub-range with the same hash, 0 if none.
+    const uint32_t nh = n >= 4 ? n - 3 : 0;
+    const uint32_t SR = W == 1 ? nh : ((((nh + 1) / 2) + 63u) & ~63u);
+    if (wv < W) {
+        uint32_t* tab = smem + LB_BM_OFF + (uint32_t)wv * TAB;
+        const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
+        for (uint32_t p0 = a; p0 < b; p0 += 512) {
+            uint32_t wd[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wd[j] = lds_u32(D, min(p0 + 64u * j + (uint32_t)lane, b - 1));
+            uint32_t res[8];
+            if (EXCH) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                    res[j] = p < b ? atomicExch(&tab[hash4<HB>(wd[j])], p + 1u) : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t q0 = p0 + 64u * j;
+                    const uint32_t p = q0 + (uint32_t)lane;
+                    const bool valid = p < b;
+                    const uint32_t h = hash4<HB>(wd[j]);
+                    uint64_t m = ballot(valid);
+#pragma unroll
+                    for (int k = 0; k < HB; ++k) {
+                        const uint32_t bit = (h >> k) & 1u;
+                        const uint64_t bk = ballot(bit != 0);
+                        m &= bk ^ ((uint64_t)bit - 1ull);
+                    }
+                    res[j] = 0;
+                    if (valid) {
+                        const uint64_t below = m & ((1ull << lane) - 1ull);
+                        if (below) res[j] = q0 + (uint32_t)(63 - __builtin_clzll(below)) + 1u;
+                        else res[j] = tab[h];
+                        if ((m >> lane) == 1ull) tab[h] = p + 1u;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                if (p < b) T[p] = res[j];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- A2: candidate distance (first occurrences of sub-range 1 resolved
+    // against table 0; C == 0 empty, 15-bit masked, first byte compared at C:
+    // lz77.cpp:178-184) and greedy match length (lz77.cpp:163-191).
+    const uint32_t S = max(32u, (((n + LC_THREADS - 1) / LC_THREADS) + 31u) & ~31u);
+    const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
+    {
+        const uint32_t lim = min(n, r1 + 257u);
+        uint32_t cd = 0, ce = 0;
+        bool cx = false;
+        uint32_t nx[8];
+        auto load_loc = [&](uint32_t q0, uint32_t* dst) {
+            if (q0 + 8 <= nh) {
+                const uint4 v0 = *reinterpret_cast<const uint4*>(T + q0);
+                const uint4 v1 = *reinterpret_cast<const uint4*>(T + q0 + 4);
+                dst[0] = v0.x; dst[1] = v0.y; dst[2] = v0.z; dst[3] = v0.w;
+                dst[4] = v1.x; dst[5] = v1.y; dst[6] = v1.z; dst[7] = v1.w;
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) dst[k] = (q0 + k < nh) ? T[q0 + k] : 0u;
+            }
+        };
+        if (r0 < r1) load_loc(r0, nx);
+        for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
+            uint32_t loc[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) loc[k] = nx[k];
+            if (p0 + 8 < r1) load_loc(p0 + 8, nx);
+            const uint64_t w0 = lds_u64(D, p0), w1 = lds_u64(D, p0 + 8);
+            uint32_t outw[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + k;
+                const uint32_t lit = win_byte(w0, w1, k);
+                uint32_t C1 = loc[k];
+                if (W == 2 && C1 == 0 && p >= SR && p < nh)
+                    C1 = smem[LB_BM_OFF + hash4<HB>((uint32_t)(win_u64(w0, w1, k) & 0xFFFFFFFFull))];
+                uint32_t d = 0, m1 = 0;
+                if (p < r1 && C1 > 1) {
+                    const uint32_t C = C1 - 1u;
+                    const uint32_t dd = (p - C) & 0x7FFFu;
+                    if (dd && (p - C == dd || lds_u8(D, C) == lit)) d = dd;
+                }
+                if (d) {
+                    const uint32_t cap = min(258u, n - p);
+                    uint32_t L;
+                    if (d == cd && p <= ce && (cx || ce >= p + cap)) {
+                        L = ce - p;
+                    } else {
+                        uint32_t e;
+                        bool x;
+                        if (p + 8 <= n) {
+                            const uint64_t diff = win_u64(w0, w1, k) ^ lds_u64(D, p - d);
+                            if (diff) {
+                                e = p + (uint32_t)(__builtin_ctzll(diff) >> 3);
+                                x = true;
+                            } else {
+                                e = p + 8;
+                                x = e >= n;
+                            }
+                        } else {
+                            e = p;
+                            while (e < n && lds_u8(D, e) == lds_u8(D, e - d)) ++e;
+                            x = true;
+                        }
+                        while (!x && e < lim) {
+                            if (e + 8 <= n) {
+                                const uint64_t df = lds_u64(D, e) ^ lds_u64(D, e - d);
+                                if (df) {
+                                    e += (uint32_t)(__builtin_ctzll(df) >> 3);
+                                    x = true;
+                                } else {
+                                    e += 8;
+                                }
+                            } else {
+                                while (e < n && lds_u8(D, e) == lds_u8(D, e - d)) ++e;
+                                x = true;
+                            }
+                        }
+                        cd = d;
+                        ce = e;
+                        cx = x;
+                        L = e - p;
+                    }
+                    L = min(L, cap);
+                    m1 = L >= 3 ? L : 0u;
+                }
+                outw[k] = lit | (m1 << 8) | ((m1 ? d : 0u) << 17);
+            }
+            if (p0 + 8 <= r1) {
+                *reinterpret_cast<uint4*>(T + p0) = make_uint4(outw[0], outw[1], outw[2], outw[3]);
+                *reinterpret_cast<uint4*>(T + p0 + 4) = make_uint4(outw[4], outw[5], outw[6], outw[7]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (p0 + k < r1) T[p0 + k] = outw[k];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- K: length codes over the bytes; bitmap cleared.
+    for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
+        if (p0 + 8 <= r1) {
+            const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
+            const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
+            D[p0 >> 2] = len_code8(tok_len(v0.x)) | (len_code8(tok_len(v0.y)) << 8) |
+                         (len_code8(tok_len(v0.z)) << 16) | (len_code8(tok_len(v0.w)) << 24);
+            D[(p0 >> 2) + 1] = len_code8(tok_len(v1.x)) | (len_code8(tok_len(v1.y)) << 8) |
+                               (len_code8(tok_len(v1.z)) << 16) | (len_code8(tok_len(v1.w)) << 24);
+        } else {
+            for (uint32_t p = p0; p < r1; ++p) code[p] = (uint8_t)len_code8(tok_len(T[p]));
+        }
+    }
+    for (uint32_t w = r0 >> 5; w < ((r1 + 31) >> 5); ++w) bm[w] = 0;
+    __syncthreads();
+
+    // ---- P1: first parse barrier of each range.  reach(p) = p + step(p);
+    // q is a barrier when max over p < q of reach(p) == q (position 0 is one):
+    // every greedy parse visits it.
+    {
+        uint32_t runmax = 0;
+        for (uint32_t p = r0; p < r1; p += 4) {
+            const uint32_t cw = D[p >> 2];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (p + k < r1) runmax = max(runmax, p + k + code_step((cw >> (8 * k)) & 0xFFu, T, p + k));
+        }
+        uint32_t v = runmax;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(v, d, 64);
+            if (lane >= d) v = max(v, t);
+        }
+        if (lane == 63) misc[wv] = v;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (int i = 0; i < LC_WAVES; ++i) {
+                const uint32_t t = misc[i];
+                misc[i] = acc;
+                acc = max(acc, t);
+            }
+        }
+        __syncthreads();
+        const uint32_t prev_lane = __shfl_up(v, 1, 64);
+        uint32_t carry = max(misc[wv], lane ? prev_lane : 0u);
+        if (tid == 0) carry = 0;
+        uint32_t first = 0xFFFFFFFFu;
+        for (uint32_t p = r0; p < r1 && first == 0xFFFFFFFFu; p += 4) {
+            const uint32_t cw = D[p >> 2];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (p + k < r1 && first == 0xFFFFFFFFu) {
+                    if (carry <= p + k) first = p + k;
+                    else carry = max(carry, p + k + code_step((cw >> (8 * k)) & 0xFFu, T, p + k));
+                }
+            }
+        }
+        fb[tid] = r0 < r1 ? first : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+
+    // ---- P2: exact greedy parse (lz77.cpp:378-406) from this range's
+    // barrier to the first barrier of a later range; token starts -> bm.
+    {
+        const uint32_t pstart = fb[tid];
+        if (pstart != 0xFFFFFFFFu) {
+            uint32_t pos = pstart;
+            while (pos < n) {
+                if (pos >= r1 && fb[pos / S] == pos) break;
+                atomicOr(&bm[pos >> 5], 1u << (pos & 31));
+                pos += code_step(code[pos], T, pos);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- C: per range bit counts, block scan, stored / fixed choice
+    // (deflate_block.cpp:414-421), emission of the range's tokens.
+    const uint32_t mybits = r0 < r1 ? range_bits(bm, T, r0, r1) : 0u;
+    const uint32_t incl = wave_incl_scan(mybits, lane);
+    if (lane == 63) misc[wv] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < LC_WAVES; ++i) {
+            const uint32_t t = misc[i];
+            misc[i] = acc;
+            acc += t;
+        }
+        misc[LC_WAVES] = acc;
+    }
+    __syncthreads();
+    const uint32_t excl = misc[wv] + incl - mybits;
+    const uint64_t fixed = 3ull + misc[LC_WAVES] + 7ull;
+    const bool is_last = c == nchunks - 1;
+    const bool stored = (uint64_t)(n + 5) * 8ull <= fixed;
+    if (tid == 0) {
+        info[c].fixed_bits = fixed;
+        info[c].raw_len = n;
+        info[c].is_stored = stored ? 1u : 0u;
+    }
+    if (stored) return;
+    uint32_t* stg = stage + (uint64_t)c * stage_words;
+    const uint32_t nw = (uint32_t)((fixed + 31) >> 5) + 1u;
+    for (uint32_t i = tid; i < nw; i += LC_THREADS) stg[i] = 0;
+    __syncthreads();
+    if (tid == 0) atomicOr(&stg[0], (is_last ? 1u : 0u) | 2u);  // BFINAL, BTYPE = 01
+    if (mybits) {
+        const uint32_t bitpos = 3u + excl;
+        uint32_t wi = bitpos >> 5;
+        uint32_t nacc = bitpos & 31u;
+        bool partial = nacc != 0;
+        uint64_t acc = 0;
+        auto put = [&](uint32_t tkn) {
+            uint32_t v, len;
+            const uint32_t l = tok_len(tkn);
+            if (l) fixed_match(l, tok_dist(tkn), v, len);
+            else fixed_lit(tkn & 0xFFu, v, len);
+            acc |= (uint64_t)v << nacc;
+            nacc += len;
+            if (nacc >= 32) {
+                if (partial) {
+                    atomicOr(&stg[wi], (uint32_t)acc);
+                    partial = false;
+                } else {
+                    stg[wi] = (uint32_t)acc;
+                }
+                ++wi;
+                acc >>= 32;
+                nacc -= 32;
+            }
+        };
+        uint32_t pk[8];
+        int cnt = 0;
+        for (uint32_t w = r0 >> 5; w < ((r1 + 31) >> 5); ++w) {
+            uint32_t x = bm[w];
+            while (x) {
+                pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
+                x &= x - 1;
+                if (cnt == 8) {
+                    uint32_t tv[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
+                    for (int k = 0; k < 8; ++k) put(tv[k]);
+                    cnt = 0;
+                }
+            }
+        }
+        for (int k = 0; k < cnt; ++k) put(T[pk[k]]);
+        if (nacc) atomicOr(&stg[wi], (uint32_t)acc);
+    }
+}
+
+// Start-up check: same-address LDS exchanges issued by one wave instruction
+// take effect in ascending lane order.
+__global__ __launch_bounds__(64) void exchange_order_kernel(uint32_t* __restrict__ violations) {
+    __shared__ uint32_t tab[64];
+    const uint32_t lane = threadIdx.x, pat = blockIdx.x;
+    tab[lane] = 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t mask = (pat & 3) == 0 ? 0u : ((pat & 3) == 1 ? 3u : ((pat & 3) == 2 ? 15u : 63u));
+    const uint32_t a = ((lane * (pat | 1u) + (pat >> 2)) >> 1) & mask;
+    const uint32_t old = atomicExch(&tab[a], lane);
+    uint32_t want = 0xFFFFFFFFu;
+    for (uint32_t j = 0; j < lane; ++j)
+        if ((((j * (pat | 1u) + (pat >> 2)) >> 1) & mask) == a) want = j;
+    if (old != want) atomicOr(&violations[0], 1u);
 }
 
 __global__ void mark_stored_kernel(uint64_t n_total, uint32_t chunk, uint32_t nchunks, ChunkInfo* info) {
@@ -814,6 +1865,70 @@ inline void record(hipEvent_t ev, hipStream_t s) {
     if (ev) (void)hipEventRecord(ev, s);
 }
 
+template <int HB, bool EXCH>
+hipError_t launch_chunk(const Plan& p) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l1_chunk_kernel<HB, EXCH>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)LC_LDS_BYTES);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((l1_chunk_kernel<HB, EXCH>), dim3(p.nchunks), dim3(LC_THREADS), LC_LDS_BYTES, p.stream, p.d_in,
+                       p.n, p.chunk, p.nchunks, p.d_tok, p.d_stage, p.stage_words, p.d_info);
+    return hipSuccess;
+}
+
+template <int W, bool EXCH>
+hipError_t launch_barrier(const Plan& p) {
+    static bool attr = false;
+    const size_t smem = LBCfg<W>::LDS_BYTES;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l1_barrier_kernel<W, EXCH>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((l1_barrier_kernel<W, EXCH>), dim3(p.nchunks), dim3(LC_THREADS), smem, p.stream, p.d_in,
+                       p.n, p.chunk, p.nchunks, p.d_tok, p.d_stage, p.stage_words, p.d_info);
+    return hipSuccess;
+}
+
+template <int HB>
+hipError_t launch_wide(const Plan& p) {
+    static bool attr = false;
+    const size_t smem = L1Cfg<HB>::TAB_BYTES > L1_PHASE_B_BYTES ? L1Cfg<HB>::TAB_BYTES : L1_PHASE_B_BYTES;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l1_lz77_kernel<HB>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(l1_lz77_kernel<HB>, dim3(p.nchunks), dim3(L1_THREADS), smem, p.stream, p.d_in, p.n, p.chunk,
+                       p.nchunks, p.d_tok, p.d_stage, p.stage_words, p.d_info);
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t check_exchange_order(hipStream_t stream, bool* ok) {
+    uint32_t* d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof(uint32_t));
+    if (e != hipSuccess) return e;
+    uint32_t h = 0;
+    e = hipMemsetAsync(d, 0, sizeof(uint32_t), stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(exchange_order_kernel, dim3(1024), dim3(64), 0, stream, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(d);
+    *ok = e == hipSuccess && h == 0;
+    return e;
+}
+
+namespace {
 }  // namespace
 
 uint32_t max_chunk_for_hash_bits(int hash_bits) { return hash_bits <= 12 ? MAX_CHUNK_L1 : MAX_CHUNK_L1 / 2; }
@@ -831,29 +1946,22 @@ hipError_t launch_parallel_compress(const Plan& p) {
     if (p.level == 0) {
         hipLaunchKernelGGL(mark_stored_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, p.n, p.chunk, nb, p.d_info);
     } else if (p.level <= 3) {
-        if (p.level == 1) {
-            const size_t smem = L1Cfg<12>::TAB_BYTES > L1_PHASE_B_BYTES ? L1Cfg<12>::TAB_BYTES : L1_PHASE_B_BYTES;
-            static bool attr12 = false;
-            if (!attr12) {
-                hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l1_lz77_kernel<12>),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-                if (e != hipSuccess) return e;
-                attr12 = true;
+        const int hb = p.level == 1 ? 12 : 14;
+        hipError_t e = hipSuccess;
+        if (p.chunk <= LC_MAX_CHUNK) {
+            if (hb == 12) {
+                if (p.lds_max >= LBCfg<2>::LDS_BYTES)
+                    e = p.exch_ok ? launch_barrier<2, true>(p) : launch_barrier<2, false>(p);
+                else
+                    e = p.exch_ok ? launch_barrier<1, true>(p) : launch_barrier<1, false>(p);
             }
-            hipLaunchKernelGGL(l1_lz77_kernel<12>, dim3(nb), dim3(L1_THREADS), smem, s, p.d_in, p.n, p.chunk, nb,
-                               p.d_d16, p.d_m1, p.d_stage, p.stage_words, p.d_info);
+            else e = p.exch_ok ? launch_chunk<14, true>(p) : launch_chunk<14, false>(p);
+        } else if (hb == 12) {
+            e = launch_wide<12>(p);
         } else {
-            const size_t smem = L1Cfg<14>::TAB_BYTES > L1_PHASE_B_BYTES ? L1Cfg<14>::TAB_BYTES : L1_PHASE_B_BYTES;
-            static bool attr14 = false;
-            if (!attr14) {
-                hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l1_lz77_kernel<14>),
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-                if (e != hipSuccess) return e;
-                attr14 = true;
-            }
-            hipLaunchKernelGGL(l1_lz77_kernel<14>, dim3(nb), dim3(L1_THREADS), smem, s, p.d_in, p.n, p.chunk, nb,
-                               p.d_d16, p.d_m1, p.d_stage, p.stage_words, p.d_info);
+            return hipErrorInvalidValue;
         }
+        if (e != hipSuccess) return e;
     } else {
         return hipErrorNotSupported;
     }

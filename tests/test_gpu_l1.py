@@ -47,3 +47,11 @@ def test_small_l1(amd, kind, n, fmt):
 def test_levels_blocks(amd, kind, level, block):
     data = corpus.generate(kind, 300_000, 77)
     _check(amd, data, level, 2, block)
+
+
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+@pytest.mark.parametrize("block", [131072 + 8, 200_000, 262144])
+def test_l1_wide_chunks(amd, kind, block):
+    """Chunks above 128 KiB take the wide kernel (level 1 only)."""
+    data = corpus.generate(kind, 700_000, 91)
+    _check(amd, data, 1, 0, block)
