@@ -388,6 +388,94 @@ void oracle_l1_candidates(const uint8_t* src, size_t n, int hash_bits, uint16_t*
 }
 
 /* ======================================================================
+ * HIP L6-class matcher (levels 4..12).  Not orot's parse: a GPU-friendly
+ * definition with the same shape as orot's standard path (lz77.cpp:435-496):
+ *   C(p)     latest q < p, q + 4 <= n, with h12(q) == h12(p), 0 if none
+ *            (every earlier position is in the table);
+ *   M(x)     longest match among the chain C(x), C(C(x)), ... of at most
+ *            max_chain candidates with x - cand <= 32768; a candidate counts
+ *            only if its first 4 bytes equal x's; ties keep the nearer one;
+ *            the walk stops at nice_len;
+ *   parse    lazy, lazy_depth steps, the probe at pos + 1 being M(pos + 1):
+ *            a literal is emitted when M(pos + 1) > best + 1 (orot's rule).
+ * ====================================================================== */
+
+typedef struct {
+    int max_chain, nice_len, lazy_depth;
+} l6_cfg;
+
+static const l6_cfg L6LEVELS[13] = {
+    {0, 0, 0},    {0, 0, 0},    {0, 0, 0},    {0, 0, 0},     {16, 128, 1},  {32, 128, 1},   {32, 258, 1},
+    {64, 258, 2}, {128, 258, 2}, {256, 258, 3}, {512, 258, 3}, {1024, 258, 3}, {2048, 258, 3},
+};
+
+#define L6_HASH_BITS 12
+
+void oracle_l6_match(const uint8_t* src, size_t n, int level, uint32_t* mlen, uint32_t* mdist) {
+    const l6_cfg* c = &L6LEVELS[clamp_level(level)];
+    uint32_t* head = (uint32_t*)calloc((size_t)1 << L6_HASH_BITS, sizeof(uint32_t));
+    uint32_t* C = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
+    for (size_t p = 0; p < n; ++p) {
+        C[p] = 0;
+        if (p + 4 > n) continue;
+        const uint32_t h = hash4(src + p, L6_HASH_BITS);
+        C[p] = head[h];
+        head[h] = (uint32_t)p + 1;
+    }
+    for (size_t x = 0; x < n; ++x) {
+        uint32_t best = 0, bd = 0;
+        if (x + 4 <= n) {
+            const int cap = n - x < 258 ? (int)(n - x) : 258;
+            uint32_t cur = C[x];
+            for (int k = 0; k < c->max_chain && cur; ++k) {
+                const uint32_t q = cur - 1;
+                if (x - q > 32768) break;
+                if (le32(src + q) == le32(src + x)) {
+                    const uint32_t l = (uint32_t)lcp(src + x, src + q, cap);
+                    if (l > best) {
+                        best = l;
+                        bd = (uint32_t)(x - q);
+                        if ((int)l >= c->nice_len) break;
+                    }
+                }
+                cur = C[q];
+            }
+        }
+        mlen[x] = best >= 3 ? best : 0;
+        mdist[x] = best >= 3 ? bd : 0;
+    }
+    free(C);
+    free(head);
+}
+
+size_t oracle_l6_lz77(const uint8_t* src, size_t n, int level, uint32_t* tokens) {
+    const l6_cfg* c = &L6LEVELS[clamp_level(level)];
+    if (n == 0) return 0;
+    uint32_t* ml = (uint32_t*)calloc(2 * n, sizeof(uint32_t));
+    uint32_t* md = ml + n;
+    oracle_l6_match(src, n, level, ml, md);
+    size_t nt = 0, pos = 0;
+    while (pos < n) {
+        if (pos + 3 > n || ml[pos] < 3) {
+            tokens[nt++] = src[pos++];
+            continue;
+        }
+        uint32_t b = ml[pos], d = md[pos];
+        int k = c->lazy_depth;
+        while (k > 0 && pos + b + 1 < n && pos + 1 + 3 <= n && ml[pos + 1] > b + 1) {
+            tokens[nt++] = src[pos++];
+            b = ml[pos];
+            d = md[pos];
+            --k;
+        }
+        tokens[nt++] = ORACLE_TOK_MATCH | (b << 16) | d;
+        pos += b;
+    }
+    free(ml);
+    return nt;
+}
+
+/* ======================================================================
  * Block statistics and orot's Fixed/Stored emission
  * (src/deflate/deflate_block.cpp:21-70, 107-183, 389-474).
  * ====================================================================== */
@@ -727,6 +815,7 @@ typedef struct {
     uint32_t* tokens;
     size_t ntok;
     int level;
+    int l6gpu;
 } chunk_job;
 
 typedef struct {
@@ -744,7 +833,8 @@ static void* worker(void* arg) {
         pthread_mutex_unlock(&q->mu);
         if (i >= q->njobs) break;
         chunk_job* j = &q->jobs[i];
-        j->ntok = oracle_lz77(j->src, j->len, j->level, j->tokens);
+        j->ntok = j->l6gpu ? oracle_l6_lz77(j->src, j->len, j->level, j->tokens)
+                           : oracle_lz77(j->src, j->len, j->level, j->tokens);
     }
     return NULL;
 }
@@ -755,10 +845,23 @@ static size_t default_block_size(int level) {
     return 1048576;
 }
 
+static size_t parallel_compress_impl(int level, int format, int threads, size_t block_size, const uint8_t* src,
+                                     size_t n, uint8_t* dst, size_t cap, int l6gpu);
+
 size_t oracle_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src, size_t n,
                                 uint8_t* dst, size_t cap) {
+    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, 0);
+}
+
+size_t oracle_gpu_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src,
+                                    size_t n, uint8_t* dst, size_t cap) {
+    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, clamp_level(level) >= 4);
+}
+
+static size_t parallel_compress_impl(int level, int format, int threads, size_t block_size, const uint8_t* src,
+                                     size_t n, uint8_t* dst, size_t cap, int l6gpu) {
     const int lv = clamp_level(level);
-    if (LEVELS[lv].bt4) return 0;
+    if (LEVELS[lv].bt4 && !l6gpu) return 0;
     if (block_size == 0) block_size = default_block_size(level);
     const size_t nb = n ? (n + block_size - 1) / block_size : 1;
     chunk_job* jobs = (chunk_job*)calloc(nb, sizeof(chunk_job));
@@ -768,6 +871,7 @@ size_t oracle_parallel_compress(int level, int format, int threads, size_t block
         jobs[i].src = src + i * block_size;
         jobs[i].len = n ? (n - i * block_size < block_size ? n - i * block_size : block_size) : 0;
         jobs[i].level = level;
+        jobs[i].l6gpu = l6gpu;
         jobs[i].tokens = (uint32_t*)calloc(jobs[i].len ? jobs[i].len : 1, sizeof(uint32_t));
         if (!jobs[i].tokens) ok = 0;
     }

@@ -54,6 +54,18 @@ void oracle_l1_candidates(const uint8_t* src, size_t n, int hash_bits, uint16_t*
 size_t oracle_parallel_compress(int level, int format, int threads, size_t block_size,
                                 const uint8_t* src, size_t n, uint8_t* dst, size_t cap);
 
+/* The HIP path's own L6-class matcher (levels 4..12; see orot_oracle.c):
+ * match length / distance per position, and the lazy token stream. */
+void oracle_l6_match(const uint8_t* src, size_t n, int level, uint32_t* mlen, uint32_t* mdist);
+size_t oracle_l6_lz77(const uint8_t* src, size_t n, int level, uint32_t* tokens);
+
+/* What the HIP path computes: levels 0..3 as oracle_parallel_compress
+ * (orot's bytes); levels 4..12 with the L6-class matcher above, orot's chunk
+ * sizes (src/parallel/parallel_compressor.cpp:16-20), and
+ * oracle_encode_blocks_dyn. */
+size_t oracle_gpu_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src,
+                                    size_t n, uint8_t* dst, size_t cap);
+
 /* Single-chunk block emission with orot's Fixed/Stored hint
  * (src/deflate/deflate_block.cpp:389-474, hint Fixed).  Appends to a bit
  * buffer starting at bit offset *bitpos of dst (dst must be zero-filled).

@@ -86,6 +86,16 @@ struct deflate_parallel_ctx {
     size_t out_cap = 0;
     uint32_t* d_tok = nullptr;
     size_t tok_cap = 0;
+    uint16_t* d_pd = nullptr;
+    size_t pd_cap = 0;
+    uint32_t* d_aux = nullptr;
+    size_t aux_cap = 0;
+    uint32_t* d_ntok = nullptr;
+    size_t ntok_cap = 0;
+    uint32_t* d_blkpos = nullptr;
+    size_t blkpos_cap = 0;
+    uint8_t* d_blocks = nullptr;
+    size_t blocks_cap = 0;
     uint32_t* d_stage = nullptr;
     size_t stage_cap = 0;
     ChunkInfo* d_info = nullptr;
@@ -103,7 +113,7 @@ struct deflate_parallel_ctx {
     ~deflate_parallel_ctx() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
-        for (void* p : {(void*)d_in, (void*)d_out, (void*)d_tok, (void*)d_stage, (void*)d_info,
+        for (void* p : {(void*)d_in, (void*)d_out, (void*)d_tok, (void*)d_pd, (void*)d_aux, (void*)d_ntok, (void*)d_blkpos, (void*)d_blocks, (void*)d_stage, (void*)d_info,
                         (void*)d_res})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -112,11 +122,12 @@ struct deflate_parallel_ctx {
     // Runs the pipeline on device buffers; returns the framed size or 0.
     size_t run(const uint8_t* in, size_t n, uint8_t* out, size_t cap) {
         const int lv = clamp_level(level);
-        if (lv >= 4) {
-            set_error_msg("deflate_parallel_compress: levels 4-12 are not available on the GPU path yet");
+        if (lv >= 4 && block > orot_gpu::l6_max_chunk()) {
+            set_error_msg("deflate_parallel_compress: block_size " + std::to_string(block) +
+                          " exceeds the GPU limit " + std::to_string(orot_gpu::l6_max_chunk()) + " for this level");
             return 0;
         }
-        if (lv >= 1) {
+        if (lv >= 1 && lv <= 3) {
             const uint32_t maxc = orot_gpu::max_chunk_for_hash_bits(lv == 1 ? 12 : 14);
             if (block > maxc) {
                 set_error_msg("deflate_parallel_compress: block_size " + std::to_string(block) +
@@ -136,7 +147,16 @@ struct deflate_parallel_ctx {
         }
         const uint32_t nb = (uint32_t)nb64;
         const uint64_t sw = orot_gpu::stage_words_for_chunk(chunk);
-        if (!grow(d_tok, tok_cap, (size_t)nb * ((chunk + 7u) & ~7u) + 8, "hipMalloc(tok)") ||
+        const size_t padded = (size_t)nb * ((chunk + 7u) & ~7u) + 8;
+        if (lv >= 4) {
+            const size_t mb = orot_gpu::l6_max_blocks(chunk);
+            if (!grow(d_pd, pd_cap, padded, "hipMalloc(pd)") || !grow(d_aux, aux_cap, padded, "hipMalloc(tokens)") ||
+                !grow(d_ntok, ntok_cap, nb, "hipMalloc(ntok)") ||
+                !grow(d_blkpos, blkpos_cap, (size_t)nb * mb, "hipMalloc(blkpos)") ||
+                !grow(d_blocks, blocks_cap, (size_t)nb * mb * orot_gpu::l6_block_record_bytes(), "hipMalloc(blocks)"))
+                return 0;
+        }
+        if (!grow(d_tok, tok_cap, padded, "hipMalloc(tok)") ||
             !grow(d_stage, stage_cap, (size_t)(sw * nb), "hipMalloc(stage)") ||
             !grow(d_info, info_cap, nb, "hipMalloc(info)") || !grow(d_res, res_cap, 1, "hipMalloc(result)"))
             return 0;
@@ -159,6 +179,11 @@ struct deflate_parallel_ctx {
         p.d_out = out;
         p.cap = cap;
         p.d_tok = d_tok;
+        p.d_pd = d_pd;
+        p.d_aux = d_aux;
+        p.d_ntok = d_ntok;
+        p.d_blkpos = d_blkpos;
+        p.d_blocks = d_blocks;
         p.d_stage = d_stage;
         p.stage_words = sw;
         p.d_info = d_info;

@@ -22,202 +22,14 @@
 //   assemble_kernel   one workgroup per chunk: shifts the chunk's bits to its
 //                     offset in the output (stored chunks copy the input).
 #include "deflate_gpu.h"
+#include "device_common.h"
 
 namespace orot_gpu {
 namespace {
 
-constexpr uint32_t CRC_POLY = 0xEDB88320u;
-constexpr uint32_t ADLER_MOD = 65521u;
 constexpr int L1_THREADS = 512;
 constexpr int L1_WAVES = L1_THREADS / 64;
 constexpr uint32_t MAX_CHUNK_L1 = 256u * 1024u;  // 12-bit tables: 8 sub-ranges of 32 KiB
-
-// ---------------------------------------------------------------------------
-// GF(2) arithmetic for CRC-32 combination (reflected: bit 31 = x^0).
-// ---------------------------------------------------------------------------
-constexpr uint32_t gf2_mul_ce(uint32_t a, uint32_t b) {
-    uint32_t p = 0;
-    for (int i = 0; i < 32; ++i) {
-        if (a & 0x80000000u) p ^= b;
-        a <<= 1;
-        b = (b >> 1) ^ (CRC_POLY & (0u - (b & 1u)));
-    }
-    return p;
-}
-
-struct PowTable {
-    uint32_t v[64];  // v[k] = x^(8 * 2^k) mod P
-};
-constexpr PowTable make_pow_table() {
-    PowTable t{};
-    uint32_t x = 0x00800000u;  // x^8
-    for (int k = 0; k < 64; ++k) {
-        t.v[k] = x;
-        x = gf2_mul_ce(x, x);
-    }
-    return t;
-}
-struct CrcTable {
-    uint32_t t[4][256];
-};
-constexpr CrcTable make_crc_table() {
-    CrcTable c{};
-    for (uint32_t i = 0; i < 256; ++i) {
-        uint32_t v = i;
-        for (int k = 0; k < 8; ++k) v = (v >> 1) ^ (CRC_POLY & (0u - (v & 1u)));
-        c.t[0][i] = v;
-    }
-    for (uint32_t i = 0; i < 256; ++i)
-        for (int s = 1; s < 4; ++s) c.t[s][i] = (c.t[s - 1][i] >> 8) ^ c.t[0][c.t[s - 1][i] & 0xFF];
-    return c;
-}
-
-__constant__ PowTable c_pow = make_pow_table();
-__constant__ CrcTable c_crc = make_crc_table();
-
-__device__ __forceinline__ uint32_t gf2_mul(uint32_t a, uint32_t b) {
-    uint32_t p = 0;
-#pragma unroll 8
-    for (int i = 0; i < 32; ++i) {
-        if (a & 0x80000000u) p ^= b;
-        a <<= 1;
-        b = (b >> 1) ^ (CRC_POLY & (0u - (b & 1u)));
-    }
-    return p;
-}
-
-// x^(8*len) mod P
-__device__ uint32_t gf2_x8n(uint64_t len) {
-    uint32_t r = 0x80000000u;
-    int k = 0;
-    while (len) {
-        if (len & 1) r = gf2_mul(c_pow.v[k], r);
-        len >>= 1;
-        ++k;
-    }
-    return r;
-}
-
-// ---------------------------------------------------------------------------
-// Small helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t ld32u(const uint8_t* p) {
-    uint32_t v;
-    __builtin_memcpy(&v, p, 4);
-    return v;
-}
-__device__ __forceinline__ uint64_t ld64u(const uint8_t* p) {
-    uint64_t v;
-    __builtin_memcpy(&v, p, 8);
-    return v;
-}
-__device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
-
-template <int HB>
-__device__ __forceinline__ uint32_t hash4(uint32_t v) {
-    return (v * 0x9E3779B1u) >> (32 - HB);
-}
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    return v;
-}
-
-// ---------------------------------------------------------------------------
-// Fixed Huffman symbols (RFC 1951 3.2.6), LSB-first (bit-reversed) codes.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t lit_bits(uint32_t b) { return b < 144 ? 8u : 9u; }
-
-__device__ __forceinline__ void fixed_lit(uint32_t b, uint32_t& v, uint32_t& len) {
-    const uint32_t code = b < 144 ? 0x30u + b : 0x190u + (b - 144u);
-    len = lit_bits(b);
-    v = __builtin_bitreverse32(code) >> (32 - len);
-}
-
-struct LenSym {
-    uint32_t code, ebits, extra;
-};
-// match length 3..258 -> (code index 0..28, extra bit count, extra value)
-__device__ __forceinline__ LenSym len_sym(uint32_t L) {
-    if (L <= 10) return {L - 3, 0, 0};
-    if (L == 258) return {28, 0, 0};
-    const uint32_t x = L - 3;
-    const uint32_t e = 31 - __builtin_clz(x) - 2;
-    return {4 * e + (x >> e), e, x & ((1u << e) - 1u)};
-}
-// distance 1..32768 -> (code 0..29, extra bit count, extra value)
-__device__ __forceinline__ LenSym dist_sym(uint32_t d) {
-    if (d <= 4) return {d - 1, 0, 0};
-    const uint32_t x = d - 1;
-    const uint32_t e = 31 - __builtin_clz(x) - 1;
-    return {2 * e + 2 + ((x >> e) & 1u), e, x & ((1u << e) - 1u)};
-}
-
-__device__ __forceinline__ uint32_t match_bits(uint32_t L, uint32_t d) {
-    const LenSym ls = len_sym(L), ds = dist_sym(d);
-    return (ls.code < 23 ? 7u : 8u) + ls.ebits + 5u + ds.ebits;  // symbols 257..279 are 7-bit
-}
-
-__device__ __forceinline__ void fixed_match(uint32_t L, uint32_t d, uint32_t& v, uint32_t& len) {
-    const LenSym ls = len_sym(L), ds = dist_sym(d);
-    const uint32_t sym = 257 + ls.code;
-    const uint32_t cl = sym < 280 ? 7u : 8u;
-    const uint32_t code = sym < 280 ? sym - 256u : 0xC0u + (sym - 280u);
-    const uint32_t rc = __builtin_bitreverse32(code) >> (32 - cl);
-    const uint32_t rd = __builtin_bitreverse32(ds.code) >> 27;
-    v = rc | (ls.extra << cl) | (rd << (cl + ls.ebits)) | (ds.extra << (cl + ls.ebits + 5));
-    len = cl + ls.ebits + 5 + ds.ebits;
-}
-
-// ---------------------------------------------------------------------------
-// Match length helpers (src/simd/simd_detect.cpp:12-31 semantics: exact
-// common-prefix length capped at `cap`).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t lcp_lane(const uint8_t* a, const uint8_t* b, uint32_t cap) {
-    uint32_t l = 0;
-    while (l + 8 <= cap) {
-        const uint64_t x = ld64u(a + l) ^ ld64u(b + l);
-        if (x) return l + (uint32_t)(__builtin_ctzll(x) >> 3);
-        l += 8;
-    }
-    while (l < cap && a[l] == b[l]) ++l;
-    return l;
-}
-
-// Whole wave compares 256 bytes per round; every lane returns the same value.
-__device__ uint32_t lcp_wave(const uint8_t* a, const uint8_t* b, uint32_t cap, int lane) {
-    for (uint32_t r = 0; r < cap; r += 256) {
-        const uint32_t off = r + 4u * (uint32_t)lane;
-        uint32_t x = 0;
-        if (off + 4 <= cap) {
-            x = ld32u(a + off) ^ ld32u(b + off);
-        } else if (off < cap) {
-            for (uint32_t k = 0; k < cap - off; ++k) x |= (uint32_t)(a[off + k] ^ b[off + k]) << (8 * k);
-        }
-        const uint64_t bm = ballot(x != 0);
-        if (bm) {
-            const int f = __builtin_ctzll(bm);
-            const uint32_t xf = __shfl(x, f, 64);
-            return r + 4u * (uint32_t)f + (uint32_t)(__builtin_ctz(xf) >> 3);
-        }
-    }
-    return cap;
-}
-
-__device__ __forceinline__ void clear_bits(uint32_t* bm, uint32_t x, uint32_t y, int lane) {
-    if (x >= y) return;
-    const uint32_t w0 = x >> 5, w1 = (y - 1) >> 5;
-    for (uint32_t w = w0 + (uint32_t)lane; w <= w1; w += 64) {
-        const uint32_t lo = (w == w0) ? (x & 31u) : 0u;
-        const uint32_t hi = (w == w1) ? ((y - 1) & 31u) : 31u;
-        const uint32_t mask = (hi == 31u ? 0xFFFFFFFFu : ((2u << hi) - 1u)) & ~((1u << lo) - 1u);
-        bm[w] &= ~mask;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Checksum partials per chunk.
@@ -304,22 +116,6 @@ struct L1Cfg {
 
 constexpr uint32_t L1_BM_WORDS = MAX_CHUNK_L1 / 32;
 constexpr uint32_t L1_PHASE_B_BYTES = (L1_BM_WORDS + 2 * L1_THREADS + 16) * 4 + L1_THREADS;
-
-// Per-position token word: literal byte | match length << 8 | distance << 17.
-__device__ __forceinline__ uint32_t tok_len(uint32_t t) { return (t >> 8) & 0x1FFu; }
-__device__ __forceinline__ uint32_t tok_dist(uint32_t t) { return t >> 17; }
-__device__ __forceinline__ uint32_t tok_bits(uint32_t t) {
-    const uint32_t l = tok_len(t);
-    return l ? match_bits(l, tok_dist(t)) : lit_bits(t & 0xFFu);
-}
-
-// Byte j (0..15) of a 16-byte window held as two little-endian u64.
-__device__ __forceinline__ uint32_t win_byte(uint64_t w0, uint64_t w1, uint32_t j) {
-    return (uint32_t)((j < 8 ? (w0 >> (8 * j)) : (w1 >> (8 * (j - 8)))) & 0xFFu);
-}
-__device__ __forceinline__ uint64_t win_u64(uint64_t w0, uint64_t w1, uint32_t j) {  // bytes j..j+7, j < 8
-    return j == 0 ? w0 : ((w0 >> (8 * j)) | (w1 << (64 - 8 * j)));
-}
 
 template <int HB>
 __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
@@ -1618,11 +1414,6 @@ __global__ void mark_stored_kernel(uint64_t n_total, uint32_t chunk, uint32_t nc
 // ---------------------------------------------------------------------------
 // Scan of chunk offsets, checksum combine, framing.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t stored_bits(uint64_t off, uint32_t L) {
-    const uint64_t pieces = L == 0 ? 1 : (L + 65534u) / 65535u;
-    const uint64_t pad0 = (8 - ((off + 3) & 7)) & 7;
-    return 3 + pad0 + 32 * pieces + 8 * (pieces - 1) + 8ull * L;
-}
 
 constexpr int FIN_THREADS = 1024;
 
@@ -1631,6 +1422,7 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(ChunkInfo* __rest
                                                                uint8_t* __restrict__ out, uint64_t cap,
                                                                Result* __restrict__ res) {
     __shared__ uint64_t sz[FIN_THREADS];
+    __shared__ uint64_t sz8[FIN_THREADS][8];
     __shared__ uint32_t st[FIN_THREADS];
     __shared__ uint32_t ln[FIN_THREADS];
     __shared__ uint32_t red0[FIN_THREADS];
@@ -1645,13 +1437,15 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(ChunkInfo* __rest
             sz[tid] = info[i].fixed_bits;
             st[tid] = info[i].is_stored;
             ln[tid] = info[i].raw_len;
+            if (st[tid] == 2)
+                for (int r = 0; r < 8; ++r) sz8[tid][r] = info[i].sz8[r];
         }
         __syncthreads();
         if (tid == 0) {
             uint64_t off = carry;
             const uint32_t cnt = min((uint32_t)FIN_THREADS, nchunks - t0);
             for (uint32_t k = 0; k < cnt; ++k) {
-                const uint64_t b = st[k] ? stored_bits(off, ln[k]) : sz[k];
+                const uint64_t b = st[k] == 1 ? stored_bits(off, ln[k]) : (st[k] == 2 ? sz8[k][off & 7] : sz[k]);
                 sz[k] = off;
                 off += b;
             }
@@ -1788,7 +1582,7 @@ __device__ __forceinline__ ChunkView make_view(const uint8_t* src, uint32_t chun
     v.raw = src + (uint64_t)c * chunk;
     v.off = info[c].bit_off;
     v.len = info[c].raw_len;
-    v.stored = info[c].is_stored;
+    v.stored = info[c].is_stored == 1 ? 1u : 0u;
     v.last = c == nchunks - 1 ? 1u : 0u;
     return v;
 }
@@ -1933,7 +1727,7 @@ namespace {
 
 uint32_t max_chunk_for_hash_bits(int hash_bits) { return hash_bits <= 12 ? MAX_CHUNK_L1 : MAX_CHUNK_L1 / 2; }
 
-uint64_t stage_words_for_chunk(uint32_t chunk) { return ((uint64_t)chunk + 64 + 255) / 4; }
+uint64_t stage_words_for_chunk(uint32_t chunk) { return ((uint64_t)chunk + chunk / 512 + 1024) / 4; }
 
 hipError_t launch_parallel_compress(const Plan& p) {
     const uint32_t nb = p.nchunks;
@@ -1963,11 +1757,16 @@ hipError_t launch_parallel_compress(const Plan& p) {
         }
         if (e != hipSuccess) return e;
     } else {
-        return hipErrorNotSupported;
+        hipError_t e = launch_l6_front(p);
+        if (e != hipSuccess) return e;
     }
     record(p.ev[2], s);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(FIN_THREADS), 0, s, p.d_info, nb, p.n, p.chunk, p.format,
                        p.d_out, p.cap, p.d_result);
+    if (p.level >= 4) {
+        hipError_t e = launch_l6_back(p);
+        if (e != hipSuccess) return e;
+    }
     record(p.ev[3], s);
     const uint32_t hdr = p.format == 1 ? 2u : (p.format == 2 ? 10u : 0u);
     hipLaunchKernelGGL(assemble_kernel, dim3(nb), dim3(256), 0, s, p.d_in, p.chunk, nb, p.d_stage, p.stage_words,

@@ -1,0 +1,907 @@
+// deflate_l6.hip -- gfx950 kernels of the L6-class path (levels 4..12).
+//
+// orot's standard path (src/deflate/lz77.cpp:435-496) walks 16-bit hash
+// chains whose tables do not fit a workgroup's LDS; this path keeps the
+// shape of orot's parse -- hash-chain candidates, longest match, nice length,
+// one-position lazy evaluation with the "next > best + 1" rule -- with a
+// definition that maps onto the GPU:
+//   C(p)   latest q < p (q + 4 <= n) with the same 12-bit hash, every earlier
+//          position inserted;
+//   M(x)   longest match among at most max_chain chain candidates within
+//          32 KiB whose first 4 bytes equal x's (ties keep the nearer);
+//   lazy   the probe at pos + 1 is M(pos + 1).
+// Chunks keep orot's block sizes (512 KiB for levels 4-6, 1 MiB above).  Tokens
+// are cut into blocks of at most 16384 symbols, each the cheapest of stored /
+// fixed / dynamic; dynamic codes are optimal length-limited (package-merge).
+//
+//   l6_pred_kernel    C(p) by 8 sub-range LDS tables (lane-ordered LDS
+//                     exchange, or ballot match-any), stored as the in-window
+//                     chain step PD[p] = p - C(p) (0: none within 32 KiB).
+//   l6_match_kernel   M(x) for every x: the chunk is swept in tiles of 4096
+//                     positions with the last 32 KiB of bytes and chain steps
+//                     in LDS rings.
+//   l6_parse_kernel   lazy decisions per position, parse barriers, exact
+//                     parse, tokens (orot's Token layout) in order.
+//   l6_block_kernel   per block: histograms, package-merge lengths, code-length
+//                     run-length header, exact sizes, block type.
+//   l6_size_kernel    chunk size for each incoming bit offset mod 8.
+//   l6_zero_kernel    clears each chunk's staging words.
+//   l6_emit_kernel    per block: header and symbols into the chunk staging.
+#include "deflate_gpu.h"
+#include "device_common.h"
+
+namespace orot_gpu {
+namespace {
+
+constexpr int L6_HB = 12;
+constexpr uint32_t L6_TAB = 1u << L6_HB;
+constexpr int L6P_THREADS = 512;
+constexpr int L6P_W = 8;
+constexpr uint32_t L6P_LDS = L6P_W * L6_TAB * 4u;  // 128 KiB
+
+constexpr int L6M_THREADS = 1024;
+constexpr uint32_t L6M_TILE = 4096;
+constexpr uint32_t L6M_DRING = 65536;  // bytes, power of two
+constexpr uint32_t L6M_PRING = 36864;  // chain steps: 32 KiB + one tile
+constexpr uint32_t L6M_LDS = L6M_DRING + L6M_PRING * 2;
+
+constexpr int L6R_THREADS = 1024;
+constexpr int L6R_WAVES = L6R_THREADS / 64;
+constexpr uint32_t L6_MAX_CHUNK = 1u << 20;
+constexpr uint32_t L6R_BM_WORDS = L6_MAX_CHUNK / 32;
+constexpr uint32_t L6R_LDS = (L6R_BM_WORDS + 3 * L6R_THREADS + 64) * 4;
+
+constexpr uint32_t BLK_SYMS = 16384;
+constexpr int L6B_THREADS = 256;
+
+struct L6Level {
+    int max_chain, nice_len, lazy_depth;
+};
+__constant__ L6Level c_l6lv[13] = {
+    {0, 0, 0},    {0, 0, 0},     {0, 0, 0},     {0, 0, 0},     {16, 128, 1},   {32, 128, 1},   {32, 258, 1},
+    {64, 258, 2}, {128, 258, 2}, {256, 258, 3}, {512, 258, 3}, {1024, 258, 3}, {2048, 258, 3},
+};
+
+__device__ __forceinline__ uint32_t ml_len(uint32_t m) { return m & 0xFFFFu; }
+__device__ __forceinline__ uint32_t ml_dist(uint32_t m) { return m >> 16; }
+
+// ---------------------------------------------------------------------------
+// C(p) and the in-window chain step PD[p].
+// ---------------------------------------------------------------------------
+template <bool EXCH>
+__global__ __launch_bounds__(L6P_THREADS) void l6_pred_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
+                                                              uint32_t chunk, uint32_t* __restrict__ tokw,
+                                                              uint16_t* __restrict__ pd) {
+    extern __shared__ uint32_t smem[];
+    const uint32_t c = blockIdx.x;
+    const uint64_t base = (uint64_t)c * chunk;
+    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
+    const uint8_t* s = src + base;
+    const uint64_t cp = (chunk + 7u) & ~7u;
+    uint32_t* T = tokw + (uint64_t)c * cp;
+    uint16_t* PD = pd + (uint64_t)c * cp;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t nh = n >= 4 ? n - 3 : 0;
+    const uint32_t SR = (((nh + L6P_W - 1) / L6P_W) + 63u) & ~63u;
+    {
+        uint32_t* tab = smem + (uint32_t)wv * L6_TAB;
+        for (uint32_t i = lane; i < L6_TAB; i += 64) tab[i] = 0;
+        const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
+        for (uint32_t p0 = a; p0 < b; p0 += 512) {
+            uint32_t wd[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wd[j] = ld32u(s + min(p0 + 64u * j + (uint32_t)lane, b - 1));
+            uint32_t res[8];
+            if (EXCH) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                    res[j] = p < b ? atomicExch(&tab[hash4<L6_HB>(wd[j])], p + 1u) : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t q0 = p0 + 64u * j;
+                    const uint32_t p = q0 + (uint32_t)lane;
+                    const bool valid = p < b;
+                    const uint32_t h = hash4<L6_HB>(wd[j]);
+                    uint64_t m = ballot(valid);
+#pragma unroll
+                    for (int k = 0; k < L6_HB; ++k) {
+                        const uint32_t bit = (h >> k) & 1u;
+                        const uint64_t bk = ballot(bit != 0);
+                        m &= bk ^ ((uint64_t)bit - 1ull);
+                    }
+                    res[j] = 0;
+                    if (valid) {
+                        const uint64_t below = m & ((1ull << lane) - 1ull);
+                        if (below) res[j] = q0 + (uint32_t)(63 - __builtin_clzll(below)) + 1u;
+                        else res[j] = tab[h];
+                        if ((m >> lane) == 1ull) tab[h] = p + 1u;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                if (p < b) T[p] = res[j];
+            }
+        }
+    }
+    __syncthreads();
+    // first occurrences of a sub-range take the latest earlier sub-range's entry
+    for (uint32_t p = tid; p < n; p += L6P_THREADS) {
+        uint32_t step = 0;
+        if (p < nh) {
+            uint32_t C1 = T[p];
+            const uint32_t w = p / SR;
+            if (C1 == 0 && w > 0) {
+                const uint32_t h = hash4<L6_HB>(ld32u(s + p));
+                for (int w2 = (int)w - 1; w2 >= 0; --w2) {
+                    const uint32_t u = smem[(uint32_t)w2 * L6_TAB + h];
+                    if (u) {
+                        C1 = u;
+                        break;
+                    }
+                }
+            }
+            if (C1 && p - (C1 - 1u) <= 32768u) step = p - (C1 - 1u);
+        }
+        PD[p] = (uint16_t)step;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// M(x) for every position: LDS rings of bytes (64 KiB) and chain steps.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ring_u8(const uint32_t* R, uint32_t a) {
+    return (R[(a >> 2) & (L6M_DRING / 4 - 1)] >> (8 * (a & 3u))) & 0xFFu;
+}
+__device__ __forceinline__ uint32_t ring_u32(const uint32_t* R, uint32_t a) {
+    const uint32_t i = a >> 2, k = a & 3u;
+    constexpr uint32_t M = L6M_DRING / 4 - 1;
+    return __builtin_amdgcn_alignbyte(R[(i + 1) & M], R[i & M], k);
+}
+__device__ __forceinline__ uint64_t ring_u64(const uint32_t* R, uint32_t a) {
+    const uint32_t i = a >> 2, k = a & 3u;
+    constexpr uint32_t M = L6M_DRING / 4 - 1;
+    const uint32_t w0 = R[i & M], w1 = R[(i + 1) & M], w2 = R[(i + 2) & M];
+    return (uint64_t)__builtin_amdgcn_alignbyte(w1, w0, k) | ((uint64_t)__builtin_amdgcn_alignbyte(w2, w1, k) << 32);
+}
+
+__global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
+                                                               uint32_t chunk, const uint16_t* __restrict__ pd,
+                                                               uint32_t* __restrict__ mlw, int level) {
+    extern __shared__ uint32_t smem[];
+    uint32_t* DR = smem;                                            // byte ring
+    uint16_t* PR = reinterpret_cast<uint16_t*>(smem + L6M_DRING / 4);  // chain-step ring
+    const uint32_t c = blockIdx.x;
+    const uint64_t base = (uint64_t)c * chunk;
+    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
+    const uint8_t* s = src + base;
+    const uint64_t cp = (chunk + 7u) & ~7u;
+    const uint16_t* PD = pd + (uint64_t)c * cp;
+    uint32_t* ML = mlw + (uint64_t)c * cp;
+    const int tid = threadIdx.x;
+    const int max_chain = c_l6lv[level].max_chain, nice = c_l6lv[level].nice_len;
+    uint32_t dfill = 0;  // bytes [0, dfill) have been loaded into the ring (word granular)
+    for (uint32_t x0 = 0; x0 < n; x0 += L6M_TILE) {
+        const uint32_t xe = min(n, x0 + L6M_TILE);
+        // bytes up to xe + 258 + 8 (rounded to words), chain steps of [x0, xe)
+        const uint32_t dneed = min((n + 3u) & ~3u, (xe + 258u + 8u + 3u) & ~3u);
+        for (uint32_t a = dfill + 4u * (uint32_t)tid; a < dneed; a += 4u * L6M_THREADS) {
+            uint32_t v;
+            if (a + 4 <= n) {
+                v = ld32u(s + a);
+            } else {
+                v = 0;
+                for (uint32_t k = 0; k < 4; ++k)
+                    if (a + k < n) v |= (uint32_t)s[a + k] << (8 * k);
+            }
+            DR[(a >> 2) & (L6M_DRING / 4 - 1)] = v;
+        }
+        dfill = max(dfill, dneed);
+        for (uint32_t q = x0 + tid; q < xe; q += L6M_THREADS) PR[q % L6M_PRING] = PD[q];
+        __syncthreads();
+        for (uint32_t x = x0 + tid; x < xe; x += L6M_THREADS) {
+            uint32_t best = 0, bd = 0;
+            if (x + 4 <= n) {
+                const uint32_t cap = min(258u, n - x);
+                const uint32_t w = ring_u32(DR, x);
+                uint32_t cur = x;
+                for (int k = 0; k < max_chain; ++k) {
+                    const uint32_t st = PR[cur % L6M_PRING];
+                    if (st == 0) break;
+                    const uint32_t q = cur - st;
+                    if (x - q > 32768u) break;
+                    if (ring_u32(DR, q) == w) {
+                        uint32_t l = 4;
+                        while (l < cap) {
+                            if (l + 8 <= cap) {
+                                const uint64_t df = ring_u64(DR, x + l) ^ ring_u64(DR, q + l);
+                                if (df) {
+                                    l += (uint32_t)(__builtin_ctzll(df) >> 3);
+                                    break;
+                                }
+                                l += 8;
+                            } else {
+                                if (ring_u8(DR, x + l) != ring_u8(DR, q + l)) break;
+                                ++l;
+                            }
+                        }
+                        l = min(l, cap);
+                        if (l > best) {
+                            best = l;
+                            bd = x - q;
+                            if ((int)l >= nice) break;
+                        }
+                    }
+                    cur = q;
+                }
+            }
+            ML[x] = best >= 3 ? (best | (bd << 16)) : 0u;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lazy decisions, parse barriers, exact parse, tokens.
+// Decision at a parse position p (lz77.cpp:442-494 with the probe at
+// pos + 1 being M(pos + 1)): literal when p + 3 > n or M(p) < 3; otherwise
+// up to lazy_depth literals while M(q + 1) > best + 1, then the match.
+// code[p] = step (next decision - p) | lazy literals << 12.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t l6_decide(const uint32_t* ML, uint32_t n, uint32_t p, int lazy) {
+    if (p + 3 > n) return 1;
+    uint32_t b = ml_len(ML[p]);
+    if (b < 3) return 1;
+    uint32_t q = p;
+    int k = lazy;
+    while (k > 0 && q + b + 1 < n && q + 4 <= n) {
+        const uint32_t nb = ml_len(ML[q + 1]);
+        if (nb > b + 1) {
+            ++q;
+            b = nb;
+            --k;
+        } else {
+            break;
+        }
+    }
+    return (q + b - p) | ((q - p) << 12);
+}
+
+__global__ __launch_bounds__(L6R_THREADS) void l6_parse_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
+                                                               uint32_t chunk, const uint32_t* __restrict__ mlw,
+                                                               uint16_t* __restrict__ codew,
+                                                               uint32_t* __restrict__ toks,
+                                                               uint32_t* __restrict__ ntok_out,
+                                                               uint32_t* __restrict__ blkpos, uint32_t maxblk,
+                                                               int level) {
+    extern __shared__ uint32_t smem[];
+    uint32_t* bm = smem;
+    uint32_t* fb = smem + L6R_BM_WORDS;
+    uint32_t* rcnt = fb + L6R_THREADS;
+    uint32_t* misc = rcnt + L6R_THREADS;
+    const uint32_t c = blockIdx.x;
+    const uint64_t base = (uint64_t)c * chunk;
+    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
+    const uint8_t* s = src + base;
+    const uint64_t cp = (chunk + 7u) & ~7u;
+    const uint32_t* ML = mlw + (uint64_t)c * cp;
+    uint16_t* CODE = codew + (uint64_t)c * cp;
+    uint32_t* TK = toks + (uint64_t)c * cp;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lazy = c_l6lv[level].lazy_depth;
+    const uint32_t S = max(32u, (((n + L6R_THREADS - 1) / L6R_THREADS) + 31u) & ~31u);
+    const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
+    // bitmap words owned by this range (S is a multiple of 32; an empty range
+    // owns none, even when r0 falls inside the last range's final word)
+    const uint32_t w0 = r0 >> 5, w1 = r0 < r1 ? (r1 + 31) >> 5 : w0;
+    // decisions (coalesced over positions) and bitmap clear
+    for (uint32_t p = tid; p < n; p += L6R_THREADS) CODE[p] = (uint16_t)l6_decide(ML, n, p, lazy);
+    for (uint32_t w = w0; w < w1; ++w) bm[w] = 0;
+    __syncthreads();
+    // first parse barrier per range: max over p < q of (p + step(p)) == q
+    {
+        uint32_t runmax = 0;
+        for (uint32_t p = r0; p < r1; ++p) runmax = max(runmax, p + (CODE[p] & 0xFFFu));
+        uint32_t v = runmax;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(v, d, 64);
+            if (lane >= d) v = max(v, t);
+        }
+        if (lane == 63) misc[wv] = v;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (int i = 0; i < L6R_WAVES; ++i) {
+                const uint32_t t = misc[i];
+                misc[i] = acc;
+                acc = max(acc, t);
+            }
+        }
+        __syncthreads();
+        const uint32_t prev_lane = __shfl_up(v, 1, 64);
+        uint32_t carry = max(misc[wv], lane ? prev_lane : 0u);
+        if (tid == 0) carry = 0;
+        uint32_t first = 0xFFFFFFFFu;
+        for (uint32_t p = r0; p < r1; ++p) {
+            if (carry <= p) {
+                first = p;
+                break;
+            }
+            carry = max(carry, p + (CODE[p] & 0xFFFu));
+        }
+        fb[tid] = r0 < r1 ? first : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    // exact parse from this range's barrier to the next range's barrier
+    {
+        const uint32_t pstart = fb[tid];
+        if (pstart != 0xFFFFFFFFu) {
+            uint32_t pos = pstart;
+            while (pos < n) {
+                if (pos >= r1 && fb[pos / S] == pos) break;
+                atomicOr(&bm[pos >> 5], 1u << (pos & 31));
+                pos += CODE[pos] & 0xFFFu;
+            }
+        }
+    }
+    __syncthreads();
+    // tokens per range (decision at p: lazy literals, then a match or one literal)
+    uint32_t cnt = 0;
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint32_t x = bm[w];
+        while (x) {
+            const uint32_t p = (w << 5) + (uint32_t)__builtin_ctz(x);
+            x &= x - 1;
+            const uint32_t cd = CODE[p];
+            cnt += 1u + (cd >> 12);
+        }
+    }
+    const uint32_t incl = wave_incl_scan(cnt, lane);
+    if (lane == 63) misc[wv] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < L6R_WAVES; ++i) {
+            const uint32_t t = misc[i];
+            misc[i] = acc;
+            acc += t;
+        }
+        ntok_out[c] = acc;
+    }
+    __syncthreads();
+    uint32_t ti = misc[wv] + incl - cnt;
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint32_t x = bm[w];
+        while (x) {
+            const uint32_t p = (w << 5) + (uint32_t)__builtin_ctz(x);
+            x &= x - 1;
+            const uint32_t cd = CODE[p];
+            const uint32_t lits = cd >> 12, step = cd & 0xFFFu;
+            for (uint32_t k = 0; k < lits; ++k) {
+                if ((ti % BLK_SYMS) == 0 && ti / BLK_SYMS < maxblk) blkpos[(uint64_t)c * maxblk + ti / BLK_SYMS] = p + k;
+                TK[ti++] = s[p + k];
+            }
+            if ((ti % BLK_SYMS) == 0 && ti / BLK_SYMS < maxblk) blkpos[(uint64_t)c * maxblk + ti / BLK_SYMS] = p + lits;
+            if (step > lits + 1) {
+                const uint32_t m = ML[p + lits];
+                TK[ti++] = 0x80000000u | (ml_len(m) << 16) | ml_dist(m);
+            } else {
+                TK[ti++] = s[p + lits];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blocks: histograms, package-merge lengths, header, sizes, block type.
+// ---------------------------------------------------------------------------
+struct BlockRec {
+    uint32_t t0, t1;        // token range
+    uint32_t raw_pos, raw_len;
+    uint32_t type;          // 0 stored, 1 fixed, 2 dynamic
+    uint32_t hlit, hdist, hclen, nrle;
+    uint32_t hdr_bits;      // dynamic: header bits after the 3-bit block header
+    uint64_t body_bits;     // fixed / dynamic: bits of header + symbols + EOB
+    uint8_t ll[288];
+    uint8_t dl[32];
+    uint8_t cl[19];
+    uint8_t rle_sym[320];
+    uint8_t rle_x[320];
+};
+
+__constant__ uint8_t c_len_extra[29] = {0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2,
+                                        2, 3, 3, 3, 3, 4, 4, 4, 4, 5, 5, 5, 5, 0};
+__constant__ uint8_t c_dist_extra[30] = {0, 0, 0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 6,
+                                         6, 7, 7, 8, 8, 9, 9, 10, 10, 11, 11, 12, 12, 13, 13};
+__constant__ uint8_t c_cl_order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
+
+__device__ __forceinline__ uint32_t fixed_ll_len(uint32_t sym) {
+    return sym < 144 ? 8u : (sym < 256 ? 9u : (sym < 280 ? 7u : 8u));
+}
+
+// Package-merge (Larmore & Hirschberg), same order as the oracle: symbols
+// sorted by (freq, index), a leaf precedes a package of equal weight.  One
+// thread; idx / wprev / wcur / leafbits are LDS scratch.
+__device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, uint8_t* lens, uint16_t* idx,
+                                uint32_t* wprev, uint32_t* wcur, uint8_t* leafbits /* [16][576] */) {
+    int n = 0;
+    for (int i = 0; i < nsym; ++i) {
+        lens[i] = 0;
+        if (freq[i]) idx[n++] = (uint16_t)i;
+    }
+    if (n == 0) return;
+    if (n == 1) {
+        lens[idx[0]] = 1;
+        return;
+    }
+    for (int i = 1; i < n; ++i) {
+        const uint16_t v = idx[i];
+        int j = i - 1;
+        while (j >= 0 && freq[idx[j]] > freq[v]) {
+            idx[j + 1] = idx[j];
+            --j;
+        }
+        idx[j + 1] = v;
+    }
+    for (int i = 0; i < n; ++i) {
+        wprev[i] = freq[idx[i]];
+        leafbits[max_bits * 576 + i] = 1;
+    }
+    int cntp = n;
+    for (int l = max_bits - 1; l >= 1; --l) {
+        const int np = cntp / 2;
+        int a = 0, p = 0, k = 0;
+        while (a < n || p < np) {
+            const uint32_t pw = p < np ? wprev[2 * p] + wprev[2 * p + 1] : 0xFFFFFFFFu;
+            if (a < n && freq[idx[a]] <= pw) {
+                wcur[k] = freq[idx[a++]];
+                leafbits[l * 576 + k++] = 1;
+            } else {
+                wcur[k] = pw;
+                leafbits[l * 576 + k++] = 0;
+                ++p;
+            }
+        }
+        for (int i = 0; i < k; ++i) wprev[i] = wcur[i];
+        cntp = k;
+    }
+    int take = 2 * n - 2;
+    for (int l = 1; l <= max_bits && take > 0; ++l) {
+        int leaves = 0;
+        for (int i = 0; i < take; ++i) leaves += leafbits[l * 576 + i];
+        for (int i = 0; i < leaves; ++i) lens[idx[i]]++;
+        take = 2 * (take - leaves);
+    }
+}
+
+__global__ __launch_bounds__(L6B_THREADS) void l6_block_kernel(const uint32_t* __restrict__ toks,
+                                                               const uint32_t* __restrict__ ntok_in,
+                                                               const uint32_t* __restrict__ blkpos, uint32_t chunk,
+                                                               uint64_t n_total, uint32_t maxblk,
+                                                               BlockRec* __restrict__ recs) {
+    __shared__ uint32_t lit[288];
+    __shared__ uint32_t dst[32];
+    __shared__ uint32_t rawsum;
+    __shared__ uint16_t idx[288];
+    __shared__ uint32_t wprev[576];
+    __shared__ uint32_t wcur[576];
+    __shared__ uint8_t leafbits[16 * 576];
+    __shared__ BlockRec R;
+    const uint32_t c = blockIdx.x / maxblk, b = blockIdx.x % maxblk;
+    const uint32_t ntok = ntok_in[c];
+    const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
+    if (b >= nblk) return;
+    const uint64_t cp = (chunk + 7u) & ~7u;
+    const uint32_t* TK = toks + (uint64_t)c * cp;
+    const uint32_t t0 = b * BLK_SYMS, t1 = min(ntok, t0 + BLK_SYMS);
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 288; i += L6B_THREADS) lit[i] = 0;
+    if (tid < 32) dst[tid] = 0;
+    if (tid == 0) rawsum = 0;
+    __syncthreads();
+    uint32_t raw = 0;
+    for (uint32_t t = t0 + tid; t < t1; t += L6B_THREADS) {
+        const uint32_t v = TK[t];
+        if (v & 0x80000000u) {
+            const uint32_t L = (v >> 16) & 0x1FFu, d = v & 0xFFFFu;
+            atomicAdd(&lit[257 + len_sym(L).code], 1u);
+            atomicAdd(&dst[dist_sym(d).code], 1u);
+            raw += L;
+        } else {
+            atomicAdd(&lit[v & 0xFFu], 1u);
+            raw += 1;
+        }
+    }
+    atomicAdd(&rawsum, raw);
+    __syncthreads();
+    if (tid == 0) {
+        lit[256] += 1;  // EOB
+        R.t0 = t0;
+        R.t1 = t1;
+        R.raw_len = rawsum;
+        R.raw_pos = b == 0 ? 0u : blkpos[(uint64_t)c * maxblk + b];
+        // lengths
+        limited_lengths(lit, 288, 15, R.ll, idx, wprev, wcur, leafbits);
+        limited_lengths(dst, 32, 15, R.dl, idx, wprev, wcur, leafbits);
+        // fixed and dynamic sizes (deflate_block.cpp:50-92 for the fixed estimate)
+        uint64_t fixedb = 3, datab = 0;
+        for (int i = 0; i < 288; ++i) {
+            fixedb += (uint64_t)fixed_ll_len(i) * lit[i];
+            datab += (uint64_t)R.ll[i] * lit[i];
+        }
+        for (int i = 0; i < 32; ++i) {
+            fixedb += 5ull * dst[i];
+            datab += (uint64_t)R.dl[i] * dst[i];
+        }
+        uint64_t extra = 0;
+        for (int i = 0; i < 29; ++i) extra += (uint64_t)c_len_extra[i] * lit[257 + i];
+        for (int i = 0; i < 30; ++i) extra += (uint64_t)c_dist_extra[i] * dst[i];
+        fixedb += extra;
+        datab += extra;
+        // dynamic header: EOB / distance fix-ups, trimmed counts, code-length RLE
+        // (deflate_block.cpp:196-290, 347-370)
+        if (!R.ll[256]) R.ll[256] = 1;
+        bool any = false;
+        for (int i = 0; i < 32; ++i) any |= R.dl[i] != 0;
+        if (!any) R.dl[0] = 1;
+        int hlit = 286;
+        while (hlit > 257 && !R.ll[hlit - 1]) --hlit;
+        int hdist = 30;
+        while (hdist > 1 && !R.dl[hdist - 1]) --hdist;
+        const int total = hlit + hdist;
+        int nr = 0;
+        for (int i = 0; i < total;) {
+            const uint32_t sym = i < hlit ? R.ll[i] : R.dl[i - hlit];
+            int run = 1;
+            while (i + run < total && run < 138) {
+                const uint32_t nx = (i + run) < hlit ? R.ll[i + run] : R.dl[i + run - hlit];
+                if (nx != sym) break;
+                ++run;
+            }
+            if (sym == 0 && run >= 3) {
+                R.rle_sym[nr] = run <= 10 ? 17 : 18;
+                R.rle_x[nr++] = (uint8_t)(run <= 10 ? run - 3 : run - 11);
+                i += run;
+            } else if (sym != 0 && run >= 4) {
+                R.rle_sym[nr] = (uint8_t)sym;
+                R.rle_x[nr++] = 0xFF;
+                const int rep = run - 1 < 6 ? run - 1 : 6;
+                R.rle_sym[nr] = 16;
+                R.rle_x[nr++] = (uint8_t)(rep - 3);
+                i += 1 + rep;
+            } else {
+                R.rle_sym[nr] = (uint8_t)sym;
+                R.rle_x[nr++] = 0xFF;
+                ++i;
+            }
+        }
+        uint32_t clf[19];
+        for (int i = 0; i < 19; ++i) clf[i] = 0;
+        for (int i = 0; i < nr; ++i) clf[R.rle_sym[i]]++;
+        limited_lengths(clf, 19, 7, R.cl, idx, wprev, wcur, leafbits);
+        int act = 0;
+        for (int i = 0; i < 19; ++i) act += R.cl[i] != 0;
+        if (act == 1) {
+            for (int i = 0; i < 19; ++i)
+                if (!R.cl[i]) {
+                    R.cl[i] = 1;
+                    break;
+                }
+        }
+        int hclen = 19;
+        while (hclen > 4 && !R.cl[c_cl_order[hclen - 1]]) --hclen;
+        uint32_t hb = 5 + 5 + 4 + 3u * (uint32_t)hclen;
+        for (int i = 0; i < nr; ++i) {
+            const int sy = R.rle_sym[i];
+            hb += R.cl[sy] + (sy == 16 ? 2u : (sy == 17 ? 3u : (sy == 18 ? 7u : 0u)));
+        }
+        // data bits with the fixed-up lengths (EOB / dist[0] may have changed)
+        uint64_t dyn_data = extra;
+        for (int i = 0; i < 288; ++i) dyn_data += (uint64_t)R.ll[i] * lit[i];
+        for (int i = 0; i < 32; ++i) dyn_data += (uint64_t)R.dl[i] * dst[i];
+        const uint64_t dynb = 3ull + hb + dyn_data;
+        const uint64_t storedb = ((uint64_t)R.raw_len + 5) * 8;
+        R.hlit = (uint32_t)hlit;
+        R.hdist = (uint32_t)hdist;
+        R.hclen = (uint32_t)hclen;
+        R.nrle = (uint32_t)nr;
+        R.hdr_bits = hb;
+        if (storedb <= fixedb && storedb <= dynb) {
+            R.type = 0;
+            R.body_bits = 0;
+        } else if (fixedb <= dynb) {
+            R.type = 1;
+            R.body_bits = fixedb;
+        } else {
+            R.type = 2;
+            R.body_bits = dynb;
+        }
+        (void)datab;
+    }
+    __syncthreads();
+    // copy the record out
+    const uint32_t* srcw = reinterpret_cast<const uint32_t*>(&R);
+    uint32_t* dstw = reinterpret_cast<uint32_t*>(&recs[(uint64_t)c * maxblk + b]);
+    for (uint32_t i = tid; i < sizeof(BlockRec) / 4; i += L6B_THREADS) dstw[i] = srcw[i];
+}
+
+__device__ __forceinline__ uint64_t block_bits(const BlockRec& r, uint64_t off) {
+    return r.type == 0 ? stored_bits(off, r.raw_len) : r.body_bits;
+}
+
+// chunk size for each incoming bit offset mod 8
+__global__ void l6_size_kernel(const uint32_t* __restrict__ ntok_in, const BlockRec* __restrict__ recs,
+                               uint32_t nchunks, uint32_t chunk, uint64_t n_total, uint32_t maxblk,
+                               ChunkInfo* __restrict__ info) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks) return;
+    const uint32_t ntok = ntok_in[c];
+    const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
+    const BlockRec* R = recs + (uint64_t)c * maxblk;
+    for (uint32_t r = 0; r < 8; ++r) {
+        uint64_t off = r;
+        for (uint32_t b = 0; b < nblk; ++b) off += block_bits(R[b], off);
+        info[c].sz8[r] = off - r;
+    }
+    info[c].is_stored = 2;
+    info[c].raw_len = (uint32_t)min<uint64_t>(chunk, n_total - (uint64_t)c * chunk);
+    info[c].fixed_bits = 0;
+}
+
+__global__ __launch_bounds__(256) void l6_zero_kernel(const ChunkInfo* __restrict__ info, uint32_t* __restrict__ stage,
+                                                      uint64_t stage_words) {
+    const uint32_t c = blockIdx.x;
+    const uint64_t bits = info[c].sz8[info[c].bit_off & 7];
+    const uint64_t nw = min<uint64_t>(stage_words, (bits + 31) / 32 + 2);
+    uint32_t* stg = stage + (uint64_t)c * stage_words;
+    for (uint64_t i = threadIdx.x; i < nw; i += 256) stg[i] = 0;
+}
+
+// bits [at, at + len) of the chunk staging get v (len <= 32)
+__device__ __forceinline__ void put_bits_at(uint32_t* stg, uint64_t at, uint32_t v, uint32_t len) {
+    if (!len) return;
+    const uint64_t w = at >> 5;
+    const uint32_t sh = (uint32_t)(at & 31);
+    const uint64_t vv = (uint64_t)(len == 32 ? v : (v & ((1u << len) - 1u))) << sh;
+    atomicOr(&stg[w], (uint32_t)vv);
+    if (sh + len > 32) atomicOr(&stg[w + 1], (uint32_t)(vv >> 32));
+}
+
+__device__ __forceinline__ uint32_t rev_code(uint32_t code, uint32_t len) {
+    return len ? __builtin_bitreverse32(code) >> (32 - len) : 0u;
+}
+
+constexpr int L6E_THREADS = 256;
+
+__global__ __launch_bounds__(L6E_THREADS) void l6_emit_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
+                                                              uint32_t chunk, uint32_t nchunks,
+                                                              const uint32_t* __restrict__ toks,
+                                                              const uint32_t* __restrict__ ntok_in,
+                                                              const BlockRec* __restrict__ recs, uint32_t maxblk,
+                                                              const ChunkInfo* __restrict__ info,
+                                                              const Result* __restrict__ res,
+                                                              uint32_t* __restrict__ stage, uint64_t stage_words) {
+    __shared__ uint16_t llc[288];
+    __shared__ uint16_t dlc[32];
+    __shared__ uint8_t lls[288];
+    __shared__ uint8_t dls[32];
+    __shared__ uint32_t wsum[L6E_THREADS / 64 + 1];
+    __shared__ uint64_t s_off;
+    __shared__ uint64_t s_sym;
+    if (res->total_bytes == 0) return;
+    const uint32_t c = blockIdx.x / maxblk, b = blockIdx.x % maxblk;
+    const uint32_t ntok = ntok_in[c];
+    const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
+    if (b >= nblk) return;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const BlockRec* RR = recs + (uint64_t)c * maxblk;
+    const BlockRec& R = RR[b];
+    const uint64_t cp = (chunk + 7u) & ~7u;
+    const uint32_t* TK = toks + (uint64_t)c * cp;
+    const uint8_t* s = src + (uint64_t)c * chunk;
+    uint32_t* stg = stage + (uint64_t)c * stage_words;
+    const bool last = (c == nchunks - 1) && (b == nblk - 1);
+    if (tid == 0) {
+        const uint64_t r = info[c].bit_off & 7;
+        uint64_t off = r;
+        for (uint32_t k = 0; k < b; ++k) off += block_bits(RR[k], off);
+        s_off = off - r;  // relative to the chunk start
+    }
+    __syncthreads();
+    const uint64_t off = s_off;
+    const uint64_t abs_res = info[c].bit_off & 7;
+    if (R.type == 0) {
+        // stored pieces (deflate_block.cpp:107-132), bytes written 4 at a time
+        const uint32_t L = R.raw_len;
+        const uint32_t pieces = L == 0 ? 1u : (L + 65534u) / 65535u;
+        const uint64_t pad0 = (8 - ((abs_res + off + 3) & 7)) & 7;
+        uint64_t at = off + 3 + pad0;  // first byte-aligned bit of piece 0 (after its header)
+        for (uint32_t k = 0; k < pieces; ++k) {
+            const uint32_t Lk = min(L - k * 65535u, 65535u);
+            const bool lastp = last && k == pieces - 1;
+            const uint64_t hdr_at = k == 0 ? off : at;
+            if (tid == 0) {
+                put_bits_at(stg, hdr_at, lastp ? 1u : 0u, 3);
+                const uint64_t la = k == 0 ? at : at + 8;
+                put_bits_at(stg, la, (Lk & 0xFFFFu) | ((~Lk & 0xFFFFu) << 16), 32);
+            }
+            const uint64_t da = (k == 0 ? at : at + 8) + 32;
+            const uint8_t* rp = s + R.raw_pos + (uint64_t)k * 65535u;
+            for (uint32_t i = 4u * (uint32_t)tid; i < Lk; i += 4u * L6E_THREADS) {
+                uint32_t v = 0;
+                const uint32_t m = min(4u, Lk - i);
+                for (uint32_t j = 0; j < m; ++j) v |= (uint32_t)rp[i + j] << (8 * j);
+                put_bits_at(stg, da + 8ull * i, v, 8 * m);
+            }
+            at = da + 8ull * Lk;
+        }
+        return;
+    }
+    // fixed or dynamic: code tables
+    if (tid == 0) {
+        for (int i = 0; i < 288; ++i) lls[i] = R.type == 1 ? (uint8_t)fixed_ll_len(i) : R.ll[i];
+        for (int i = 0; i < 32; ++i) dls[i] = R.type == 1 ? 5 : R.dl[i];
+        // canonical codes (RFC 1951 3.2.2), bit-reversed
+        for (int tbl = 0; tbl < 2; ++tbl) {
+            const uint8_t* ln = tbl ? dls : lls;
+            uint16_t* cd = tbl ? dlc : llc;
+            const int ns = tbl ? 32 : 288;
+            uint32_t blc[16] = {0}, nxt[16] = {0};
+            for (int i = 0; i < ns; ++i)
+                if (ln[i]) blc[ln[i]]++;
+            uint32_t code = 0;
+            for (int bb = 1; bb <= 15; ++bb) {
+                code = (code + blc[bb - 1]) << 1;
+                nxt[bb] = code;
+            }
+            for (int i = 0; i < ns; ++i) cd[i] = ln[i] ? (uint16_t)rev_code(nxt[ln[i]]++, ln[i]) : 0;
+        }
+        uint64_t at = off;
+        put_bits_at(stg, at, (last ? 1u : 0u) | (R.type << 1), 3);
+        at += 3;
+        if (R.type == 2) {
+            put_bits_at(stg, at, R.hlit - 257, 5);
+            put_bits_at(stg, at + 5, R.hdist - 1, 5);
+            put_bits_at(stg, at + 10, R.hclen - 4, 4);
+            at += 14;
+            for (uint32_t i = 0; i < R.hclen; ++i) {
+                put_bits_at(stg, at, R.cl[c_cl_order[i]], 3);
+                at += 3;
+            }
+            uint32_t clb[16] = {0}, cln[16] = {0}, clc[19];
+            for (int i = 0; i < 19; ++i)
+                if (R.cl[i]) clb[R.cl[i]]++;
+            uint32_t code = 0;
+            for (int bb = 1; bb <= 15; ++bb) {
+                code = (code + clb[bb - 1]) << 1;
+                cln[bb] = code;
+            }
+            for (int i = 0; i < 19; ++i) clc[i] = R.cl[i] ? rev_code(cln[R.cl[i]]++, R.cl[i]) : 0u;
+            for (uint32_t i = 0; i < R.nrle; ++i) {
+                const uint32_t sy = R.rle_sym[i];
+                put_bits_at(stg, at, clc[sy], R.cl[sy]);
+                at += R.cl[sy];
+                const uint32_t xb = sy == 16 ? 2u : (sy == 17 ? 3u : (sy == 18 ? 7u : 0u));
+                if (xb) {
+                    put_bits_at(stg, at, R.rle_x[i], xb);
+                    at += xb;
+                }
+            }
+        }
+        s_sym = at;
+    }
+    __syncthreads();
+    const uint64_t sym0 = s_sym;
+    // symbols: each thread a contiguous slice of the block's tokens
+    const uint32_t nt = R.t1 - R.t0;
+    const uint32_t per = (nt + L6E_THREADS - 1) / L6E_THREADS;
+    const uint32_t a = R.t0 + min(nt, (uint32_t)tid * per), e = R.t0 + min(nt, (uint32_t)tid * per + per);
+    // a match takes up to 15 + 5 + 15 + 13 = 48 bits with dynamic codes
+    auto tok_code = [&](uint32_t v, uint64_t& code, uint32_t& len) {
+        if (v & 0x80000000u) {
+            const uint32_t L = (v >> 16) & 0x1FFu, d = v & 0xFFFFu;
+            const LenSym ls = len_sym(L), ds = dist_sym(d);
+            const uint32_t sy = 257 + ls.code;
+            const uint32_t l1 = lls[sy], l2 = dls[ds.code];
+            code = (uint64_t)llc[sy] | ((uint64_t)ls.extra << l1) | ((uint64_t)dlc[ds.code] << (l1 + ls.ebits)) |
+                   ((uint64_t)ds.extra << (l1 + ls.ebits + l2));
+            len = l1 + ls.ebits + l2 + ds.ebits;
+        } else {
+            code = llc[v & 0xFFu];
+            len = lls[v & 0xFFu];
+        }
+    };
+    uint32_t mybits = 0;
+    for (uint32_t t = a; t < e; ++t) {
+        uint64_t cd;
+        uint32_t ln;
+        tok_code(TK[t], cd, ln);
+        mybits += ln;
+    }
+    const uint32_t incl = wave_incl_scan(mybits, lane);
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < L6E_THREADS / 64; ++i) {
+            const uint32_t t = wsum[i];
+            wsum[i] = acc;
+            acc += t;
+        }
+        wsum[L6E_THREADS / 64] = acc;
+    }
+    __syncthreads();
+    uint64_t at = sym0 + wsum[wv] + incl - mybits;
+    for (uint32_t t = a; t < e; ++t) {
+        uint64_t cd;
+        uint32_t ln;
+        tok_code(TK[t], cd, ln);
+        put_bits_at(stg, at, (uint32_t)cd, min(ln, 32u));
+        if (ln > 32) put_bits_at(stg, at + 32, (uint32_t)(cd >> 32), ln - 32);
+        at += ln;
+    }
+    if (tid == 0) put_bits_at(stg, sym0 + wsum[L6E_THREADS / 64], llc[256], lls[256]);  // EOB
+}
+
+template <bool EXCH>
+hipError_t launch_pred(const Plan& p) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l6_pred_kernel<EXCH>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)L6P_LDS);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((l6_pred_kernel<EXCH>), dim3(p.nchunks), dim3(L6P_THREADS), L6P_LDS, p.stream, p.d_in, p.n,
+                       p.chunk, p.d_tok, p.d_pd);
+    return hipSuccess;
+}
+
+}  // namespace
+
+uint32_t l6_max_blocks(uint32_t chunk) { return (chunk + BLK_SYMS - 1) / BLK_SYMS; }
+uint32_t l6_max_chunk() { return L6_MAX_CHUNK; }
+size_t l6_block_record_bytes() { return sizeof(BlockRec); }
+
+// LZ77 and block analysis of the L6-class path (before the offset scan).
+hipError_t launch_l6_front(const Plan& p) {
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l6_match_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)L6M_LDS);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l6_parse_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)L6R_LDS);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    const uint32_t maxblk = l6_max_blocks(p.chunk);
+    hipError_t e = p.exch_ok ? launch_pred<true>(p) : launch_pred<false>(p);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(l6_match_kernel, dim3(p.nchunks), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.n, p.chunk,
+                       p.d_pd, p.d_tok, p.level);
+    hipLaunchKernelGGL(l6_parse_kernel, dim3(p.nchunks), dim3(L6R_THREADS), L6R_LDS, p.stream, p.d_in, p.n, p.chunk,
+                       p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.level);
+    hipLaunchKernelGGL(l6_block_kernel, dim3(p.nchunks * maxblk), dim3(L6B_THREADS), 0, p.stream, p.d_aux, p.d_ntok,
+                       p.d_blkpos, p.chunk, p.n, maxblk, reinterpret_cast<BlockRec*>(p.d_blocks));
+    hipLaunchKernelGGL(l6_size_kernel, dim3((p.nchunks + 255) / 256), dim3(256), 0, p.stream, p.d_ntok,
+                       reinterpret_cast<const BlockRec*>(p.d_blocks), p.nchunks, p.chunk, p.n, maxblk, p.d_info);
+    return hipGetLastError();
+}
+
+// Emission of the L6-class path (after the offset scan).
+hipError_t launch_l6_back(const Plan& p) {
+    const uint32_t maxblk = l6_max_blocks(p.chunk);
+    hipLaunchKernelGGL(l6_zero_kernel, dim3(p.nchunks), dim3(256), 0, p.stream, p.d_info, p.d_stage, p.stage_words);
+    hipLaunchKernelGGL(l6_emit_kernel, dim3(p.nchunks * maxblk), dim3(L6E_THREADS), 0, p.stream, p.d_in, p.n, p.chunk,
+                       p.nchunks, p.d_aux, p.d_ntok, reinterpret_cast<const BlockRec*>(p.d_blocks), maxblk, p.d_info,
+                       p.d_result, p.d_stage, p.stage_words);
+    return hipGetLastError();
+}
+
+}  // namespace orot_gpu

@@ -39,6 +39,10 @@ def oracle() -> ctypes.CDLL:
         L.oracle_l1_candidates.argtypes = [_vp, _sz, ctypes.c_int, _vp]
         L.oracle_parallel_compress.restype = _sz
         L.oracle_parallel_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz, _vp, _sz, _vp, _sz]
+        L.oracle_gpu_parallel_compress.restype = _sz
+        L.oracle_gpu_parallel_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz, _vp, _sz, _vp, _sz]
+        L.oracle_l6_lz77.restype = _sz
+        L.oracle_l6_lz77.argtypes = [_vp, _sz, ctypes.c_int, _vp]
         L.oracle_encode_block_fixed.restype = ctypes.c_int
         L.oracle_encode_block_fixed.argtypes = [_vp, _sz, _vp, _sz, ctypes.c_int, _vp, _vp]
         L.oracle_limited_lengths.restype = ctypes.c_int
@@ -105,6 +109,26 @@ def parallel_compress(data, level: int, fmt: int, block_size: int = 0, threads: 
     if n == 0:
         raise RuntimeError("oracle_parallel_compress failed")
     return out[:n].tobytes()
+
+
+def gpu_parallel_compress(data, level: int, fmt: int, block_size: int = 0, threads: int = 1) -> bytes:
+    """What the HIP path computes (orot's bytes at levels 0-3, the L6-class
+    design at levels 4-12)."""
+    a = _u8(data)
+    cap = a.size + a.size // 8 + 4096
+    out = np.zeros(cap, dtype=np.uint8)
+    n = oracle().oracle_gpu_parallel_compress(level, fmt, threads, block_size, a.ctypes.data, a.size,
+                                              out.ctypes.data, cap)
+    if n == 0:
+        raise RuntimeError("oracle_gpu_parallel_compress failed")
+    return out[:n].tobytes()
+
+
+def l6_lz77(data, level: int) -> np.ndarray:
+    a = _u8(data)
+    t = np.zeros(max(1, a.size), dtype=np.uint32)
+    n = oracle().oracle_l6_lz77(a.ctypes.data, a.size, level, t.ctypes.data)
+    return t[:n]
 
 
 def limited_lengths(freq, max_bits: int) -> np.ndarray:
