@@ -36,6 +36,38 @@ from orot_amd import corpus  # noqa: E402
 SEGMENT = 64 << 20
 SEEDS = {"zeros": 0, "text": 0x5EED0001, "random": 0x5EED0002, "records": 0x5EED0003}
 HBM_PEAK = 8.0e12
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the dominant kernel on the
+# default workload, scaled by the same counters on a plain copy kernel.
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "l1_barrier_kernel_256MiB_pmc.txt")
+
+
+def measured_traffic(path: str = TRAFFIC_SUMMARY):
+    """HBM bytes per launch from the committed counter summary (its line
+    'traffic_bytes_per_launch: N'), or None when there is none."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> int:
+    """A plain elementwise kernel that reads and writes nbytes each (bitwise
+    NOT of a uint8 tensor): FETCH_SIZE / WRITE_SIZE of this kernel give the
+    counter's scale against bytes known to move."""
+    import torch
+
+    a = torch.full((nbytes,), 0x5A, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.bitwise_not(a, out=b)
+    torch.cuda.synchronize()
+    ok = bool((b[:4096] == 0xA5).all().item())
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "reps": reps, "ok": ok}), flush=True)
+    return 0 if ok else 1
 
 
 def workload() -> np.ndarray:
@@ -109,7 +141,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the copy kernel used to scale HBM counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here holds a single MI355X; "
               "batches shard across GPUs with no exchange (see the design notes).", file=sys.stderr)
@@ -187,12 +223,12 @@ def main() -> int:
         "stage_ms_last": [round(x, 4) for x in pc.stage_times()],
         "roofline": {
             "bound": "hbm",
-            "kernel": "l1_lz77_kernel",
+            "kernel": "l1_barrier_kernel",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved * 1e9 / HBM_PEAK, 5),
-            "traffic": None,
+            "traffic": measured_traffic(),
         },
     }
     if not args.no_cpu:
