@@ -75,6 +75,35 @@ def workload() -> np.ndarray:
     return np.concatenate(parts)
 
 
+def _gen(spec):
+    kind, size, seed = spec
+    return corpus.generate(kind, int(size), int(seed))
+
+
+def make_buffers(specs):
+    """Generate (kind, size, seed) buffers in a process pool.  Called before
+    anything touches the GPU, so the forked workers hold no device state."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    workers = max(1, min(cpu_threads(), 16))
+    if workers == 1 or len(specs) < 4:
+        return [_gen(s) for s in specs]
+    with ProcessPoolExecutor(workers) as ex:
+        return list(ex.map(_gen, specs, chunksize=max(1, len(specs) // (4 * workers))))
+
+
+# Level-6 workload: 1 GiB Zipf-word text plus 1 GiB 64-byte records as
+# 256 buffers of 8 MiB (even index text, odd index records), gzip.
+L6_COUNT = 256
+L6_SIZE = 8 << 20
+L6_MASTER = 0x6C360000
+
+
+def l6_specs():
+    return [("text" if i % 2 == 0 else "records", L6_SIZE, corpus.buffer_seed(L6_MASTER, i))
+            for i in range(L6_COUNT)]
+
+
 def cpu_threads() -> int:
     n = os.cpu_count() or 1
     share = os.environ.get("OMP_NUM_THREADS")
@@ -83,74 +112,126 @@ def cpu_threads() -> int:
     return max(1, n)
 
 
+class OrotCPU:
+    """orot's parallel compressor on host cores: the reference build when
+    present (kind "reference"), else the oracle's restatement (kind "port")."""
+
+    def __init__(self, level: int, fmt: int, block: int, threads: int):
+        ref_x86 = os.path.join(ROOT, "oracle", "_ref", "liborot_ref_x86.so")
+        self.level, self.fmt, self.block, self.threads = level, fmt, block, threads
+        self.ctx = None
+        if os.path.exists(ref_x86):
+            L = ctypes.CDLL(ref_x86)
+            L.deflate_parallel_new.restype = ctypes.c_void_p
+            L.deflate_parallel_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
+            L.deflate_parallel_compress.restype = ctypes.c_size_t
+            L.deflate_parallel_compress.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                    ctypes.c_void_p, ctypes.c_size_t]
+            L.deflate_parallel_free.argtypes = [ctypes.c_void_p]
+            self.L = L
+            self.ctx = L.deflate_parallel_new(level, fmt, threads, block)
+            self.kind = "reference"
+        else:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_ffi  # noqa: E402
+
+            self.O = oracle_ffi.oracle()
+            self.kind = "port"
+
+    def compress(self, data: np.ndarray, out: np.ndarray) -> int:
+        if self.ctx is not None:
+            return self.L.deflate_parallel_compress(self.ctx, data.ctypes.data, data.size, out.ctypes.data, out.size)
+        return self.O.oracle_parallel_compress(self.level, self.fmt, self.threads, self.block, data.ctypes.data,
+                                               data.size, out.ctypes.data, out.size)
+
+    def close(self):
+        if self.ctx is not None:
+            self.L.deflate_parallel_free(self.ctx)
+            self.ctx = None
+
+
 def cpu_baseline(data: np.ndarray, budget_s: float = 15.0):
-    """orot's parallel L1 path on host cores: the reference build when present,
-    else the oracle's restatement of it."""
-    ref_x86 = os.path.join(ROOT, "oracle", "_ref", "liborot_ref_x86.so")
+    """orot parallel L1, raw, 128 KiB chunks on the host cores: median of up to
+    10 runs over the whole buffer."""
     threads = cpu_threads()
-    cap = data.size + data.size // 8 + 4096
-    out = np.empty(cap, dtype=np.uint8)
-    if os.path.exists(ref_x86):
-        L = ctypes.CDLL(ref_x86)
-        L.deflate_parallel_new.restype = ctypes.c_void_p
-        L.deflate_parallel_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
-        L.deflate_parallel_compress.restype = ctypes.c_size_t
-        L.deflate_parallel_compress.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
-                                                ctypes.c_void_p, ctypes.c_size_t]
-        L.deflate_parallel_free.argtypes = [ctypes.c_void_p]
-        ctx = L.deflate_parallel_new(1, 0, threads, 131072)
-
-        def run():
-            return L.deflate_parallel_compress(ctx, data.ctypes.data, data.size, out.ctypes.data, cap)
-
-        kind = "reference"
-    else:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_ffi  # noqa: E402
-
-        O = oracle_ffi.oracle()
-
-        def run():
-            return O.oracle_parallel_compress(1, 0, threads, 131072, data.ctypes.data, data.size, out.ctypes.data, cap)
-
-        kind = "port"
-        ctx = None
-    run()  # warm-up
+    cpu = OrotCPU(1, 0, 131072, threads)
+    out = np.empty(data.size + data.size // 8 + 4096, dtype=np.uint8)
+    cpu.compress(data, out)  # warm-up
     times = []
     t_start = time.perf_counter()
     while len(times) < 10 and (time.perf_counter() - t_start) < budget_s:
         t0 = time.perf_counter()
-        clen = run()
+        clen = cpu.compress(data, out)
         times.append(time.perf_counter() - t0)
-    if ctx is not None:
-        L.deflate_parallel_free(ctx)
+    cpu.close()
     med = statistics.median(times)
     return {
         "value": round(data.size / med / 1e9, 4),
         "unit": "GB/s",
         "cores": threads,
-        "kind": kind,
+        "kind": cpu.kind,
         "sample": f"orot parallel L1 raw, 128 KiB chunks, the full {data.size >> 20} MiB 4-segment buffer, "
                   f"median of {len(times)} runs after 1 warm-up ({clen} B out)",
     }
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run the copy kernel used to scale HBM counters")
-    args = ap.parse_args()
-    if args.calibrate_copy:
-        return calibrate_copy()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: every box here holds a single MI355X; "
-              "batches shard across GPUs with no exchange (see the design notes).", file=sys.stderr)
-        return 2
+def cpu_multi(bufs, level: int, fmt: int, budget_s: float = 20.0):
+    """orot's parallel compressor over a prefix of the buffers, one call per
+    buffer, until budget_s has passed.  Returns (baseline record, per-buffer
+    compressed sizes of the sample)."""
+    threads = cpu_threads()
+    cpu = OrotCPU(level, fmt, 0, threads)
+    out = np.empty(max(b.size for b in bufs) * 9 // 8 + 4096, dtype=np.uint8)
+    cpu.compress(bufs[0], out)  # warm-up
+    sizes = []
+    nbytes = 0
+    t0 = time.perf_counter()
+    while len(sizes) < len(bufs) and (time.perf_counter() - t0) < budget_s:
+        b = bufs[len(sizes)]
+        sizes.append(cpu.compress(b, out))
+        nbytes += b.size
+    dt = time.perf_counter() - t0
+    cpu.close()
+    rec = {
+        "value": round(nbytes / dt / 1e9, 4),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": cpu.kind,
+        "sample": f"orot parallel L{level} {['raw', 'zlib', 'gzip'][fmt]}, default chunks, the first "
+                  f"{len(sizes)} of {len(bufs)} buffers ({nbytes >> 20} MiB), one call per buffer",
+    }
+    return rec, sizes
 
+
+METRIC = "input GB/s at ratio within 3% of orot L1/L6 (bit-exact inflate); p50/p99 latency"
+
+
+def pct(vals, q):
+    v = sorted(vals)
+    return v[min(len(v) - 1, int(len(v) * q))]
+
+
+def base_record(args, value, ms_per_step, config):
+    return {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded zeros / Zipf text / random / records generators)",
+        "config": config,
+    }
+
+
+def bench_l1(args) -> dict:
+    """Default: one 256 MiB buffer (4 x 64 MiB segments), level 1, raw,
+    128 KiB chunks; a step is one deflate_parallel_compress_device call."""
     data = workload()
     n = data.size
     d_in = orot_amd.DeviceBuffer.from_host(data)
@@ -187,35 +268,22 @@ def main() -> int:
         lz.append(st[1])
     orot_amd.lib().orot_gpu_synchronize()
     total = time.perf_counter() - t0
-    value = n * args.steps / total / 1e9
-    lat_ms = sorted(x * 1e3 for x in lat)
+    lat_ms = [x * 1e3 for x in lat]
     t_kernel = statistics.mean(lz) * 1e-3
     b_alg = n + clen
     achieved = b_alg / t_kernel / 1e9
 
-    rec = {
-        "metric": "input GB/s at ratio within 3% of orot L1/L6 (bit-exact inflate); p50/p99 latency",
-        "value": round(value, 3),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(total / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic (seeded zeros / Zipf text / random / records generators)",
-        "config": {
-            "workload": "plain GPU path, L1 greedy, raw DEFLATE, 256 MiB = 4 x 64 MiB segments "
-                        "(zeros, Zipf text, random, 64-byte records), 128 KiB chunks",
-            "level": 1,
-            "format": "raw",
-            "chunk": 131072,
-            "input_bytes": n,
-        },
-        "p50_ms": round(lat_ms[len(lat_ms) // 2], 3),
-        "p99_ms": round(lat_ms[min(len(lat_ms) - 1, int(len(lat_ms) * 0.99))], 3),
+    rec = base_record(args, n * args.steps / total / 1e9, total / args.steps * 1e3, {
+        "workload": "plain GPU path, L1 greedy, raw DEFLATE, 256 MiB = 4 x 64 MiB segments "
+                    "(zeros, Zipf text, random, 64-byte records), 128 KiB chunks",
+        "level": 1,
+        "format": "raw",
+        "chunk": 131072,
+        "input_bytes": n,
+    })
+    rec.update({
+        "p50_ms": round(pct(lat_ms, 0.5), 3),
+        "p99_ms": round(pct(lat_ms, 0.99), 3),
         "compressed_bytes": clen,
         "ratio": round(clen / n, 5),
         "inflate_ok": inflated_ok,
@@ -230,17 +298,131 @@ def main() -> int:
             "frac": round(achieved * 1e9 / HBM_PEAK, 5),
             "traffic": measured_traffic(),
         },
-    }
+    })
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(data)
+    pc.close()
+    d_in.free()
+    d_out.free()
+    return rec
+
+
+def bench_l6(args) -> dict:
+    """Level 6, gzip: 256 x 8 MiB buffers (text / records), one
+    deflate_parallel_compress_device call per buffer; a step compresses all
+    256.  Every output is inflated by zlib once, and the compressed size is
+    compared with orot's at level 6 on the buffers its CPU leg covers."""
+    bufs = make_buffers(l6_specs())
+    sizes = [b.size for b in bufs]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(offs[-1])
+    slot = orot_amd.compress_bound(L6_SIZE, orot_amd.Format.GZIP) + 4096
+    d_in = orot_amd.DeviceBuffer(n)
+    for b, o in zip(bufs, offs[:-1]):
+        d_in.upload(b, int(o))
+    d_out = orot_amd.DeviceBuffer(slot * len(bufs))
+    pc = orot_amd.ParallelCompressor(6, orot_amd.Format.GZIP, 0, 0)
+    pc.set_timing(True)
+
+    def run_all(record=None):
+        out = []
+        for i, b in enumerate(bufs):
+            s0 = time.perf_counter()
+            k = pc.compress_device(d_in.ptr + int(offs[i]), b.size, d_out.ptr + i * slot, slot)
+            if record is not None:
+                record.append((time.perf_counter() - s0, pc.stage_times()))
+            if k == 0:
+                raise SystemExit(f"compress failed on buffer {i}: {orot_amd.last_error()}")
+            out.append(k)
+        return out
+
+    clens = run_all()
+    ok = True
+    for i, b in enumerate(bufs):
+        comp = d_out.download(clens[i], i * slot).tobytes()
+        ok = ok and zlib.decompress(comp, 31) == b.tobytes()
+    for _ in range(args.warmup):
+        run_all()
+    orot_amd.lib().orot_gpu_synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_all(per)
+    total = time.perf_counter() - t0
+    lat_ms = [x[0] * 1e3 for x in per]
+    front = statistics.mean(x[1][1] for x in per) * 1e-3
+    clen = sum(clens)
+    b_alg = (n + clen) / len(bufs)
+    rec = base_record(args, n * args.steps / total / 1e9, total / args.steps * 1e3, {
+        "workload": "L6-class lazy parse, dynamic Huffman blocks (15-bit codes, <= 16384 symbols), gzip, "
+                    "256 x 8 MiB buffers (128 Zipf text + 128 64-byte records), one call per buffer",
+        "level": 6,
+        "format": "gzip",
+        "chunk": 524288,
+        "buffers": len(bufs),
+        "input_bytes": n,
+    })
+    rec.update({
+        "p50_ms": round(pct(lat_ms, 0.5), 3),
+        "p99_ms": round(pct(lat_ms, 0.99), 3),
+        "compressed_bytes": clen,
+        "ratio": round(clen / n, 5),
+        "inflate_ok": ok,
+        "stage_ms_mean": [round(statistics.mean(x[1][k] for x in per), 4) for k in range(4)],
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "l6 front (pred + match + parse + block analysis)",
+            "achieved": round(b_alg / front / 1e9, 2),
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": round(b_alg / front / HBM_PEAK, 5),
+            "traffic": None,
+        },
+    })
+    if not args.no_cpu:
+        base, orot_sizes = cpu_multi(bufs, 6, 2)
+        rec["cpu_baseline"] = base
+        m = len(orot_sizes)
+        by_kind = {}
+        for i in range(m):
+            k = "text" if i % 2 == 0 else "records"
+            g, o = by_kind.get(k, (0, 0))
+            by_kind[k] = (g + clens[i], o + orot_sizes[i])
+        rec["ratio_vs_orot"] = {
+            "buffers": m,
+            "gpu_over_orot": round(sum(clens[:m]) / max(1, sum(orot_sizes)), 5),
+            "per_kind": {k: round(g / max(1, o), 5) for k, (g, o) in by_kind.items()},
+        }
+    pc.close()
+    d_in.free()
+    d_out.free()
+    return rec
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("l1", "l6"), default="l1",
+                    help="l1: 256 MiB four-segment buffer at level 1 (default); "
+                         "l6: 256 x 8 MiB text/records buffers at level 6, gzip")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the copy kernel used to scale HBM counters")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here holds a single MI355X; "
+              "batches shard across GPUs with no exchange (see the design notes).", file=sys.stderr)
+        return 2
+    rec = bench_l6(args) if args.workload == "l6" else bench_l1(args)
     try:
         rec["device"] = orot_amd.device_info()[0]
     except Exception:
         pass
     print(json.dumps(rec), flush=True)
-    pc.close()
-    d_in.free()
-    d_out.free()
     return 0
 
 

@@ -225,11 +225,15 @@ class DeviceBuffer:
 
     def upload(self, arr: np.ndarray, offset: int = 0) -> None:
         a = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+        if offset < 0 or offset + a.size > self.nbytes:
+            raise ValueError(f"upload of {a.size} B at {offset} exceeds the {self.nbytes} B buffer")
         if a.size and lib().orot_gpu_memcpy(self.ptr + offset, a.ctypes.data, a.size, 0) != 0:
             raise Error(Result.MEM_ERROR, last_error())
 
     def download(self, n: int | None = None, offset: int = 0) -> np.ndarray:
         n = self.nbytes - offset if n is None else int(n)
+        if offset < 0 or n < 0 or offset + n > self.nbytes:
+            raise ValueError(f"download of {n} B at {offset} exceeds the {self.nbytes} B buffer")
         out = np.empty(n, dtype=np.uint8)
         if n and lib().orot_gpu_memcpy(out.ctypes.data, self.ptr + offset, n, 1) != 0:
             raise Error(Result.MEM_ERROR, last_error())
