@@ -4,7 +4,8 @@ Python mirror of orot's C / C++ interface (orot include/orot/deflate.h):
 ``compress_bound``, ``ParallelCompressor``, ``crc32``, ``adler32``, the
 ``Format`` / ``Level`` constants and the ``Error`` exception, all served by
 liborot_amd.so through its C ABI (include/orot/deflate.h,
-include/orot/deflate_gpu.h).  The library must be built
+include/orot/deflate_gpu.h), plus ``BatchCompressor`` over the batch
+entry points of include/orot/deflate_batch.h.  The library must be built
 (``python -m orot_amd.build``); importing this package without it raises.
 """
 from __future__ import annotations
@@ -36,7 +37,33 @@ EXPORTED_SYMBOLS = (
     "orot_gpu_synchronize",
     "orot_gpu_device_info",
     "orot_gpu_last_error",
+    "deflate_batch_new",
+    "deflate_batch_free",
+    "deflate_batch_compress_device",
+    "deflate_batch_compress",
+    "deflate_batch_get_config",
+    "deflate_batch_set_config",
+    "deflate_batch_autotune",
+    "deflate_batch_set_timing",
+    "deflate_batch_stage_times",
 )
+
+BATCH_GRAPH = 1
+BATCH_AUTOTUNE = 2
+
+
+class BatchItem(ctypes.Structure):
+    """deflate_batch_item (include/orot/deflate_batch.h)."""
+
+    _fields_ = [("in_offset", ctypes.c_uint64), ("in_size", ctypes.c_uint64),
+                ("out_offset", ctypes.c_uint64), ("out_capacity", ctypes.c_uint64)]
+
+
+class BatchConfig(ctypes.Structure):
+    """deflate_batch_config (include/orot/deflate_batch.h)."""
+
+    _fields_ = [("block_size", ctypes.c_size_t), ("hash_bits", ctypes.c_int), ("max_chain", ctypes.c_int),
+                ("nice_len", ctypes.c_int), ("lazy_depth", ctypes.c_int)]
 
 
 class Format(enum.IntEnum):
@@ -123,6 +150,26 @@ def lib() -> ctypes.CDLL:
     L.orot_gpu_device_info.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(ctypes.c_int)]
     L.orot_gpu_last_error.restype = ctypes.c_char_p
     L.orot_gpu_last_error.argtypes = []
+    items_p = ctypes.POINTER(BatchItem)
+    L.deflate_batch_new.restype = vp
+    L.deflate_batch_new.argtypes = [ctypes.c_int, ctypes.c_int, sz, ctypes.c_uint]
+    L.deflate_batch_free.restype = None
+    L.deflate_batch_free.argtypes = [vp]
+    L.deflate_batch_compress_device.restype = ctypes.c_int
+    L.deflate_batch_compress_device.argtypes = [vp, vp, vp, items_p, sz, ctypes.POINTER(ctypes.c_uint64)]
+    L.deflate_batch_compress.restype = ctypes.c_int
+    L.deflate_batch_compress.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(sz), ctypes.POINTER(vp),
+                                         ctypes.POINTER(sz), ctypes.POINTER(sz), sz]
+    L.deflate_batch_get_config.restype = ctypes.c_int
+    L.deflate_batch_get_config.argtypes = [vp, ctypes.POINTER(BatchConfig)]
+    L.deflate_batch_set_config.restype = ctypes.c_int
+    L.deflate_batch_set_config.argtypes = [vp, ctypes.POINTER(BatchConfig)]
+    L.deflate_batch_autotune.restype = ctypes.c_int
+    L.deflate_batch_autotune.argtypes = [vp, vp, vp, items_p, sz]
+    L.deflate_batch_set_timing.restype = None
+    L.deflate_batch_set_timing.argtypes = [vp, ctypes.c_int]
+    L.deflate_batch_stage_times.restype = ctypes.c_int
+    L.deflate_batch_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
     _lib = L
     return L
 
@@ -261,3 +308,95 @@ def device_info():
     if lib().orot_gpu_device_info(name, 256, ctypes.byref(cus)) != 0:
         raise Error(Result.MEM_ERROR, last_error())
     return name.value.decode(), cus.value
+
+
+class BatchCompressor:
+    """Many independent buffers per submission (include/orot/deflate_batch.h).
+    Each output equals deflate_parallel_compress of that buffer alone with the
+    same level, format and block size."""
+
+    def __init__(self, level: int = Level.FAST, fmt: int = Format.ZLIB, block_size: int = 0, flags: int = 0):
+        self._ctx = lib().deflate_batch_new(int(level), int(fmt), int(block_size), int(flags))
+        if not self._ctx:
+            raise Error(Result.MEM_ERROR, f"BatchCompressor: {last_error()}")
+        self.level, self.format = int(level), int(fmt)
+
+    @staticmethod
+    def items(in_offsets, in_sizes, out_offsets, out_caps):
+        n = len(in_sizes)
+        arr = (BatchItem * n)()
+        for i in range(n):
+            arr[i].in_offset = int(in_offsets[i])
+            arr[i].in_size = int(in_sizes[i])
+            arr[i].out_offset = int(out_offsets[i])
+            arr[i].out_capacity = int(out_caps[i])
+        return arr
+
+    def compress_device(self, d_in: int, d_out: int, items) -> np.ndarray:
+        """Device-resident batch; returns the framed sizes (0: did not fit)."""
+        n = len(items)
+        sizes = np.zeros(max(n, 1), dtype=np.uint64)
+        rc = lib().deflate_batch_compress_device(self._ctx, d_in, d_out, items, n,
+                                                 sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        if rc != 0:
+            raise Error(Result.MEM_ERROR, f"deflate_batch_compress_device: {last_error()}")
+        return sizes[:n]
+
+    def compress(self, buffers) -> list:
+        """Host buffers in, compressed bytes out (one entry per buffer)."""
+        arrs = [_as_u8(b) for b in buffers]
+        n = len(arrs)
+        outs = [np.empty(compress_bound(a.size, Format.GZIP) + 64 * (a.size // 65535 + 4), dtype=np.uint8)
+                for a in arrs]
+        in_p = (ctypes.c_void_p * max(n, 1))(*[a.ctypes.data for a in arrs])
+        out_p = (ctypes.c_void_p * max(n, 1))(*[o.ctypes.data for o in outs])
+        in_s = (ctypes.c_size_t * max(n, 1))(*[a.size for a in arrs])
+        out_c = (ctypes.c_size_t * max(n, 1))(*[o.size for o in outs])
+        got = (ctypes.c_size_t * max(n, 1))()
+        rc = lib().deflate_batch_compress(self._ctx, in_p, in_s, out_p, out_c, got, n)
+        if rc != 0:
+            raise Error(Result.MEM_ERROR, f"deflate_batch_compress: {last_error()}")
+        return [outs[i][:got[i]].tobytes() for i in range(n)]
+
+    def config(self) -> dict:
+        c = BatchConfig()
+        if lib().deflate_batch_get_config(self._ctx, ctypes.byref(c)) != 0:
+            raise Error(Result.PARAM_ERROR, last_error())
+        return {"block_size": c.block_size, "hash_bits": c.hash_bits, "max_chain": c.max_chain,
+                "nice_len": c.nice_len, "lazy_depth": c.lazy_depth}
+
+    def set_config(self, block_size: int, hash_bits: int = 0, max_chain: int = 0, nice_len: int = 0,
+                   lazy_depth: int = 0) -> None:
+        c = BatchConfig(block_size, hash_bits, max_chain, nice_len, lazy_depth)
+        if lib().deflate_batch_set_config(self._ctx, ctypes.byref(c)) != 0:
+            raise Error(Result.PARAM_ERROR, f"deflate_batch_set_config: {last_error()}")
+
+    def autotune(self, d_in: int, d_out: int, items) -> dict:
+        if lib().deflate_batch_autotune(self._ctx, d_in, d_out, items, len(items)) != 0:
+            raise Error(Result.MEM_ERROR, f"deflate_batch_autotune: {last_error()}")
+        return self.config()
+
+    def set_timing(self, on: bool) -> None:
+        lib().deflate_batch_set_timing(self._ctx, 1 if on else 0)
+
+    def stage_times(self) -> list:
+        buf = (ctypes.c_float * 8)()
+        k = lib().deflate_batch_stage_times(self._ctx, buf, 8)
+        return [buf[i] for i in range(k)]
+
+    def close(self) -> None:
+        if self._ctx:
+            lib().deflate_batch_free(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -3,58 +3,49 @@
 // Implements orot's parallel compress entry points (src/api/stream_api.cpp:
 // 380-406 -> src/parallel/parallel_compressor.cpp:22-150), the bound
 // (src/api/deflate_api.cpp:81-98) and the checksum entry points
-// (src/api/deflate_api.cpp:223-231) on top of the gfx950 kernels.  The
-// context owns its HIP stream, its device arenas (grown on demand, kept across
-// calls) and optional timing events.  There is no CPU compression path: a
-// missing or failing GPU makes every entry point fail (NULL / 0) with a
-// message in orot_gpu_last_error().
+// (src/api/deflate_api.cpp:223-231) on top of the gfx950 kernels, plus the
+// batch entry points of deflate_batch.h.  Each context owns an Engine (HIP
+// stream, device arenas grown on demand and kept across calls, pinned batch
+// tables).  There is no CPU compression path: a missing or failing GPU makes
+// every entry point fail (NULL / 0 / -1) with a message in
+// orot_gpu_last_error().
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/orot/deflate.h"
+#include "../../include/orot/deflate_batch.h"
 #include "../../include/orot/deflate_gpu.h"
 #include "deflate_gpu.h"
+#include "engine.h"
+#include "host_common.h"
 
+using orot_gpu::BufferSpec;
 using orot_gpu::ChunkInfo;
-using orot_gpu::Result;
+using orot_gpu::Engine;
+using orot_gpu::EngineConfig;
+using orot_gpu::set_error;
+using orot_gpu::set_error_msg;
+
+namespace orot_gpu {
 
 namespace {
-
 thread_local std::string g_last_error;
-
-void set_error(const char* where, hipError_t e) {
-    g_last_error = std::string(where) + ": " + hipGetErrorString(e);
 }
+
+void set_error(const char* where, hipError_t e) { g_last_error = std::string(where) + ": " + hipGetErrorString(e); }
 void set_error_msg(const std::string& m) { g_last_error = m; }
+const char* last_error_message() { return g_last_error.c_str(); }
 
-size_t default_block_size(int level) {  // src/parallel/parallel_compressor.cpp:16-20
-    if (level <= 3) return 131072;
-    if (level <= 6) return 524288;
-    return 1048576;
-}
+}  // namespace orot_gpu
 
-int clamp_level(int level) { return level < 0 ? 0 : (level > 12 ? 12 : level); }
-
-// Result of the start-up lane-order check of LDS exchanges: -1 not run yet,
-// 0 failed (the L1 kernels use their ballot variant), 1 passed.
-std::mutex g_exch_mu;
-int g_exch_state = -1;
-
-bool exchange_order_ok(hipStream_t stream) {
-    std::lock_guard<std::mutex> lk(g_exch_mu);
-    if (g_exch_state < 0) {
-        bool ok = false;
-        hipError_t e = orot_gpu::check_exchange_order(stream, &ok);
-        if (e != hipSuccess) return false;
-        g_exch_state = ok ? 1 : 0;
-    }
-    return g_exch_state == 1;
-}
+namespace {
 
 template <class T>
 bool grow(T*& p, size_t& cap, size_t need, const char* what) {
@@ -73,150 +64,135 @@ bool grow(T*& p, size_t& cap, size_t need, const char* what) {
     return true;
 }
 
+// Device output arena for a host call: orot's bound, or the stored-block
+// worst case when small chunks make the per-block headers exceed it.
+size_t host_out_bound(size_t n, int format, uint32_t chunk) {
+    const size_t nchunks = n ? (n + chunk - 1) / chunk : 1;
+    return std::max(deflate_compress_bound(n, (deflate_format)format) + 64, n + 6 * (nchunks + n / 65535 + 2) + 64);
+}
+
 }  // namespace
 
 struct deflate_parallel_ctx {
-    int level = 1;
-    int format = 0;
-    size_t block = 131072;
-    hipStream_t stream = nullptr;
+    Engine eng;
+    std::mutex mu;
     uint8_t* d_in = nullptr;
     size_t in_cap = 0;
     uint8_t* d_out = nullptr;
     size_t out_cap = 0;
-    uint32_t* d_tok = nullptr;
-    size_t tok_cap = 0;
-    uint16_t* d_pd = nullptr;
-    size_t pd_cap = 0;
-    uint32_t* d_aux = nullptr;
-    size_t aux_cap = 0;
-    uint32_t* d_ntok = nullptr;
-    size_t ntok_cap = 0;
-    uint32_t* d_blkpos = nullptr;
-    size_t blkpos_cap = 0;
-    uint8_t* d_blocks = nullptr;
-    size_t blocks_cap = 0;
-    uint32_t* d_stage = nullptr;
-    size_t stage_cap = 0;
-    ChunkInfo* d_info = nullptr;
-    size_t info_cap = 0;
-    Result* d_res = nullptr;
-    size_t res_cap = 0;
-    bool timing = false;
-    hipEvent_t ev[8] = {};
-    float stage_ms[4] = {};
-    int nstage = 0;
-    bool exch_ok = false;
-    size_t lds_max = 65536;
-    std::mutex mu;
 
     ~deflate_parallel_ctx() {
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        for (void* p : {(void*)d_in, (void*)d_out, (void*)d_tok, (void*)d_pd, (void*)d_aux, (void*)d_ntok, (void*)d_blkpos, (void*)d_blocks, (void*)d_stage, (void*)d_info,
-                        (void*)d_res})
-            if (p) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
     }
 
-    // Runs the pipeline on device buffers; returns the framed size or 0.
+    // One buffer on the device; returns the framed size or 0.
     size_t run(const uint8_t* in, size_t n, uint8_t* out, size_t cap) {
-        const int lv = clamp_level(level);
-        if (lv >= 4 && block > orot_gpu::l6_max_chunk()) {
-            set_error_msg("deflate_parallel_compress: block_size " + std::to_string(block) +
-                          " exceeds the GPU limit " + std::to_string(orot_gpu::l6_max_chunk()) + " for this level");
-            return 0;
-        }
-        if (lv >= 1 && lv <= 3) {
-            const uint32_t maxc = orot_gpu::max_chunk_for_hash_bits(lv == 1 ? 12 : 14);
-            if (block > maxc) {
-                set_error_msg("deflate_parallel_compress: block_size " + std::to_string(block) +
-                              " exceeds the GPU limit " + std::to_string(maxc) + " for this level");
-                return 0;
-            }
-        }
-        if (block > 0xFFFFFFFFull) {
-            set_error_msg("deflate_parallel_compress: block_size too large");
-            return 0;
-        }
-        const uint32_t chunk = (uint32_t)block;
-        const uint64_t nb64 = n ? (n + chunk - 1) / chunk : 1;
-        if (nb64 > 0x7FFFFFFF) {
-            set_error_msg("deflate_parallel_compress: too many chunks");
-            return 0;
-        }
-        const uint32_t nb = (uint32_t)nb64;
-        const uint64_t sw = orot_gpu::stage_words_for_chunk(chunk);
-        const size_t padded = (size_t)nb * ((chunk + 7u) & ~7u) + 8;
-        if (lv >= 4) {
-            const size_t mb = orot_gpu::l6_max_blocks(chunk);
-            if (!grow(d_pd, pd_cap, padded, "hipMalloc(pd)") || !grow(d_aux, aux_cap, padded, "hipMalloc(tokens)") ||
-                !grow(d_ntok, ntok_cap, nb, "hipMalloc(ntok)") ||
-                !grow(d_blkpos, blkpos_cap, (size_t)nb * mb, "hipMalloc(blkpos)") ||
-                !grow(d_blocks, blocks_cap, (size_t)nb * mb * orot_gpu::l6_block_record_bytes(), "hipMalloc(blocks)"))
-                return 0;
-        }
-        if (!grow(d_tok, tok_cap, padded, "hipMalloc(tok)") ||
-            !grow(d_stage, stage_cap, (size_t)(sw * nb), "hipMalloc(stage)") ||
-            !grow(d_info, info_cap, nb, "hipMalloc(info)") || !grow(d_res, res_cap, 1, "hipMalloc(result)"))
-            return 0;
-        if (timing && !ev[0]) {
-            for (auto& e : ev) {
-                hipError_t r = hipEventCreate(&e);
-                if (r != hipSuccess) {
-                    set_error("hipEventCreate", r);
-                    return 0;
-                }
-            }
-        }
-        orot_gpu::Plan p{};
-        p.d_in = in;
-        p.n = n;
-        p.chunk = chunk;
-        p.nchunks = nb;
-        p.level = lv;
-        p.format = format;
-        p.d_out = out;
-        p.cap = cap;
-        p.d_tok = d_tok;
-        p.d_pd = d_pd;
-        p.d_aux = d_aux;
-        p.d_ntok = d_ntok;
-        p.d_blkpos = d_blkpos;
-        p.d_blocks = d_blocks;
-        p.d_stage = d_stage;
-        p.stage_words = sw;
-        p.d_info = d_info;
-        p.d_result = d_res;
-        p.stream = stream;
-        p.exch_ok = exch_ok;
-        p.lds_max = lds_max;
-        for (int i = 0; i < 8; ++i) p.ev[i] = timing ? ev[i] : nullptr;
-        hipError_t e = orot_gpu::launch_parallel_compress(p);
-        if (e != hipSuccess) {
-            set_error("launch_parallel_compress", e);
-            return 0;
-        }
-        Result r{};
-        e = hipMemcpyAsync(&r, d_res, sizeof(Result), hipMemcpyDeviceToHost, stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) {
-            set_error("deflate_parallel_compress", e);
-            return 0;
-        }
-        nstage = 0;
-        if (timing) {
-            for (int i = 0; i < 4; ++i) {
-                float ms = 0;
-                (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
-                stage_ms[i] = ms;
-            }
-            nstage = 4;
-        }
-        if (r.total_bytes == 0) set_error_msg("deflate_parallel_compress: output does not fit out_capacity");
-        return (size_t)r.total_bytes;
+        const BufferSpec b{0, n, 0, cap};
+        uint64_t got = 0;
+        if (!eng.submit(in, out, &b, 1, false) || !eng.wait(&got)) return 0;
+        if (got == 0) set_error_msg("deflate_parallel_compress: output does not fit out_capacity");
+        return (size_t)got;
     }
 };
+
+struct deflate_batch_ctx {
+    Engine eng;
+    std::mutex mu;
+    bool graph = false;
+    bool autotune_pending = false;
+    uint8_t* d_in = nullptr;
+    size_t in_cap = 0;
+    uint8_t* d_out = nullptr;
+    size_t out_cap = 0;
+
+    ~deflate_batch_ctx() {
+        if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
+    }
+};
+
+namespace {
+
+bool run_batch(Engine& eng, const uint8_t* d_in, uint8_t* d_out, const std::vector<BufferSpec>& specs, bool graph,
+               uint64_t* sizes) {
+    return eng.submit(d_in, d_out, specs.data(), specs.size(), graph) && eng.wait(sizes);
+}
+
+// Candidate configurations of the autotuner for a level.
+std::vector<EngineConfig> tune_candidates(const EngineConfig& base) {
+    std::vector<EngineConfig> v;
+    const int lv = base.level;
+    if (lv >= 1 && lv <= 3) {
+        for (int hb : {12, 14})
+            for (uint32_t ch : {65536u, 131072u, 262144u}) {
+                if (ch > orot_gpu::max_chunk_for_hash_bits(hb)) continue;
+                EngineConfig c = base;
+                c.hash_bits = hb;
+                c.chunk = ch;
+                v.push_back(c);
+            }
+    } else if (lv >= 4) {
+        const orot_gpu::L6Params d = orot_gpu::l6_level_params(lv);
+        for (uint32_t ch : {262144u, 524288u, 1048576u})
+            for (int mc : {d.max_chain / 2, d.max_chain, d.max_chain * 2}) {
+                if (mc < 1 || mc > 4096) continue;
+                EngineConfig c = base;
+                c.chunk = ch;
+                c.l6 = d;
+                c.l6.max_chain = mc;
+                v.push_back(c);
+            }
+    }
+    return v;
+}
+
+int autotune(Engine& eng, const uint8_t* d_in, uint8_t* d_out, const deflate_batch_item* items, size_t count) {
+    const int lv = eng.config().level;
+    if (lv == 0 || count == 0) return 0;
+    // sample: a prefix of up to 256 MiB of input (at least one buffer)
+    std::vector<BufferSpec> sample;
+    uint64_t bytes = 0;
+    for (size_t i = 0; i < count && (sample.empty() || bytes < (256ull << 20)); ++i) {
+        sample.push_back({items[i].in_offset, items[i].in_size, items[i].out_offset, items[i].out_capacity});
+        bytes += items[i].in_size;
+    }
+    std::vector<uint64_t> sizes(sample.size());
+    auto measure = [&](const EngineConfig& c, double& secs, uint64_t& total) -> bool {
+        if (!eng.set_config(c)) return false;
+        if (!run_batch(eng, d_in, d_out, sample, false, sizes.data())) return false;  // warm-up, arena growth
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!run_batch(eng, d_in, d_out, sample, false, sizes.data())) return false;
+        secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        total = 0;
+        for (uint64_t s : sizes) {
+            if (s == 0) return false;  // did not fit: not a candidate
+            total += s;
+        }
+        return true;
+    };
+    EngineConfig base{};
+    base.level = lv;
+    base.format = eng.config().format;
+    orot_gpu::resolve_config(base);
+    double base_t = 0;
+    uint64_t base_sz = 0;
+    if (!measure(base, base_t, base_sz)) return -1;
+    EngineConfig best = base;
+    double best_t = base_t;
+    for (const EngineConfig& c : tune_candidates(base)) {
+        double t = 0;
+        uint64_t sz = 0;
+        if (!measure(c, t, sz)) continue;
+        if ((double)sz <= (double)base_sz * 1.015 && t < best_t) {
+            best = c;
+            best_t = t;
+        }
+    }
+    return eng.set_config(best) ? 0 : -1;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -229,30 +205,19 @@ size_t deflate_compress_bound(size_t n, deflate_format format) {
 
 deflate_parallel_ctx* deflate_parallel_new(int level, deflate_format format, int num_threads, size_t block_size) {
     (void)num_threads;
-    int ndev = 0;
-    hipError_t e = hipGetDeviceCount(&ndev);
-    if (e != hipSuccess || ndev == 0) {
-        set_error_msg("deflate_parallel_new: no usable GPU");
+    if (block_size > 0xFFFFFFFFull) {
+        set_error_msg("deflate_parallel_new: block_size too large");
         return nullptr;
     }
     auto* ctx = new (std::nothrow) deflate_parallel_ctx;
     if (!ctx) return nullptr;
-    ctx->level = level;
-    ctx->format = (format == DEFLATE_FORMAT_ZLIB || format == DEFLATE_FORMAT_GZIP) ? (int)format : 0;
-    ctx->block = block_size ? block_size : default_block_size(level);
-    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        set_error("hipStreamCreate", e);
-        ctx->stream = nullptr;
+    EngineConfig cfg;
+    cfg.level = level;
+    cfg.format = (int)format;
+    cfg.chunk = (uint32_t)block_size;
+    if (!ctx->eng.init(cfg)) {
         delete ctx;
         return nullptr;
-    }
-    ctx->exch_ok = exchange_order_ok(ctx->stream);
-    {
-        int dev = 0, v = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess && v > 0)
-            ctx->lds_max = (size_t)v;
     }
     return ctx;
 }
@@ -265,15 +230,11 @@ size_t deflate_parallel_compress(deflate_parallel_ctx* ctx, const void* in, size
     // orot checks only the empty-input bound and the framing bytes
     // (parallel_compressor.cpp:40-45, 91-108); this path also refuses to
     // overflow the payload.
-    // Arena for the device output: the orot bound, or the stored-block worst
-    // case when small chunks make per-block headers exceed it.
-    const size_t nchunks = n ? (n + ctx->block - 1) / ctx->block : 1;
-    const size_t bound = std::max(deflate_compress_bound(n, (deflate_format)ctx->format) + 64,
-                                  n + 6 * (nchunks + n / 65535 + 2) + 64);
+    const size_t bound = host_out_bound(n, ctx->eng.config().format, ctx->eng.config().chunk);
     if (!grow(ctx->d_in, ctx->in_cap, n, "hipMalloc(in)") || !grow(ctx->d_out, ctx->out_cap, bound, "hipMalloc(out)"))
         return 0;
     if (n) {
-        hipError_t e = hipMemcpyAsync(ctx->d_in, in, n, hipMemcpyHostToDevice, ctx->stream);
+        hipError_t e = hipMemcpyAsync(ctx->d_in, in, n, hipMemcpyHostToDevice, ctx->eng.stream());
         if (e != hipSuccess) {
             set_error("hipMemcpy(in)", e);
             return 0;
@@ -281,8 +242,8 @@ size_t deflate_parallel_compress(deflate_parallel_ctx* ctx, const void* in, size
     }
     const size_t got = ctx->run(ctx->d_in, n, ctx->d_out, std::min(cap, bound));
     if (!got) return 0;
-    hipError_t e = hipMemcpyAsync(out, ctx->d_out, got, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipError_t e = hipMemcpyAsync(out, ctx->d_out, got, hipMemcpyDeviceToHost, ctx->eng.stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->eng.stream());
     if (e != hipSuccess) {
         set_error("hipMemcpy(out)", e);
         return 0;
@@ -300,14 +261,146 @@ size_t deflate_parallel_compress_device(deflate_parallel_ctx* ctx, const void* d
 void deflate_parallel_set_timing(deflate_parallel_ctx* ctx, int on) {
     if (!ctx) return;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    ctx->timing = on != 0;
+    ctx->eng.set_timing(on != 0);
 }
 
 int deflate_parallel_stage_times(const deflate_parallel_ctx* ctx, float* ms, int max_values) {
     if (!ctx || !ms) return 0;
-    const int k = std::min(ctx->nstage, max_values);
-    for (int i = 0; i < k; ++i) ms[i] = ctx->stage_ms[i];
-    return k;
+    return ctx->eng.stage_times(ms, max_values);
+}
+
+// ---- batches -------------------------------------------------------------
+
+deflate_batch_ctx* deflate_batch_new(int level, deflate_format format, size_t block_size, unsigned flags) {
+    if (block_size > 0xFFFFFFFFull) {
+        set_error_msg("deflate_batch_new: block_size too large");
+        return nullptr;
+    }
+    auto* ctx = new (std::nothrow) deflate_batch_ctx;
+    if (!ctx) return nullptr;
+    EngineConfig cfg;
+    cfg.level = level;
+    cfg.format = (int)format;
+    cfg.chunk = (uint32_t)block_size;
+    if (!ctx->eng.init(cfg)) {
+        delete ctx;
+        return nullptr;
+    }
+    ctx->graph = (flags & DEFLATE_BATCH_GRAPH) != 0;
+    ctx->autotune_pending = (flags & DEFLATE_BATCH_AUTOTUNE) != 0 && block_size == 0;
+    return ctx;
+}
+
+void deflate_batch_free(deflate_batch_ctx* ctx) { delete ctx; }
+
+int deflate_batch_compress_device(deflate_batch_ctx* ctx, const void* d_in, void* d_out,
+                                  const deflate_batch_item* items, size_t count, uint64_t* out_sizes) {
+    if (!ctx || (count && (!items || !out_sizes))) return -1;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const auto* in = static_cast<const uint8_t*>(d_in);
+    auto* out = static_cast<uint8_t*>(d_out);
+    if (ctx->autotune_pending && count) {
+        ctx->autotune_pending = false;
+        if (autotune(ctx->eng, in, out, items, count) != 0) return -1;
+    }
+    std::vector<BufferSpec> specs(count);
+    for (size_t i = 0; i < count; ++i)
+        specs[i] = {items[i].in_offset, items[i].in_size, items[i].out_offset, items[i].out_capacity};
+    return run_batch(ctx->eng, in, out, specs, ctx->graph, out_sizes) ? 0 : -1;
+}
+
+int deflate_batch_compress(deflate_batch_ctx* ctx, const void* const* in, const size_t* in_sizes, void* const* out,
+                           const size_t* out_caps, size_t* out_sizes, size_t count) {
+    if (!ctx || (count && (!in || !in_sizes || !out || !out_caps || !out_sizes))) return -1;
+    std::vector<deflate_batch_item> items(count);
+    uint64_t ipos = 0, opos = 0;
+    for (size_t i = 0; i < count; ++i) {
+        const size_t cap = std::min(out_caps[i], host_out_bound(in_sizes[i], ctx->eng.config().format,
+                                                                ctx->eng.config().chunk));
+        items[i] = {ipos, in_sizes[i], opos, cap};
+        ipos += (in_sizes[i] + 255) & ~(uint64_t)255;
+        opos += (cap + 255) & ~(uint64_t)255;
+    }
+    std::vector<uint64_t> sizes(count);
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (!grow(ctx->d_in, ctx->in_cap, ipos, "hipMalloc(in)") ||
+            !grow(ctx->d_out, ctx->out_cap, opos, "hipMalloc(out)"))
+            return -1;
+        for (size_t i = 0; i < count; ++i) {
+            if (!in_sizes[i]) continue;
+            hipError_t e = hipMemcpyAsync(ctx->d_in + items[i].in_offset, in[i], in_sizes[i], hipMemcpyHostToDevice,
+                                          ctx->eng.stream());
+            if (e != hipSuccess) {
+                set_error("hipMemcpy(in)", e);
+                return -1;
+            }
+        }
+    }
+    if (deflate_batch_compress_device(ctx, ctx->d_in, ctx->d_out, items.data(), count, sizes.data()) != 0) return -1;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (size_t i = 0; i < count; ++i) {
+        out_sizes[i] = (size_t)sizes[i];
+        if (!sizes[i]) continue;
+        hipError_t e = hipMemcpyAsync(out[i], ctx->d_out + items[i].out_offset, sizes[i], hipMemcpyDeviceToHost,
+                                      ctx->eng.stream());
+        if (e != hipSuccess) {
+            set_error("hipMemcpy(out)", e);
+            return -1;
+        }
+    }
+    hipError_t e = hipStreamSynchronize(ctx->eng.stream());
+    if (e != hipSuccess) {
+        set_error("hipStreamSynchronize", e);
+        return -1;
+    }
+    return 0;
+}
+
+int deflate_batch_get_config(deflate_batch_ctx* ctx, deflate_batch_config* cfg) {
+    if (!ctx || !cfg) return -1;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const EngineConfig& c = ctx->eng.config();
+    cfg->block_size = c.chunk;
+    cfg->hash_bits = c.hash_bits;
+    cfg->max_chain = c.l6.max_chain;
+    cfg->nice_len = c.l6.nice_len;
+    cfg->lazy_depth = c.l6.lazy_depth;
+    return 0;
+}
+
+int deflate_batch_set_config(deflate_batch_ctx* ctx, const deflate_batch_config* cfg) {
+    if (!ctx || !cfg || cfg->block_size > 0xFFFFFFFFull) return -1;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    EngineConfig c = ctx->eng.config();
+    c.chunk = (uint32_t)cfg->block_size;
+    c.hash_bits = cfg->hash_bits;
+    c.l6.max_chain = cfg->max_chain;
+    c.l6.nice_len = cfg->nice_len;
+    c.l6.lazy_depth = cfg->lazy_depth;
+    if (!ctx->eng.set_config(c)) return -1;
+    ctx->autotune_pending = false;
+    return 0;
+}
+
+int deflate_batch_autotune(deflate_batch_ctx* ctx, const void* d_in, void* d_out, const deflate_batch_item* items,
+                           size_t count) {
+    if (!ctx || (count && !items)) return -1;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->autotune_pending = false;
+    return autotune(ctx->eng, static_cast<const uint8_t*>(d_in), static_cast<uint8_t*>(d_out), items, count);
+}
+
+void deflate_batch_set_timing(deflate_batch_ctx* ctx, int on) {
+    if (!ctx) return;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->eng.set_timing(on != 0);
+}
+
+int deflate_batch_stage_times(deflate_batch_ctx* ctx, float* ms, int max_values) {
+    if (!ctx || !ms) return 0;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return ctx->eng.stage_times(ms, max_values);
 }
 
 }  // extern "C"
@@ -422,6 +515,6 @@ int orot_gpu_device_info(char* name, size_t name_len, int* compute_units) {
     return 0;
 }
 
-const char* orot_gpu_last_error(void) { return g_last_error.c_str(); }
+const char* orot_gpu_last_error(void) { return orot_gpu::last_error_message(); }
 
 }  // extern "C"

@@ -1,5 +1,11 @@
-// deflate_gpu.h -- internal interface between the C-ABI host layer
-// (deflate_api.cpp) and the gfx950 kernels (deflate_kernels.hip).
+// deflate_gpu.h -- internal interface between the host layer (deflate_api.cpp,
+// deflate_engine.cpp) and the gfx950 kernels (deflate_kernels.hip,
+// deflate_l6.hip).
+//
+// Every launch compresses a batch: a table of buffers, each cut into chunks
+// (DEFLATE blocks) of at most `chunk` bytes.  The single-buffer entry points
+// are batches of one.  Work counts live in device memory (BatchHdr), so a
+// captured graph replays for any batch that fits the arenas.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,11 +14,41 @@
 
 namespace orot_gpu {
 
+constexpr uint32_t CHUNK_LAST = 0x80000000u;  // ChunkDesc::buf flag: last chunk of its buffer
+
+// One chunk of one buffer.
+struct ChunkDesc {
+    uint64_t in_off;  // first byte of the chunk in the input arena
+    uint32_t len;     // bytes
+    uint32_t buf;     // buffer index | CHUNK_LAST
+};
+
+// One buffer of the batch.
+struct BufDesc {
+    uint64_t in_off;   // first byte of the buffer in the input arena
+    uint64_t in_len;
+    uint64_t out_off;  // first byte of its framed output in the output arena
+    uint64_t cap;      // output capacity
+    uint32_t chunk0;   // its chunks are [chunk0, chunk0 + nchunks)
+    uint32_t nchunks;
+};
+
+// Work-queue heads of the persistent kernels.
+enum Queue { Q_LZ = 0, Q_PRED = 1, Q_MATCH = 2, Q_PARSE = 3, Q_COUNT = 8 };
+
+// Batch header in device memory, rewritten (queue heads zero) at each submit.
+struct BatchHdr {
+    uint32_t nchunks;
+    uint32_t nbufs;
+    uint32_t pad_[6];
+    uint32_t queue[Q_COUNT];
+};
+
 // Per-chunk record in HBM.  Written by the checksum and LZ77 kernels, read by
 // the scan (finalize) and assembly kernels.
 struct ChunkInfo {
     uint64_t fixed_bits;  // size of the chunk's fixed-Huffman block, header..EOB
-    uint64_t bit_off;     // offset of the chunk inside the raw DEFLATE payload
+    uint64_t bit_off;     // offset of the chunk inside its buffer's raw DEFLATE payload
     uint32_t raw_len;     // input bytes of the chunk
     uint32_t is_stored;   // 0: one block of fixed_bits, 1: stored block(s),
                           // 2: several blocks, size by incoming bit offset mod 8 in sz8
@@ -23,7 +59,7 @@ struct ChunkInfo {
     uint64_t sz8[8];      // is_stored == 2: chunk bits when it starts at bit offset = r (mod 8)
 };
 
-// Result words written by the finalize kernel.
+// Per-buffer result written by the finalize kernel.
 struct Result {
     uint64_t total_bytes;    // framed output size, 0 when it exceeds the capacity
     uint64_t payload_bits;   // raw DEFLATE bit length
@@ -31,15 +67,28 @@ struct Result {
     uint32_t pad_;
 };
 
+// Matcher parameters of the L6 class (levels 4..12).
+struct L6Params {
+    int max_chain;   // chain candidates examined per position
+    int nice_len;    // stop the walk at this length
+    int lazy_depth;  // literals the lazy rule may defer
+};
+
 struct Plan {
-    const uint8_t* d_in;
-    uint64_t n;             // input bytes
-    uint32_t chunk;         // chunk (DEFLATE block) size
-    uint32_t nchunks;
-    int level;              // clamped 0..12
+    const uint8_t* d_in;    // input arena
+    uint8_t* d_out;         // output arena
+    const ChunkDesc* d_desc;
+    const uint32_t* d_order;  // queue order of the chunks (longest first)
+    const BufDesc* d_bufs;
+    BatchHdr* d_hdr;
+    uint32_t nchunks;       // host copy of the counts: grid sizes of direct launches
+    uint32_t nbufs;
+    bool persistent_grid;   // launch at resident capacity (graph capture): counts come from d_hdr only
+    uint32_t chunk;         // chunk capacity (per-chunk scratch stride)
+    int level;              // 0 stored, 1..3 L1 class, >= 4 L6 class
+    int hash_bits;          // L1 class: 12 or 14
+    L6Params l6;
     int format;             // 0 raw, 1 zlib, 2 gzip
-    uint8_t* d_out;
-    uint64_t cap;
     uint32_t* d_tok;        // per position (chunk-padded): token word / match word
     uint16_t* d_pd;         // L6 class: per position chain step, then decision code
     uint32_t* d_aux;        // L6 class: tokens in order (chunk-padded)
@@ -49,11 +98,12 @@ struct Plan {
     uint32_t* d_stage;      // nchunks * stage_words
     uint64_t stage_words;   // per chunk
     ChunkInfo* d_info;      // nchunks
-    Result* d_result;
+    Result* d_result;       // nbufs
     hipStream_t stream;
     hipEvent_t ev[8];       // optional timing events (nullptr = off)
     bool exch_ok;           // LDS exchanges of one wave apply in lane order
     size_t lds_max;         // largest dynamic LDS allocation per workgroup
+    int cus;                // compute units
 };
 
 // One-time start-up check behind Plan::exch_ok.
@@ -69,6 +119,7 @@ uint64_t stage_words_for_chunk(uint32_t chunk);
 hipError_t launch_parallel_compress(const Plan& plan);
 
 // L6-class path (levels 4..12): LZ77 + block analysis, then emission.
+L6Params l6_level_params(int level);
 uint32_t l6_max_blocks(uint32_t chunk);
 uint32_t l6_max_chunk();
 size_t l6_block_record_bytes();

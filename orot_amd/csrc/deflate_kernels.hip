@@ -34,22 +34,17 @@ constexpr uint32_t MAX_CHUNK_L1 = 256u * 1024u;  // 12-bit tables: 8 sub-ranges 
 // ---------------------------------------------------------------------------
 // Checksum partials per chunk.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void checksum_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
-                                                       uint32_t chunk, int kind, ChunkInfo* __restrict__ info) {
-    __shared__ uint32_t T[4][256];
-    __shared__ uint32_t r0[256];
-    __shared__ uint32_t r1[256];
+// CRC-32 register (zero initial value) or Adler sums of one chunk, by 256
+// threads: each thread a contiguous segment, segments combined with the
+// x^(8 * bytes after) shift (CRC) or the weighted Adler sum.
+__device__ __forceinline__ void chunk_checksum(const uint8_t* __restrict__ s, uint32_t len, int kind,
+                                               ChunkInfo* __restrict__ ci, const uint32_t (*T)[256],
+                                               uint32_t* r0, uint32_t* r1) {
     const int tid = threadIdx.x;
-    const uint32_t c = blockIdx.x;
-    const uint64_t base = (uint64_t)c * chunk;
-    const uint32_t len = (uint32_t)min<uint64_t>(chunk, n_total - base);
-    const uint8_t* s = src + base;
     const uint32_t seg = (((len + 255u) / 256u) + 15u) & ~15u;
     const uint32_t a = min(len, (uint32_t)tid * seg), b = min(len, a + seg);
     const uint32_t after = len - b;
     if (kind == 0) {
-        for (int i = tid; i < 1024; i += 256) T[i >> 8][i & 255] = c_crc.t[i >> 8][i & 255];
-        __syncthreads();
         uint32_t crc = 0;
         uint32_t p = a;
         for (; p + 4 <= b; p += 4) {
@@ -63,7 +58,7 @@ __global__ __launch_bounds__(256) void checksum_kernel(const uint8_t* __restrict
             if (tid < w) r0[tid] ^= r0[tid + w];
             __syncthreads();
         }
-        if (tid == 0) info[c].crc = r0[0];
+        if (tid == 0) ci->crc = r0[0];
     } else {
         uint32_t s1 = 0, s2 = 0;
         uint32_t p = a;
@@ -98,10 +93,40 @@ __global__ __launch_bounds__(256) void checksum_kernel(const uint8_t* __restrict
             __syncthreads();
         }
         if (tid == 0) {
-            info[c].adler_s1 = r0[0];
-            info[c].adler_s2 = r1[0];
+            ci->adler_s1 = r0[0];
+            ci->adler_s2 = r1[0];
         }
     }
+    __syncthreads();
+}
+
+// Checksum partials of every chunk of the batch (grid-stride over chunks).
+__global__ __launch_bounds__(256) void checksum_kernel(const uint8_t* __restrict__ src,
+                                                       const ChunkDesc* __restrict__ desc,
+                                                       const BatchHdr* __restrict__ hdr, int kind,
+                                                       ChunkInfo* __restrict__ info) {
+    __shared__ uint32_t T[4][256];
+    __shared__ uint32_t r0[256];
+    __shared__ uint32_t r1[256];
+    for (int i = threadIdx.x; i < 1024; i += 256) T[i >> 8][i & 255] = c_crc.t[i >> 8][i & 255];
+    __syncthreads();
+    const uint32_t nchunks = hdr->nchunks;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x)
+        chunk_checksum(src + desc[c].in_off, desc[c].len, kind, &info[c], T, r0, r1);
+}
+
+// Checksum partials of a plain buffer in uniform chunks (whole-buffer entry points).
+__global__ __launch_bounds__(256) void checksum_uniform_kernel(const uint8_t* __restrict__ src, uint64_t n,
+                                                               uint32_t chunk, int kind,
+                                                               ChunkInfo* __restrict__ info) {
+    __shared__ uint32_t T[4][256];
+    __shared__ uint32_t r0[256];
+    __shared__ uint32_t r1[256];
+    for (int i = threadIdx.x; i < 1024; i += 256) T[i >> 8][i & 255] = c_crc.t[i >> 8][i & 255];
+    __syncthreads();
+    const uint32_t c = blockIdx.x;
+    const uint64_t base = (uint64_t)c * chunk;
+    chunk_checksum(src + base, (uint32_t)min<uint64_t>(chunk, n - base), kind, &info[c], T, r0, r1);
 }
 
 // ---------------------------------------------------------------------------
@@ -118,18 +143,13 @@ constexpr uint32_t L1_BM_WORDS = MAX_CHUNK_L1 / 32;
 constexpr uint32_t L1_PHASE_B_BYTES = (L1_BM_WORDS + 2 * L1_THREADS + 16) * 4 + L1_THREADS;
 
 template <int HB>
-__global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
-                                                             uint32_t chunk, uint32_t nchunks,
-                                                             uint32_t* __restrict__ tokw,
-                                                             uint32_t* __restrict__ stage, uint64_t stage_words,
-                                                             ChunkInfo* __restrict__ info) {
-    extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void l1_lz77_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
+    const ChunkDesc* __restrict__ desc, uint32_t chunk, uint32_t* __restrict__ tokw,
+    uint32_t* __restrict__ stage, uint64_t stage_words, ChunkInfo* __restrict__ info) {
     constexpr int W = L1Cfg<HB>::W;
     constexpr uint32_t TAB = L1Cfg<HB>::TAB;
-    const uint32_t c = blockIdx.x;
-    const uint64_t base = (uint64_t)c * chunk;
-    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
-    const uint8_t* s = src + base;
+    const uint32_t n = desc[c].len;
+    const uint8_t* s = src + desc[c].in_off;
     uint32_t* T = tokw + (uint64_t)c * ((chunk + 7u) & ~7u);  // 32-byte aligned per chunk
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
@@ -441,7 +461,7 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
     __syncthreads();
     const uint32_t excl = scan[wv] + incl - mybits;
     const uint64_t fixed = 3ull + scan[L1_WAVES] + 7ull;
-    const bool is_last = c == nchunks - 1;
+    const bool is_last = (desc[c].buf & CHUNK_LAST) != 0;
     const bool stored = (uint64_t)(n + 5) * 8ull <= fixed;
     if (tid == 0) {
         info[c].fixed_bits = fixed;
@@ -500,6 +520,20 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
     }
 }
 
+template <int HB>
+__global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __restrict__ src,
+    const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
+    uint32_t chunk, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
+    ChunkInfo* __restrict__ info) {
+    extern __shared__ uint32_t smem[];
+    for (;;) {
+        const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
+        if (k == NO_ITEM) break;
+        l1_lz77_body<HB>(order[k], smem, src, desc, chunk, tokw, stage, stage_words, info);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // L1-class kernel for chunks up to 128 KiB: one 1024-thread workgroup per
 // chunk, the chunk's match lengths resident in LDS.
@@ -555,18 +589,13 @@ __device__ __forceinline__ uint32_t step_at(const uint8_t* code, const uint32_t*
 }
 
 template <int HB, bool EXCH>
-__global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
-                                                              uint32_t chunk, uint32_t nchunks,
-                                                              uint32_t* __restrict__ tokw,
-                                                              uint32_t* __restrict__ stage, uint64_t stage_words,
-                                                              ChunkInfo* __restrict__ info) {
-    extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void l1_chunk_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
+    const ChunkDesc* __restrict__ desc, uint32_t chunk, uint32_t* __restrict__ tokw,
+    uint32_t* __restrict__ stage, uint64_t stage_words, ChunkInfo* __restrict__ info) {
     constexpr int W = LCCfg<HB>::W;
     constexpr uint32_t TAB = LCCfg<HB>::TAB;
-    const uint32_t c = blockIdx.x;
-    const uint64_t base = (uint64_t)c * chunk;
-    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
-    const uint8_t* s = src + base;
+    const uint32_t n = desc[c].len;
+    const uint8_t* s = src + desc[c].in_off;
     uint32_t* T = tokw + (uint64_t)c * ((chunk + 7u) & ~7u);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     uint32_t* tabs = smem;
@@ -904,7 +933,7 @@ __global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __r
     __syncthreads();
     const uint32_t excl = misc[wv] + incl - mybits;
     const uint64_t fixed = 3ull + misc[LC_WAVES] + 7ull;
-    const bool is_last = c == nchunks - 1;
+    const bool is_last = (desc[c].buf & CHUNK_LAST) != 0;
     const bool stored = (uint64_t)(n + 5) * 8ull <= fixed;
     if (tid == 0) {
         info[c].fixed_bits = fixed;
@@ -960,6 +989,20 @@ __global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __r
         }
         for (int k = 0; k < cnt; ++k) put(T[pk[k]]);
         if (nacc) atomicOr(&stg[wi], (uint32_t)acc);
+    }
+}
+
+template <int HB, bool EXCH>
+__global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __restrict__ src,
+    const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
+    uint32_t chunk, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
+    ChunkInfo* __restrict__ info) {
+    extern __shared__ uint32_t smem[];
+    for (;;) {
+        const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
+        if (k == NO_ITEM) break;
+        l1_chunk_body<HB, EXCH>(order[k], smem, src, desc, chunk, tokw, stage, stage_words, info);
+        __syncthreads();
     }
 }
 
@@ -1040,18 +1083,13 @@ __device__ __forceinline__ uint32_t range_bits(const uint32_t* bm, const uint32_
 }
 
 template <int W, bool EXCH>
-__global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
-                                                                uint32_t chunk, uint32_t nchunks,
-                                                                uint32_t* __restrict__ tokw,
-                                                                uint32_t* __restrict__ stage, uint64_t stage_words,
-                                                                ChunkInfo* __restrict__ info) {
-    extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
+    const ChunkDesc* __restrict__ desc, uint32_t chunk, uint32_t* __restrict__ tokw,
+    uint32_t* __restrict__ stage, uint64_t stage_words, ChunkInfo* __restrict__ info) {
     constexpr int HB = 12;
     constexpr uint32_t TAB = 1u << HB;
-    const uint32_t c = blockIdx.x;
-    const uint64_t base = (uint64_t)c * chunk;
-    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
-    const uint8_t* s = src + base;
+    const uint32_t n = desc[c].len;
+    const uint8_t* s = src + desc[c].in_off;
     uint32_t* T = tokw + (uint64_t)c * ((chunk + 7u) & ~7u);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     uint32_t* D = smem;                         // chunk bytes, then length codes
@@ -1327,7 +1365,7 @@ __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* _
     __syncthreads();
     const uint32_t excl = misc[wv] + incl - mybits;
     const uint64_t fixed = 3ull + misc[LC_WAVES] + 7ull;
-    const bool is_last = c == nchunks - 1;
+    const bool is_last = (desc[c].buf & CHUNK_LAST) != 0;
     const bool stored = (uint64_t)(n + 5) * 8ull <= fixed;
     if (tid == 0) {
         info[c].fixed_bits = fixed;
@@ -1386,6 +1424,20 @@ __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* _
     }
 }
 
+template <int W, bool EXCH>
+__global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* __restrict__ src,
+    const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
+    uint32_t chunk, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
+    ChunkInfo* __restrict__ info) {
+    extern __shared__ uint32_t smem[];
+    for (;;) {
+        const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
+        if (k == NO_ITEM) break;
+        l1_barrier_body<W, EXCH>(order[k], smem, src, desc, chunk, tokw, stage, stage_words, info);
+        __syncthreads();
+    }
+}
+
 // Start-up check: same-address LDS exchanges issued by one wave instruction
 // take effect in ascending lane order.
 __global__ __launch_bounds__(64) void exchange_order_kernel(uint32_t* __restrict__ violations) {
@@ -1402,120 +1454,155 @@ __global__ __launch_bounds__(64) void exchange_order_kernel(uint32_t* __restrict
     if (old != want) atomicOr(&violations[0], 1u);
 }
 
-__global__ void mark_stored_kernel(uint64_t n_total, uint32_t chunk, uint32_t nchunks, ChunkInfo* info) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchunks) return;
-    const uint64_t base = (uint64_t)c * chunk;
-    info[c].raw_len = (uint32_t)min<uint64_t>(chunk, n_total - base);
-    info[c].is_stored = 1;
-    info[c].fixed_bits = 0;
+// Level 0: every chunk is stored blocks (grid-stride over chunks).
+__global__ void mark_stored_kernel(const ChunkDesc* __restrict__ desc, const BatchHdr* __restrict__ hdr,
+                                   ChunkInfo* __restrict__ info) {
+    const uint32_t nchunks = hdr->nchunks;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
+        info[c].raw_len = desc[c].len;
+        info[c].is_stored = 1;
+        info[c].fixed_bits = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Scan of chunk offsets, checksum combine, framing.
+// Per buffer: scan of chunk bit offsets, checksum combine, framing.
+//
+// A chunk's size in bits depends on the incoming offset mod 8 only (stored
+// blocks pad to a byte), so each chunk is the table f[r] = bits when it
+// starts at an offset = r (mod 8).  Tables compose associatively,
+// (A then B)[r] = A[r] + B[(r + A[r]) mod 8], and a Hillis-Steele scan of
+// the compositions gives every chunk's offset.
 // ---------------------------------------------------------------------------
+constexpr int FIN_THREADS = 256;
 
-constexpr int FIN_THREADS = 1024;
+__device__ __forceinline__ void chunk_table(const ChunkInfo& ci, uint64_t f[8]) {
+    if (ci.is_stored == 1) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) f[r] = stored_bits((uint64_t)r, ci.raw_len);
+    } else if (ci.is_stored == 2) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) f[r] = ci.sz8[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) f[r] = ci.fixed_bits;
+    }
+}
 
-__global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(ChunkInfo* __restrict__ info, uint32_t nchunks,
-                                                               uint64_t n, uint32_t chunk, int format,
-                                                               uint8_t* __restrict__ out, uint64_t cap,
+__global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(ChunkInfo* __restrict__ info,
+                                                               const ChunkDesc* __restrict__ desc,
+                                                               const BufDesc* __restrict__ bufs,
+                                                               const BatchHdr* __restrict__ hdr, int format,
+                                                               uint8_t* __restrict__ out_arena,
                                                                Result* __restrict__ res) {
-    __shared__ uint64_t sz[FIN_THREADS];
-    __shared__ uint64_t sz8[FIN_THREADS][8];
-    __shared__ uint32_t st[FIN_THREADS];
-    __shared__ uint32_t ln[FIN_THREADS];
+    __shared__ uint64_t F[FIN_THREADS][8];
     __shared__ uint32_t red0[FIN_THREADS];
     __shared__ uint32_t red1[FIN_THREADS];
-    __shared__ uint64_t carry;
     const int tid = threadIdx.x;
-    if (tid == 0) carry = 0;
-    __syncthreads();
-    for (uint32_t t0 = 0; t0 < nchunks; t0 += FIN_THREADS) {
-        const uint32_t i = t0 + (uint32_t)tid;
-        if (i < nchunks) {
-            sz[tid] = info[i].fixed_bits;
-            st[tid] = info[i].is_stored;
-            ln[tid] = info[i].raw_len;
-            if (st[tid] == 2)
-                for (int r = 0; r < 8; ++r) sz8[tid][r] = info[i].sz8[r];
-        }
-        __syncthreads();
-        if (tid == 0) {
-            uint64_t off = carry;
-            const uint32_t cnt = min((uint32_t)FIN_THREADS, nchunks - t0);
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const uint64_t b = st[k] == 1 ? stored_bits(off, ln[k]) : (st[k] == 2 ? sz8[k][off & 7] : sz[k]);
-                sz[k] = off;
-                off += b;
-            }
-            carry = off;
-        }
-        __syncthreads();
-        if (i < nchunks) info[i].bit_off = sz[tid];
-        __syncthreads();
-    }
-    const uint64_t payload_bits = carry;
-    // checksum combine: chunk i covers [i*chunk, i*chunk + len_i)
-    uint32_t x0 = 0, x1 = 0;
-    for (uint32_t i = (uint32_t)tid; i < nchunks; i += FIN_THREADS) {
-        const uint64_t endi = min<uint64_t>(n, (uint64_t)(i + 1) * chunk);
-        const uint64_t after = n - endi;
-        if (format == 2) {
-            x0 ^= gf2_mul(gf2_x8n(after), info[i].crc);
-        } else if (format == 1) {
-            const uint32_t s1 = info[i].adler_s1;
-            x0 = (x0 + s1) % ADLER_MOD;
-            x1 = (uint32_t)((x1 + (uint64_t)s1 * (after % ADLER_MOD) + info[i].adler_s2) % ADLER_MOD);
-        }
-    }
-    red0[tid] = x0;
-    red1[tid] = x1;
-    __syncthreads();
-    for (int w = FIN_THREADS / 2; w > 0; w >>= 1) {
-        if (tid < w) {
-            if (format == 2) {
-                red0[tid] ^= red0[tid + w];
+    const uint32_t nbufs = hdr->nbufs;
+    for (uint32_t bi = blockIdx.x; bi < nbufs; bi += gridDim.x) {
+        const BufDesc B = bufs[bi];
+        uint64_t carry = 0;
+        for (uint32_t t0 = 0; t0 < B.nchunks; t0 += FIN_THREADS) {
+            const uint32_t i = t0 + (uint32_t)tid;
+            const uint32_t cnt = min((uint32_t)FIN_THREADS, B.nchunks - t0);
+            uint64_t f[8];
+            if (i < B.nchunks) {
+                chunk_table(info[B.chunk0 + i], f);
             } else {
-                red0[tid] = (red0[tid] + red0[tid + w]) % ADLER_MOD;
-                red1[tid] = (red1[tid] + red1[tid + w]) % ADLER_MOD;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) f[r] = 0;
+            }
+            for (int d = 1; d < FIN_THREADS; d <<= 1) {
+#pragma unroll
+                for (int r = 0; r < 8; ++r) F[tid][r] = f[r];
+                __syncthreads();
+                if (tid >= d) {
+                    uint64_t a[8];
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) a[r] = F[tid - d][r];
+                    uint64_t g[8];
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) g[r] = a[r] + f[(r + a[r]) & 7];
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) f[r] = g[r];
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) F[tid][r] = f[r];
+            __syncthreads();
+            const uint32_t r0 = (uint32_t)(carry & 7);
+            if (i < B.nchunks) info[B.chunk0 + i].bit_off = carry + (tid ? F[tid - 1][r0] : 0ull);
+            carry += F[cnt - 1][r0];
+            __syncthreads();
+        }
+        const uint64_t payload_bits = carry;
+        const uint64_t n = B.in_len;
+        // checksum combine: a chunk ends `after` bytes before the buffer's end
+        uint32_t x0 = 0, x1 = 0;
+        for (uint32_t i = (uint32_t)tid; i < B.nchunks; i += FIN_THREADS) {
+            const uint32_t c = B.chunk0 + i;
+            const uint64_t after = B.in_off + n - (desc[c].in_off + desc[c].len);
+            if (format == 2) {
+                x0 ^= gf2_mul(gf2_x8n(after), info[c].crc);
+            } else if (format == 1) {
+                const uint32_t s1 = info[c].adler_s1;
+                x0 = (x0 + s1) % ADLER_MOD;
+                x1 = (uint32_t)((x1 + (uint64_t)s1 * (after % ADLER_MOD) + info[c].adler_s2) % ADLER_MOD);
+            }
+        }
+        red0[tid] = x0;
+        red1[tid] = x1;
+        __syncthreads();
+        for (int w = FIN_THREADS / 2; w > 0; w >>= 1) {
+            if (tid < w) {
+                if (format == 2) {
+                    red0[tid] ^= red0[tid + w];
+                } else {
+                    red0[tid] = (red0[tid] + red0[tid + w]) % ADLER_MOD;
+                    red1[tid] = (red1[tid] + red1[tid + w]) % ADLER_MOD;
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            uint32_t ck = 0;
+            if (format == 2) {
+                ck = ~(gf2_mul(gf2_x8n(n), 0xFFFFFFFFu) ^ red0[0]);
+            } else if (format == 1) {
+                const uint32_t s1 = (1u + red0[0]) % ADLER_MOD;
+                const uint32_t s2 = (uint32_t)((n % ADLER_MOD + red1[0]) % ADLER_MOD);
+                ck = (s2 << 16) | s1;
+            }
+            const uint32_t hb = format == 1 ? 2u : (format == 2 ? 10u : 0u);
+            const uint32_t tb = format == 1 ? 4u : (format == 2 ? 8u : 0u);
+            const uint64_t payload = (payload_bits + 7) >> 3;
+            const uint64_t total = hb + payload + tb;
+            Result& R = res[bi];
+            R.payload_bits = payload_bits;
+            R.checksum = ck;
+            R.total_bytes = total <= B.cap ? total : 0;
+            if (total <= B.cap) {
+                uint8_t* out = out_arena + B.out_off;
+                if (format == 1) {
+                    out[0] = 0x78;
+                    out[1] = 0x9C;
+                    uint8_t* t = out + hb + payload;
+                    t[0] = (uint8_t)(ck >> 24);
+                    t[1] = (uint8_t)(ck >> 16);
+                    t[2] = (uint8_t)(ck >> 8);
+                    t[3] = (uint8_t)ck;
+                } else if (format == 2) {
+                    const uint8_t g[10] = {0x1F, 0x8B, 8, 0, 0, 0, 0, 0, 0, 255};
+                    for (int k = 0; k < 10; ++k) out[k] = g[k];
+                    uint8_t* t = out + hb + payload;
+                    for (int k = 0; k < 4; ++k) t[k] = (uint8_t)(ck >> (8 * k));
+                    for (int k = 0; k < 4; ++k) t[4 + k] = (uint8_t)((uint32_t)n >> (8 * k));
+                }
             }
         }
         __syncthreads();
-    }
-    if (tid == 0) {
-        uint32_t ck = 0;
-        if (format == 2) {
-            ck = ~(gf2_mul(gf2_x8n(n), 0xFFFFFFFFu) ^ red0[0]);
-        } else if (format == 1) {
-            const uint32_t s1 = (1u + red0[0]) % ADLER_MOD;
-            const uint32_t s2 = (uint32_t)((n % ADLER_MOD + red1[0]) % ADLER_MOD);
-            ck = (s2 << 16) | s1;
-        }
-        const uint32_t hdr = format == 1 ? 2u : (format == 2 ? 10u : 0u);
-        const uint32_t trl = format == 1 ? 4u : (format == 2 ? 8u : 0u);
-        const uint64_t payload = (payload_bits + 7) >> 3;
-        const uint64_t total = hdr + payload + trl;
-        res->payload_bits = payload_bits;
-        res->checksum = ck;
-        res->total_bytes = total <= cap ? total : 0;
-        if (total <= cap) {
-            if (format == 1) {
-                out[0] = 0x78;
-                out[1] = 0x9C;
-                uint8_t* t = out + hdr + payload;
-                t[0] = (uint8_t)(ck >> 24);
-                t[1] = (uint8_t)(ck >> 16);
-                t[2] = (uint8_t)(ck >> 8);
-                t[3] = (uint8_t)ck;
-            } else if (format == 2) {
-                const uint8_t g[10] = {0x1F, 0x8B, 8, 0, 0, 0, 0, 0, 0, 255};
-                for (int k = 0; k < 10; ++k) out[k] = g[k];
-                uint8_t* t = out + hdr + payload;
-                for (int k = 0; k < 4; ++k) t[k] = (uint8_t)(ck >> (8 * k));
-                for (int k = 0; k < 4; ++k) t[4 + k] = (uint8_t)((uint32_t)n >> (8 * k));
-            }
-        }
     }
 }
 
@@ -1574,40 +1661,47 @@ __device__ __forceinline__ uint32_t chunk_bits(const ChunkView& cv, uint64_t r, 
     return stored_byte(cv, (r - 3 - pad0) >> 3) & mask;
 }
 
-__device__ __forceinline__ ChunkView make_view(const uint8_t* src, uint32_t chunk, uint32_t nchunks,
-                                               const uint32_t* stage, uint64_t stage_words, const ChunkInfo* info,
-                                               uint32_t c) {
+__device__ __forceinline__ ChunkView make_view(const uint8_t* src, const ChunkDesc* desc, const uint32_t* stage,
+                                               uint64_t stage_words, const ChunkInfo* info, uint32_t c) {
     ChunkView v;
     v.stg = stage + (uint64_t)c * stage_words;
-    v.raw = src + (uint64_t)c * chunk;
+    v.raw = src + desc[c].in_off;
     v.off = info[c].bit_off;
     v.len = info[c].raw_len;
     v.stored = info[c].is_stored == 1 ? 1u : 0u;
-    v.last = c == nchunks - 1 ? 1u : 0u;
+    v.last = (desc[c].buf & CHUNK_LAST) ? 1u : 0u;
     return v;
 }
 
-__global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict__ src, uint32_t chunk,
-                                                       uint32_t nchunks, const uint32_t* __restrict__ stage,
-                                                       uint64_t stage_words, const ChunkInfo* __restrict__ info,
-                                                       const Result* __restrict__ res, uint8_t* __restrict__ out,
-                                                       uint32_t hdr) {
-    if (res->total_bytes == 0) return;
-    const uint32_t c = blockIdx.x;
-    const ChunkView cv = make_view(src, chunk, nchunks, stage, stage_words, info, c);
-    const bool has_next = c + 1 < nchunks;
-    ChunkView nv = cv;
-    if (has_next) nv = make_view(src, chunk, nchunks, stage, stage_words, info, c + 1);
-    const uint64_t o = cv.off;
-    const uint64_t e = has_next ? nv.off : res->payload_bits;
-    const uint64_t b0 = (o + 7) >> 3, b1 = (e + 7) >> 3;
-    uint8_t* dst = out + hdr;
-    for (uint64_t b = b0 + threadIdx.x; b < b1; b += 256) {
-        const uint64_t r = 8 * b - o;
-        const uint32_t k = (uint32_t)min<uint64_t>(8, e - 8 * b);
-        uint32_t v = chunk_bits(cv, r, k);
-        if (k < 8 && has_next) v |= chunk_bits(nv, 0, 8 - k) << k;
-        dst[b] = (uint8_t)v;
+// Each chunk's bits, shifted to its offset in its buffer's output (grid-stride
+// over chunks; the byte a chunk shares with the next is written by the first).
+__global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict__ src,
+                                                       const ChunkDesc* __restrict__ desc,
+                                                       const BufDesc* __restrict__ bufs,
+                                                       const BatchHdr* __restrict__ hdr,
+                                                       const uint32_t* __restrict__ stage, uint64_t stage_words,
+                                                       const ChunkInfo* __restrict__ info,
+                                                       const Result* __restrict__ res, uint8_t* __restrict__ out_arena,
+                                                       uint32_t hdr_bytes) {
+    const uint32_t nchunks = hdr->nchunks;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint32_t bi = desc[c].buf & ~CHUNK_LAST;
+        if (res[bi].total_bytes == 0) continue;
+        const ChunkView cv = make_view(src, desc, stage, stage_words, info, c);
+        const bool has_next = !cv.last;
+        ChunkView nv = cv;
+        if (has_next) nv = make_view(src, desc, stage, stage_words, info, c + 1);
+        const uint64_t o = cv.off;
+        const uint64_t e = has_next ? nv.off : res[bi].payload_bits;
+        const uint64_t b0 = (o + 7) >> 3, b1 = (e + 7) >> 3;
+        uint8_t* dst = out_arena + bufs[bi].out_off + hdr_bytes;
+        for (uint64_t b = b0 + threadIdx.x; b < b1; b += 256) {
+            const uint64_t r = 8 * b - o;
+            const uint32_t k = (uint32_t)min<uint64_t>(8, e - 8 * b);
+            uint32_t v = chunk_bits(cv, r, k);
+            if (k < 8 && has_next) v |= chunk_bits(nv, 0, 8 - k) << k;
+            dst[b] = (uint8_t)v;
+        }
     }
 }
 
@@ -1659,48 +1753,59 @@ inline void record(hipEvent_t ev, hipStream_t s) {
     if (ev) (void)hipEventRecord(ev, s);
 }
 
+// Workgroups of `threads` threads and `smem` bytes of LDS resident per CU.
+inline uint32_t per_cu(const Plan& p, size_t smem, int threads) {
+    const uint32_t by_lds = smem ? (uint32_t)(p.lds_max / smem) : 32u;
+    const uint32_t by_waves = (uint32_t)(32 / (threads / 64));
+    const uint32_t k = by_lds < by_waves ? by_lds : by_waves;
+    return k ? k : 1u;
+}
+
+template <class K>
+hipError_t allow_lds(K* kernel, size_t smem) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)smem);
+}
+
 template <int HB, bool EXCH>
 hipError_t launch_chunk(const Plan& p) {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l1_chunk_kernel<HB, EXCH>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)LC_LDS_BYTES);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
-    hipLaunchKernelGGL((l1_chunk_kernel<HB, EXCH>), dim3(p.nchunks), dim3(LC_THREADS), LC_LDS_BYTES, p.stream, p.d_in,
-                       p.n, p.chunk, p.nchunks, p.d_tok, p.d_stage, p.stage_words, p.d_info);
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, LC_LDS_BYTES, LC_THREADS));
+    hipLaunchKernelGGL((l1_chunk_kernel<HB, EXCH>), dim3(g), dim3(LC_THREADS), LC_LDS_BYTES, p.stream, p.d_in,
+                       p.d_desc, p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_stage, p.stage_words, p.d_info);
     return hipSuccess;
 }
 
 template <int W, bool EXCH>
 hipError_t launch_barrier(const Plan& p) {
-    static bool attr = false;
     const size_t smem = LBCfg<W>::LDS_BYTES;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l1_barrier_kernel<W, EXCH>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
-    hipLaunchKernelGGL((l1_barrier_kernel<W, EXCH>), dim3(p.nchunks), dim3(LC_THREADS), smem, p.stream, p.d_in,
-                       p.n, p.chunk, p.nchunks, p.d_tok, p.d_stage, p.stage_words, p.d_info);
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, smem, LC_THREADS));
+    hipLaunchKernelGGL((l1_barrier_kernel<W, EXCH>), dim3(g), dim3(LC_THREADS), smem, p.stream, p.d_in, p.d_desc,
+                       p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_stage, p.stage_words, p.d_info);
     return hipSuccess;
 }
 
 template <int HB>
 hipError_t launch_wide(const Plan& p) {
-    static bool attr = false;
     const size_t smem = L1Cfg<HB>::TAB_BYTES > L1_PHASE_B_BYTES ? L1Cfg<HB>::TAB_BYTES : L1_PHASE_B_BYTES;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l1_lz77_kernel<HB>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
-    hipLaunchKernelGGL(l1_lz77_kernel<HB>, dim3(p.nchunks), dim3(L1_THREADS), smem, p.stream, p.d_in, p.n, p.chunk,
-                       p.nchunks, p.d_tok, p.d_stage, p.stage_words, p.d_info);
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, smem, L1_THREADS));
+    hipLaunchKernelGGL(l1_lz77_kernel<HB>, dim3(g), dim3(L1_THREADS), smem, p.stream, p.d_in, p.d_desc, p.d_order,
+                       p.d_hdr, p.chunk, p.d_tok, p.d_stage, p.stage_words, p.d_info);
     return hipSuccess;
+}
+
+hipError_t launch_l1_class(const Plan& p) {
+    if (p.hash_bits == 12) {
+        if (p.chunk <= LC_MAX_CHUNK) {
+            if (p.lds_max >= LBCfg<2>::LDS_BYTES)
+                return p.exch_ok ? launch_barrier<2, true>(p) : launch_barrier<2, false>(p);
+            return p.exch_ok ? launch_barrier<1, true>(p) : launch_barrier<1, false>(p);
+        }
+        if (p.chunk <= MAX_CHUNK_L1) return launch_wide<12>(p);
+        return hipErrorInvalidValue;
+    }
+    if (p.hash_bits == 14 && p.chunk <= LC_MAX_CHUNK)
+        return p.exch_ok ? launch_chunk<14, true>(p) : launch_chunk<14, false>(p);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -1722,55 +1827,53 @@ hipError_t check_exchange_order(hipStream_t stream, bool* ok) {
     return e;
 }
 
-namespace {
-}  // namespace
+// LDS limits of the L1-class kernels; set once per process before any launch
+// (never while a stream is being captured).
+hipError_t set_kernel_attributes_l1(size_t lds_max) {
+    hipError_t e = allow_lds(&l1_chunk_kernel<14, true>, LC_LDS_BYTES);
+    if (e == hipSuccess) e = allow_lds(&l1_chunk_kernel<14, false>, LC_LDS_BYTES);
+    if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel<1, true>, LBCfg<1>::LDS_BYTES);
+    if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel<1, false>, LBCfg<1>::LDS_BYTES);
+    if (e == hipSuccess && lds_max >= LBCfg<2>::LDS_BYTES) e = allow_lds(&l1_barrier_kernel<2, true>, LBCfg<2>::LDS_BYTES);
+    if (e == hipSuccess && lds_max >= LBCfg<2>::LDS_BYTES) e = allow_lds(&l1_barrier_kernel<2, false>, LBCfg<2>::LDS_BYTES);
+    const size_t wide = L1Cfg<12>::TAB_BYTES > L1_PHASE_B_BYTES ? L1Cfg<12>::TAB_BYTES : L1_PHASE_B_BYTES;
+    if (e == hipSuccess) e = allow_lds(&l1_lz77_kernel<12>, wide);
+    return e;
+}
 
 uint32_t max_chunk_for_hash_bits(int hash_bits) { return hash_bits <= 12 ? MAX_CHUNK_L1 : MAX_CHUNK_L1 / 2; }
 
 uint64_t stage_words_for_chunk(uint32_t chunk) { return ((uint64_t)chunk + chunk / 512 + 1024) / 4; }
 
 hipError_t launch_parallel_compress(const Plan& p) {
-    const uint32_t nb = p.nchunks;
     const hipStream_t s = p.stream;
+    const uint32_t light = p.cus * 8u;
     record(p.ev[0], s);
     if (p.format != 0)
-        hipLaunchKernelGGL(checksum_kernel, dim3(nb), dim3(256), 0, s, p.d_in, p.n, p.chunk, p.format == 2 ? 0 : 1,
-                           p.d_info);
+        hipLaunchKernelGGL(checksum_kernel, dim3(grid_for(p.persistent_grid, p.nchunks, light)), dim3(256), 0, s,
+                           p.d_in, p.d_desc, p.d_hdr, p.format == 2 ? 0 : 1, p.d_info);
     record(p.ev[1], s);
+    hipError_t e = hipSuccess;
     if (p.level == 0) {
-        hipLaunchKernelGGL(mark_stored_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, p.n, p.chunk, nb, p.d_info);
+        hipLaunchKernelGGL(mark_stored_kernel, dim3(grid_for(p.persistent_grid, (p.nchunks + 255) / 256, light)),
+                           dim3(256), 0, s, p.d_desc, p.d_hdr, p.d_info);
     } else if (p.level <= 3) {
-        const int hb = p.level == 1 ? 12 : 14;
-        hipError_t e = hipSuccess;
-        if (p.chunk <= LC_MAX_CHUNK) {
-            if (hb == 12) {
-                if (p.lds_max >= LBCfg<2>::LDS_BYTES)
-                    e = p.exch_ok ? launch_barrier<2, true>(p) : launch_barrier<2, false>(p);
-                else
-                    e = p.exch_ok ? launch_barrier<1, true>(p) : launch_barrier<1, false>(p);
-            }
-            else e = p.exch_ok ? launch_chunk<14, true>(p) : launch_chunk<14, false>(p);
-        } else if (hb == 12) {
-            e = launch_wide<12>(p);
-        } else {
-            return hipErrorInvalidValue;
-        }
-        if (e != hipSuccess) return e;
+        e = launch_l1_class(p);
     } else {
-        hipError_t e = launch_l6_front(p);
-        if (e != hipSuccess) return e;
+        e = launch_l6_front(p);
     }
+    if (e != hipSuccess) return e;
     record(p.ev[2], s);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(FIN_THREADS), 0, s, p.d_info, nb, p.n, p.chunk, p.format,
-                       p.d_out, p.cap, p.d_result);
+    hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(p.persistent_grid, p.nbufs, light)), dim3(FIN_THREADS), 0, s,
+                       p.d_info, p.d_desc, p.d_bufs, p.d_hdr, p.format, p.d_out, p.d_result);
     if (p.level >= 4) {
-        hipError_t e = launch_l6_back(p);
+        e = launch_l6_back(p);
         if (e != hipSuccess) return e;
     }
     record(p.ev[3], s);
     const uint32_t hdr = p.format == 1 ? 2u : (p.format == 2 ? 10u : 0u);
-    hipLaunchKernelGGL(assemble_kernel, dim3(nb), dim3(256), 0, s, p.d_in, p.chunk, nb, p.d_stage, p.stage_words,
-                       p.d_info, p.d_result, p.d_out, hdr);
+    hipLaunchKernelGGL(assemble_kernel, dim3(grid_for(p.persistent_grid, p.nchunks, light)), dim3(256), 0, s, p.d_in,
+                       p.d_desc, p.d_bufs, p.d_hdr, p.d_stage, p.stage_words, p.d_info, p.d_result, p.d_out, hdr);
     record(p.ev[4], s);
     return hipGetLastError();
 }
@@ -1780,7 +1883,7 @@ hipError_t launch_checksum(const uint8_t* d_in, uint64_t n, int kind, uint32_t i
     const uint32_t chunk = 65536;
     const uint32_t nb = n ? (uint32_t)((n + chunk - 1) / chunk) : 1;
     if (nb > tmp_chunks) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(checksum_kernel, dim3(nb), dim3(256), 0, stream, d_in, n, chunk, kind, d_tmp);
+    hipLaunchKernelGGL(checksum_uniform_kernel, dim3(nb), dim3(256), 0, stream, d_in, n, chunk, kind, d_tmp);
     hipLaunchKernelGGL(checksum_combine_kernel, dim3(1), dim3(FIN_THREADS), 0, stream, d_tmp, nb, n, chunk, kind,
                        init, d_out_value);
     return hipGetLastError();

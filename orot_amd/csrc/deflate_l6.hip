@@ -54,13 +54,6 @@ constexpr uint32_t L6R_LDS = (L6R_BM_WORDS + 3 * L6R_THREADS + 64) * 4;
 constexpr uint32_t BLK_SYMS = 16384;
 constexpr int L6B_THREADS = 256;
 
-struct L6Level {
-    int max_chain, nice_len, lazy_depth;
-};
-__constant__ L6Level c_l6lv[13] = {
-    {0, 0, 0},    {0, 0, 0},     {0, 0, 0},     {0, 0, 0},     {16, 128, 1},   {32, 128, 1},   {32, 258, 1},
-    {64, 258, 2}, {128, 258, 2}, {256, 258, 3}, {512, 258, 3}, {1024, 258, 3}, {2048, 258, 3},
-};
 
 __device__ __forceinline__ uint32_t ml_len(uint32_t m) { return m & 0xFFFFu; }
 __device__ __forceinline__ uint32_t ml_dist(uint32_t m) { return m >> 16; }
@@ -69,14 +62,11 @@ __device__ __forceinline__ uint32_t ml_dist(uint32_t m) { return m >> 16; }
 // C(p) and the in-window chain step PD[p].
 // ---------------------------------------------------------------------------
 template <bool EXCH>
-__global__ __launch_bounds__(L6P_THREADS) void l6_pred_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
-                                                              uint32_t chunk, uint32_t* __restrict__ tokw,
-                                                              uint16_t* __restrict__ pd) {
-    extern __shared__ uint32_t smem[];
-    const uint32_t c = blockIdx.x;
-    const uint64_t base = (uint64_t)c * chunk;
-    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
-    const uint8_t* s = src + base;
+__device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
+                                             const ChunkDesc* __restrict__ desc, uint32_t chunk,
+                                             uint32_t* __restrict__ tokw, uint16_t* __restrict__ pd) {
+    const uint32_t n = desc[c].len;
+    const uint8_t* s = src + desc[c].in_off;
     const uint64_t cp = (chunk + 7u) & ~7u;
     uint32_t* T = tokw + (uint64_t)c * cp;
     uint16_t* PD = pd + (uint64_t)c * cp;
@@ -151,6 +141,21 @@ __global__ __launch_bounds__(L6P_THREADS) void l6_pred_kernel(const uint8_t* __r
     }
 }
 
+template <bool EXCH>
+__global__ __launch_bounds__(L6P_THREADS) void l6_pred_kernel(const uint8_t* __restrict__ src,
+                                                              const ChunkDesc* __restrict__ desc,
+                                                              const uint32_t* __restrict__ order,
+                                                              BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                              uint32_t* __restrict__ tokw, uint16_t* __restrict__ pd) {
+    extern __shared__ uint32_t smem[];
+    for (;;) {
+        const uint32_t k = next_item(&hdr->queue[Q_PRED], hdr->nchunks, smem);
+        if (k == NO_ITEM) break;
+        l6_pred_body<EXCH>(order[k], smem, src, desc, chunk, tokw, pd);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // M(x) for every position: LDS rings of bytes (64 KiB) and chain steps.
 // ---------------------------------------------------------------------------
@@ -169,21 +174,19 @@ __device__ __forceinline__ uint64_t ring_u64(const uint32_t* R, uint32_t a) {
     return (uint64_t)__builtin_amdgcn_alignbyte(w1, w0, k) | ((uint64_t)__builtin_amdgcn_alignbyte(w2, w1, k) << 32);
 }
 
-__global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
-                                                               uint32_t chunk, const uint16_t* __restrict__ pd,
-                                                               uint32_t* __restrict__ mlw, int level) {
-    extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
+                                              const ChunkDesc* __restrict__ desc, uint32_t chunk,
+                                              const uint16_t* __restrict__ pd, uint32_t* __restrict__ mlw,
+                                              const L6Params prm) {
     uint32_t* DR = smem;                                            // byte ring
     uint16_t* PR = reinterpret_cast<uint16_t*>(smem + L6M_DRING / 4);  // chain-step ring
-    const uint32_t c = blockIdx.x;
-    const uint64_t base = (uint64_t)c * chunk;
-    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
-    const uint8_t* s = src + base;
+    const uint32_t n = desc[c].len;
+    const uint8_t* s = src + desc[c].in_off;
     const uint64_t cp = (chunk + 7u) & ~7u;
     const uint16_t* PD = pd + (uint64_t)c * cp;
     uint32_t* ML = mlw + (uint64_t)c * cp;
     const int tid = threadIdx.x;
-    const int max_chain = c_l6lv[level].max_chain, nice = c_l6lv[level].nice_len;
+    const int max_chain = prm.max_chain, nice = prm.nice_len;
     uint32_t dfill = 0;  // bytes [0, dfill) have been loaded into the ring (word granular)
     for (uint32_t x0 = 0; x0 < n; x0 += L6M_TILE) {
         const uint32_t xe = min(n, x0 + L6M_TILE);
@@ -245,6 +248,21 @@ __global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __
     }
 }
 
+__global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __restrict__ src,
+                                                               const ChunkDesc* __restrict__ desc,
+                                                               const uint32_t* __restrict__ order,
+                                                               BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                               const uint16_t* __restrict__ pd,
+                                                               uint32_t* __restrict__ mlw, const L6Params prm) {
+    extern __shared__ uint32_t smem[];
+    for (;;) {
+        const uint32_t k = next_item(&hdr->queue[Q_MATCH], hdr->nchunks, smem);
+        if (k == NO_ITEM) break;
+        l6_match_body(order[k], smem, src, desc, chunk, pd, mlw, prm);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Lazy decisions, parse barriers, exact parse, tokens.
 // Decision at a parse position p (lz77.cpp:442-494 with the probe at
@@ -271,28 +289,23 @@ __device__ __forceinline__ uint32_t l6_decide(const uint32_t* ML, uint32_t n, ui
     return (q + b - p) | ((q - p) << 12);
 }
 
-__global__ __launch_bounds__(L6R_THREADS) void l6_parse_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
-                                                               uint32_t chunk, const uint32_t* __restrict__ mlw,
-                                                               uint16_t* __restrict__ codew,
-                                                               uint32_t* __restrict__ toks,
-                                                               uint32_t* __restrict__ ntok_out,
-                                                               uint32_t* __restrict__ blkpos, uint32_t maxblk,
-                                                               int level) {
-    extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
+                                              const ChunkDesc* __restrict__ desc, uint32_t chunk,
+                                              const uint32_t* __restrict__ mlw, uint16_t* __restrict__ codew,
+                                              uint32_t* __restrict__ toks, uint32_t* __restrict__ ntok_out,
+                                              uint32_t* __restrict__ blkpos, uint32_t maxblk, const L6Params prm) {
     uint32_t* bm = smem;
     uint32_t* fb = smem + L6R_BM_WORDS;
     uint32_t* rcnt = fb + L6R_THREADS;
     uint32_t* misc = rcnt + L6R_THREADS;
-    const uint32_t c = blockIdx.x;
-    const uint64_t base = (uint64_t)c * chunk;
-    const uint32_t n = (uint32_t)min<uint64_t>(chunk, n_total - base);
-    const uint8_t* s = src + base;
+    const uint32_t n = desc[c].len;
+    const uint8_t* s = src + desc[c].in_off;
     const uint64_t cp = (chunk + 7u) & ~7u;
     const uint32_t* ML = mlw + (uint64_t)c * cp;
     uint16_t* CODE = codew + (uint64_t)c * cp;
     uint32_t* TK = toks + (uint64_t)c * cp;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int lazy = c_l6lv[level].lazy_depth;
+    const int lazy = prm.lazy_depth;
     const uint32_t S = max(32u, (((n + L6R_THREADS - 1) / L6R_THREADS) + 31u) & ~31u);
     const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
     // bitmap words owned by this range (S is a multiple of 32; an empty range
@@ -397,6 +410,25 @@ __global__ __launch_bounds__(L6R_THREADS) void l6_parse_kernel(const uint8_t* __
     }
 }
 
+__global__ __launch_bounds__(L6R_THREADS) void l6_parse_kernel(const uint8_t* __restrict__ src,
+                                                               const ChunkDesc* __restrict__ desc,
+                                                               const uint32_t* __restrict__ order,
+                                                               BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                               const uint32_t* __restrict__ mlw,
+                                                               uint16_t* __restrict__ codew,
+                                                               uint32_t* __restrict__ toks,
+                                                               uint32_t* __restrict__ ntok_out,
+                                                               uint32_t* __restrict__ blkpos, uint32_t maxblk,
+                                                               const L6Params prm) {
+    extern __shared__ uint32_t smem[];
+    for (;;) {
+        const uint32_t k = next_item(&hdr->queue[Q_PARSE], hdr->nchunks, smem);
+        if (k == NO_ITEM) break;
+        l6_parse_body(order[k], smem, src, desc, chunk, mlw, codew, toks, ntok_out, blkpos, maxblk, prm);
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Blocks: histograms, package-merge lengths, header, sizes, block type.
 // ---------------------------------------------------------------------------
@@ -479,11 +511,10 @@ __device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, ui
     }
 }
 
-__global__ __launch_bounds__(L6B_THREADS) void l6_block_kernel(const uint32_t* __restrict__ toks,
-                                                               const uint32_t* __restrict__ ntok_in,
-                                                               const uint32_t* __restrict__ blkpos, uint32_t chunk,
-                                                               uint64_t n_total, uint32_t maxblk,
-                                                               BlockRec* __restrict__ recs) {
+__device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint32_t* __restrict__ toks,
+                                              const uint32_t* __restrict__ ntok_in,
+                                              const uint32_t* __restrict__ blkpos, uint32_t chunk, uint32_t maxblk,
+                                              BlockRec* __restrict__ recs) {
     __shared__ uint32_t lit[288];
     __shared__ uint32_t dst[32];
     __shared__ uint32_t rawsum;
@@ -492,7 +523,6 @@ __global__ __launch_bounds__(L6B_THREADS) void l6_block_kernel(const uint32_t* _
     __shared__ uint32_t wcur[576];
     __shared__ uint8_t leafbits[16 * 576];
     __shared__ BlockRec R;
-    const uint32_t c = blockIdx.x / maxblk, b = blockIdx.x % maxblk;
     const uint32_t ntok = ntok_in[c];
     const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
     if (b >= nblk) return;
@@ -630,36 +660,54 @@ __global__ __launch_bounds__(L6B_THREADS) void l6_block_kernel(const uint32_t* _
     for (uint32_t i = tid; i < sizeof(BlockRec) / 4; i += L6B_THREADS) dstw[i] = srcw[i];
 }
 
+// Per block (grid-stride over chunk x block slots; slots past a chunk's last
+// block are skipped).
+__global__ __launch_bounds__(L6B_THREADS) void l6_block_kernel(const uint32_t* __restrict__ toks,
+                                                               const uint32_t* __restrict__ ntok_in,
+                                                               const uint32_t* __restrict__ blkpos,
+                                                               const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                               uint32_t maxblk, BlockRec* __restrict__ recs) {
+    const uint32_t items = hdr->nchunks * maxblk;
+    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
+        l6_block_body(w / maxblk, w % maxblk, toks, ntok_in, blkpos, chunk, maxblk, recs);
+        __syncthreads();
+    }
+}
+
 __device__ __forceinline__ uint64_t block_bits(const BlockRec& r, uint64_t off) {
     return r.type == 0 ? stored_bits(off, r.raw_len) : r.body_bits;
 }
 
 // chunk size for each incoming bit offset mod 8
 __global__ void l6_size_kernel(const uint32_t* __restrict__ ntok_in, const BlockRec* __restrict__ recs,
-                               uint32_t nchunks, uint32_t chunk, uint64_t n_total, uint32_t maxblk,
-                               ChunkInfo* __restrict__ info) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchunks) return;
-    const uint32_t ntok = ntok_in[c];
-    const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
-    const BlockRec* R = recs + (uint64_t)c * maxblk;
-    for (uint32_t r = 0; r < 8; ++r) {
-        uint64_t off = r;
-        for (uint32_t b = 0; b < nblk; ++b) off += block_bits(R[b], off);
-        info[c].sz8[r] = off - r;
+                               const ChunkDesc* __restrict__ desc, const BatchHdr* __restrict__ hdr,
+                               uint32_t maxblk, ChunkInfo* __restrict__ info) {
+    const uint32_t nchunks = hdr->nchunks;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
+        const uint32_t ntok = ntok_in[c];
+        const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
+        const BlockRec* R = recs + (uint64_t)c * maxblk;
+        for (uint32_t r = 0; r < 8; ++r) {
+            uint64_t off = r;
+            for (uint32_t b = 0; b < nblk; ++b) off += block_bits(R[b], off);
+            info[c].sz8[r] = off - r;
+        }
+        info[c].is_stored = 2;
+        info[c].raw_len = desc[c].len;
+        info[c].fixed_bits = 0;
     }
-    info[c].is_stored = 2;
-    info[c].raw_len = (uint32_t)min<uint64_t>(chunk, n_total - (uint64_t)c * chunk);
-    info[c].fixed_bits = 0;
 }
 
-__global__ __launch_bounds__(256) void l6_zero_kernel(const ChunkInfo* __restrict__ info, uint32_t* __restrict__ stage,
+__global__ __launch_bounds__(256) void l6_zero_kernel(const ChunkInfo* __restrict__ info,
+                                                      const BatchHdr* __restrict__ hdr, uint32_t* __restrict__ stage,
                                                       uint64_t stage_words) {
-    const uint32_t c = blockIdx.x;
-    const uint64_t bits = info[c].sz8[info[c].bit_off & 7];
-    const uint64_t nw = min<uint64_t>(stage_words, (bits + 31) / 32 + 2);
-    uint32_t* stg = stage + (uint64_t)c * stage_words;
-    for (uint64_t i = threadIdx.x; i < nw; i += 256) stg[i] = 0;
+    const uint32_t nchunks = hdr->nchunks;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint64_t bits = info[c].sz8[info[c].bit_off & 7];
+        const uint64_t nw = min<uint64_t>(stage_words, (bits + 31) / 32 + 2);
+        uint32_t* stg = stage + (uint64_t)c * stage_words;
+        for (uint64_t i = threadIdx.x; i < nw; i += 256) stg[i] = 0;
+    }
 }
 
 // bits [at, at + len) of the chunk staging get v (len <= 32)
@@ -678,14 +726,12 @@ __device__ __forceinline__ uint32_t rev_code(uint32_t code, uint32_t len) {
 
 constexpr int L6E_THREADS = 256;
 
-__global__ __launch_bounds__(L6E_THREADS) void l6_emit_kernel(const uint8_t* __restrict__ src, uint64_t n_total,
-                                                              uint32_t chunk, uint32_t nchunks,
-                                                              const uint32_t* __restrict__ toks,
-                                                              const uint32_t* __restrict__ ntok_in,
-                                                              const BlockRec* __restrict__ recs, uint32_t maxblk,
-                                                              const ChunkInfo* __restrict__ info,
-                                                              const Result* __restrict__ res,
-                                                              uint32_t* __restrict__ stage, uint64_t stage_words) {
+__device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8_t* __restrict__ src,
+                                             const ChunkDesc* __restrict__ desc, uint32_t chunk,
+                                             const uint32_t* __restrict__ toks, const uint32_t* __restrict__ ntok_in,
+                                             const BlockRec* __restrict__ recs, uint32_t maxblk,
+                                             const ChunkInfo* __restrict__ info, const Result* __restrict__ res,
+                                             uint32_t* __restrict__ stage, uint64_t stage_words) {
     __shared__ uint16_t llc[288];
     __shared__ uint16_t dlc[32];
     __shared__ uint8_t lls[288];
@@ -693,8 +739,7 @@ __global__ __launch_bounds__(L6E_THREADS) void l6_emit_kernel(const uint8_t* __r
     __shared__ uint32_t wsum[L6E_THREADS / 64 + 1];
     __shared__ uint64_t s_off;
     __shared__ uint64_t s_sym;
-    if (res->total_bytes == 0) return;
-    const uint32_t c = blockIdx.x / maxblk, b = blockIdx.x % maxblk;
+    if (res[desc[c].buf & ~CHUNK_LAST].total_bytes == 0) return;
     const uint32_t ntok = ntok_in[c];
     const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
     if (b >= nblk) return;
@@ -703,9 +748,9 @@ __global__ __launch_bounds__(L6E_THREADS) void l6_emit_kernel(const uint8_t* __r
     const BlockRec& R = RR[b];
     const uint64_t cp = (chunk + 7u) & ~7u;
     const uint32_t* TK = toks + (uint64_t)c * cp;
-    const uint8_t* s = src + (uint64_t)c * chunk;
+    const uint8_t* s = src + desc[c].in_off;
     uint32_t* stg = stage + (uint64_t)c * stage_words;
-    const bool last = (c == nchunks - 1) && (b == nblk - 1);
+    const bool last = (desc[c].buf & CHUNK_LAST) != 0 && (b == nblk - 1);
     if (tid == 0) {
         const uint64_t r = info[c].bit_off & 7;
         uint64_t off = r;
@@ -848,58 +893,96 @@ __global__ __launch_bounds__(L6E_THREADS) void l6_emit_kernel(const uint8_t* __r
     if (tid == 0) put_bits_at(stg, sym0 + wsum[L6E_THREADS / 64], llc[256], lls[256]);  // EOB
 }
 
+__global__ __launch_bounds__(L6E_THREADS) void l6_emit_kernel(const uint8_t* __restrict__ src,
+                                                              const ChunkDesc* __restrict__ desc,
+                                                              const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                              const uint32_t* __restrict__ toks,
+                                                              const uint32_t* __restrict__ ntok_in,
+                                                              const BlockRec* __restrict__ recs, uint32_t maxblk,
+                                                              const ChunkInfo* __restrict__ info,
+                                                              const Result* __restrict__ res,
+                                                              uint32_t* __restrict__ stage, uint64_t stage_words) {
+    const uint32_t items = hdr->nchunks * maxblk;
+    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
+        l6_emit_body(w / maxblk, w % maxblk, src, desc, chunk, toks, ntok_in, recs, maxblk, info, res, stage,
+                     stage_words);
+        __syncthreads();
+    }
+}
+
+inline uint32_t l6_per_cu(const Plan& p, size_t smem, int threads) {
+    const uint32_t by_lds = smem ? (uint32_t)(p.lds_max / smem) : 32u;
+    const uint32_t by_waves = (uint32_t)(32 / (threads / 64));
+    const uint32_t k = by_lds < by_waves ? by_lds : by_waves;
+    return k ? k : 1u;
+}
+
+template <class K>
+hipError_t l6_allow_lds(K* kernel, size_t smem) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)smem);
+}
+
 template <bool EXCH>
 hipError_t launch_pred(const Plan& p) {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l6_pred_kernel<EXCH>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)L6P_LDS);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
-    hipLaunchKernelGGL((l6_pred_kernel<EXCH>), dim3(p.nchunks), dim3(L6P_THREADS), L6P_LDS, p.stream, p.d_in, p.n,
-                       p.chunk, p.d_tok, p.d_pd);
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6P_LDS, L6P_THREADS));
+    hipLaunchKernelGGL((l6_pred_kernel<EXCH>), dim3(g), dim3(L6P_THREADS), L6P_LDS, p.stream, p.d_in, p.d_desc,
+                       p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_pd);
     return hipSuccess;
 }
 
 }  // namespace
 
+// LDS limits of the L6-class kernels; set once per process before any launch.
+hipError_t set_kernel_attributes_l6(size_t lds_max) {
+    (void)lds_max;
+    hipError_t e = l6_allow_lds(&l6_pred_kernel<true>, L6P_LDS);
+    if (e == hipSuccess) e = l6_allow_lds(&l6_pred_kernel<false>, L6P_LDS);
+    if (e == hipSuccess) e = l6_allow_lds(&l6_match_kernel, L6M_LDS);
+    if (e == hipSuccess) e = l6_allow_lds(&l6_parse_kernel, L6R_LDS);
+    return e;
+}
+
+L6Params l6_level_params(int level) {
+    // {max_chain, nice_len, lazy_depth} per level; 0..3 are not L6-class
+    static const L6Params t[13] = {
+        {0, 0, 0},    {0, 0, 0},     {0, 0, 0},     {0, 0, 0},     {16, 128, 1},   {32, 128, 1},   {32, 258, 1},
+        {64, 258, 2}, {128, 258, 2}, {256, 258, 3}, {512, 258, 3}, {1024, 258, 3}, {2048, 258, 3},
+    };
+    return t[level < 0 ? 0 : (level > 12 ? 12 : level)];
+}
 uint32_t l6_max_blocks(uint32_t chunk) { return (chunk + BLK_SYMS - 1) / BLK_SYMS; }
 uint32_t l6_max_chunk() { return L6_MAX_CHUNK; }
 size_t l6_block_record_bytes() { return sizeof(BlockRec); }
 
 // LZ77 and block analysis of the L6-class path (before the offset scan).
 hipError_t launch_l6_front(const Plan& p) {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l6_match_kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)L6M_LDS);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&l6_parse_kernel),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)L6R_LDS);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
     const uint32_t maxblk = l6_max_blocks(p.chunk);
     hipError_t e = p.exch_ok ? launch_pred<true>(p) : launch_pred<false>(p);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(l6_match_kernel, dim3(p.nchunks), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.n, p.chunk,
-                       p.d_pd, p.d_tok, p.level);
-    hipLaunchKernelGGL(l6_parse_kernel, dim3(p.nchunks), dim3(L6R_THREADS), L6R_LDS, p.stream, p.d_in, p.n, p.chunk,
-                       p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.level);
-    hipLaunchKernelGGL(l6_block_kernel, dim3(p.nchunks * maxblk), dim3(L6B_THREADS), 0, p.stream, p.d_aux, p.d_ntok,
-                       p.d_blkpos, p.chunk, p.n, maxblk, reinterpret_cast<BlockRec*>(p.d_blocks));
-    hipLaunchKernelGGL(l6_size_kernel, dim3((p.nchunks + 255) / 256), dim3(256), 0, p.stream, p.d_ntok,
-                       reinterpret_cast<const BlockRec*>(p.d_blocks), p.nchunks, p.chunk, p.n, maxblk, p.d_info);
+    const unsigned gm = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6M_LDS, L6M_THREADS));
+    hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.d_desc, p.d_order,
+                       p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6);
+    const unsigned gr = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6R_LDS, L6R_THREADS));
+    hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), L6R_LDS, p.stream, p.d_in, p.d_desc, p.d_order,
+                       p.d_hdr, p.chunk, p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.l6);
+    const unsigned gb = grid_for(p.persistent_grid, p.nchunks * maxblk, p.cus * 4u);
+    hipLaunchKernelGGL(l6_block_kernel, dim3(gb), dim3(L6B_THREADS), 0, p.stream, p.d_aux, p.d_ntok, p.d_blkpos,
+                       p.d_hdr, p.chunk, maxblk, reinterpret_cast<BlockRec*>(p.d_blocks));
+    const unsigned gs = grid_for(p.persistent_grid, (p.nchunks + 255) / 256, p.cus);
+    hipLaunchKernelGGL(l6_size_kernel, dim3(gs), dim3(256), 0, p.stream, p.d_ntok,
+                       reinterpret_cast<const BlockRec*>(p.d_blocks), p.d_desc, p.d_hdr, maxblk, p.d_info);
     return hipGetLastError();
 }
 
 // Emission of the L6-class path (after the offset scan).
 hipError_t launch_l6_back(const Plan& p) {
     const uint32_t maxblk = l6_max_blocks(p.chunk);
-    hipLaunchKernelGGL(l6_zero_kernel, dim3(p.nchunks), dim3(256), 0, p.stream, p.d_info, p.d_stage, p.stage_words);
-    hipLaunchKernelGGL(l6_emit_kernel, dim3(p.nchunks * maxblk), dim3(L6E_THREADS), 0, p.stream, p.d_in, p.n, p.chunk,
-                       p.nchunks, p.d_aux, p.d_ntok, reinterpret_cast<const BlockRec*>(p.d_blocks), maxblk, p.d_info,
+    const unsigned gz = grid_for(p.persistent_grid, p.nchunks, p.cus * 8u);
+    hipLaunchKernelGGL(l6_zero_kernel, dim3(gz), dim3(256), 0, p.stream, p.d_info, p.d_hdr, p.d_stage, p.stage_words);
+    const unsigned ge = grid_for(p.persistent_grid, p.nchunks * maxblk, p.cus * 8u);
+    hipLaunchKernelGGL(l6_emit_kernel, dim3(ge), dim3(L6E_THREADS), 0, p.stream, p.d_in, p.d_desc, p.d_hdr, p.chunk,
+                       p.d_aux, p.d_ntok, reinterpret_cast<const BlockRec*>(p.d_blocks), maxblk, p.d_info,
                        p.d_result, p.d_stage, p.stage_words);
     return hipGetLastError();
 }

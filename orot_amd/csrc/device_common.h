@@ -226,5 +226,28 @@ __device__ __forceinline__ uint64_t stored_bits(uint64_t off, uint32_t L) {
     return 3 + pad0 + 32 * pieces + 8 * (pieces - 1) + 8ull * L;
 }
 
+constexpr uint32_t NO_ITEM = 0xFFFFFFFFu;
+
+// Next item of a persistent workgroup's work queue: one atomic per item,
+// broadcast through the LDS word *s_item.  Every thread of the workgroup must
+// call it.  The word is live only between the two barriers inside, so it may
+// be any word of the workgroup's LDS that holds nothing across the call (the
+// persistent kernels pass the first word of their dynamic LDS, which each
+// chunk's work writes before it reads).
+__device__ __forceinline__ uint32_t next_item(uint32_t* queue, uint32_t count, uint32_t* s_item) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t k = *s_item;
+    __syncthreads();
+    return k < count ? k : NO_ITEM;
+}
+
+// Grid size for a launch over `count` items with at most `cap` workgroups;
+// persistent launches (graph capture) always take `cap`.
+inline unsigned grid_for(bool persistent, uint32_t count, uint32_t cap) {
+    if (persistent) return cap;
+    return count == 0 ? 1u : (count < cap ? count : cap);
+}
+
 }  // namespace
 }  // namespace orot_gpu

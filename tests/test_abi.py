@@ -13,8 +13,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     names = set()
-    for hdr in ("deflate.h", "deflate_gpu.h"):
-        with open(os.path.join(ROOT, "include", "orot", hdr)) as f:
+    inc = os.path.join(ROOT, "include", "orot")
+    for hdr in sorted(os.listdir(inc)):
+        if not hdr.endswith(".h"):
+            continue
+        with open(os.path.join(inc, hdr)) as f:
             text = f.read()
         c_part = text.split("#ifdef __cplusplus\n#include")[0]
         for m in re.finditer(r"^\s*[A-Za-z_][\w \*]*?\b(\w+)\s*\(", c_part, re.M):
@@ -54,7 +57,9 @@ def test_no_gpu_means_no_context():
         "import orot_amd,sys\n"
         "L=orot_amd.lib()\n"
         "ctx=L.deflate_parallel_new(1,0,0,0)\n"
-        "sys.exit(0 if (ctx is None) == (L.orot_gpu_synchronize()!=0) else 1)\n"
+        "bctx=L.deflate_batch_new(1,1,0,0)\n"
+        "nogpu=L.orot_gpu_synchronize()!=0\n"
+        "sys.exit(0 if (ctx is None) == nogpu and (bctx is None) == nogpu else 1)\n"
     )
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env={**os.environ, "PYTHONPATH": ROOT})
     assert r.returncode == 0
