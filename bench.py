@@ -307,71 +307,136 @@ def bench_l1(args) -> dict:
     return rec
 
 
-def bench_l6(args) -> dict:
-    """Level 6, gzip: 256 x 8 MiB buffers (text / records), one
-    deflate_parallel_compress_device call per buffer; a step compresses all
-    256.  Every output is inflated by zlib once, and the compressed size is
-    compared with orot's at level 6 on the buffers its CPU leg covers."""
-    bufs = make_buffers(l6_specs())
-    sizes = [b.size for b in bufs]
-    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    n = int(offs[-1])
-    slot = orot_amd.compress_bound(L6_SIZE, orot_amd.Format.GZIP) + 4096
-    d_in = orot_amd.DeviceBuffer(n)
-    for b, o in zip(bufs, offs[:-1]):
-        d_in.upload(b, int(o))
-    d_out = orot_amd.DeviceBuffer(slot * len(bufs))
-    pc = orot_amd.ParallelCompressor(6, orot_amd.Format.GZIP, 0, 0)
-    pc.set_timing(True)
+# Batch workload: 4096 buffers, sizes log-uniform in [4 KiB, 4 MiB], content
+# type uniform over the four generators, level 1, zlib.
+BATCH_COUNT = 4096
+BATCH_LO = 4 << 10
+BATCH_HI = 4 << 20
+BATCH_MASTER = 0xB47C0003
 
-    def run_all(record=None):
+
+def batch_specs():
+    sizes = corpus.log_uniform_sizes(BATCH_COUNT, BATCH_LO, BATCH_HI, BATCH_MASTER)
+    kinds = corpus.stream(BATCH_MASTER ^ 0x4B1D, BATCH_COUNT) % np.uint64(4)
+    return [(corpus.GENERATORS[int(k)], int(n), corpus.buffer_seed(BATCH_MASTER, i))
+            for i, (k, n) in enumerate(zip(kinds, sizes))]
+
+
+def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds) -> dict:
+    """Many buffers: the plain path (one deflate_parallel_compress_device call
+    per buffer, orot's default chunks) and the batch path (one submission of
+    every buffer: graph replay, autotuned configuration).  A step is one
+    submission of the whole batch."""
+    bufs = make_buffers(specs)
+    sizes = np.array([b.size for b in bufs], dtype=np.int64)
+    n = int(sizes.sum())
+    in_off = np.zeros(len(bufs), dtype=np.int64)
+    in_off[1:] = np.cumsum((sizes[:-1] + 255) // 256 * 256)
+    caps = np.array([orot_amd.compress_bound(int(k), fmt) + 64 * (int(k) // 65535 + 4) + 4096 for k in sizes],
+                    dtype=np.int64)
+    out_off = np.zeros(len(bufs), dtype=np.int64)
+    out_off[1:] = np.cumsum((caps[:-1] + 255) // 256 * 256)
+    d_in = orot_amd.DeviceBuffer(int(in_off[-1] + sizes[-1]) + 256)
+    for b, o in zip(bufs, in_off):
+        if b.size:
+            d_in.upload(b, int(o))
+    d_out = orot_amd.DeviceBuffer(int(out_off[-1] + caps[-1]))
+    items = orot_amd.BatchCompressor.items(in_off, sizes, out_off, caps)
+
+    def digests(out_sizes):
+        return [hashlib.sha256(d_out.download(int(k), int(o)).tobytes()).digest() for k, o in zip(out_sizes, out_off)]
+
+    # plain path: one call per buffer, orot's default configuration
+    pc = orot_amd.ParallelCompressor(level, fmt, 0, 0)
+
+    def plain_pass():
         out = []
-        for i, b in enumerate(bufs):
-            s0 = time.perf_counter()
-            k = pc.compress_device(d_in.ptr + int(offs[i]), b.size, d_out.ptr + i * slot, slot)
-            if record is not None:
-                record.append((time.perf_counter() - s0, pc.stage_times()))
+        for i in range(len(bufs)):
+            k = pc.compress_device(d_in.ptr + int(in_off[i]), int(sizes[i]), d_out.ptr + int(out_off[i]),
+                                   int(caps[i]))
             if k == 0:
                 raise SystemExit(f"compress failed on buffer {i}: {orot_amd.last_error()}")
             out.append(k)
         return out
 
-    clens = run_all()
+    plain_sizes = plain_pass()
     ok = True
+    wb = {0: -15, 1: 15, 2: 31}[fmt]
     for i, b in enumerate(bufs):
-        comp = d_out.download(clens[i], i * slot).tobytes()
-        ok = ok and zlib.decompress(comp, 31) == b.tobytes()
+        ok = ok and zlib.decompress(d_out.download(plain_sizes[i], int(out_off[i])).tobytes(), wb) == b.tobytes()
+    plain_dig = digests(plain_sizes)
+    plain_runs = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(plain_runs):
+        plain_pass()
+    plain_t = (time.perf_counter() - t0) / plain_runs
+    pc.close()
+
+    # batch path, default configuration: every buffer's bytes equal the plain path's
+    with orot_amd.BatchCompressor(level, fmt, 0, 0) as bd:
+        mix_equal = digests(bd.compress_device(d_in.ptr, d_out.ptr, items)) == plain_dig
+
+    # batch path, autotuned, graph-submitted
+    bc = orot_amd.BatchCompressor(level, fmt, 0, orot_amd.BATCH_GRAPH | orot_amd.BATCH_AUTOTUNE)
+    t_tune = time.perf_counter()
+    out_sizes = bc.compress_device(d_in.ptr, d_out.ptr, items)  # tunes on a prefix, then runs
+    t_tune = time.perf_counter() - t_tune
+    tuned = bc.config()
+    for i, b in enumerate(bufs):
+        if out_sizes[i] == 0:
+            raise SystemExit(f"batch buffer {i} did not fit")
+        ok = ok and zlib.decompress(d_out.download(int(out_sizes[i]), int(out_off[i])).tobytes(), wb) == b.tobytes()
     for _ in range(args.warmup):
-        run_all()
-    orot_amd.lib().orot_gpu_synchronize()
-    per = []
+        bc.compress_device(d_in.ptr, d_out.ptr, items)
+    lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        run_all(per)
+        s0 = time.perf_counter()
+        bc.compress_device(d_in.ptr, d_out.ptr, items)
+        lat.append((time.perf_counter() - s0) * 1e3)
     total = time.perf_counter() - t0
-    lat_ms = [x[0] * 1e3 for x in per]
-    front = statistics.mean(x[1][1] for x in per) * 1e-3
-    clen = sum(clens)
-    b_alg = (n + clen) / len(bufs)
+    bc.close()
+    clen = int(np.sum(out_sizes))
+    plain_clen = int(sum(plain_sizes))
+
+    # stage times of the tuned configuration (direct launches, HIP events)
+    with orot_amd.BatchCompressor(level, fmt, 0, 0) as bt:
+        bt.set_config(tuned["block_size"], tuned["hash_bits"], tuned["max_chain"], tuned["nice_len"],
+                      tuned["lazy_depth"])
+        bt.set_timing(True)
+        bt.compress_device(d_in.ptr, d_out.ptr, items)
+        stages = bt.stage_times()
+    front = stages[1] * 1e-3
+    b_alg = n + clen
+    kernel = "l1_barrier_kernel" if level <= 3 else "l6 front (pred + match + parse + block analysis)"
+    if level <= 3 and tuned["block_size"] > 131072:
+        kernel = "l1_lz77_kernel"
+    if level <= 3 and tuned["hash_bits"] == 14:
+        kernel = "l1_chunk_kernel"
+
     rec = base_record(args, n * args.steps / total / 1e9, total / args.steps * 1e3, {
-        "workload": "L6-class lazy parse, dynamic Huffman blocks (15-bit codes, <= 16384 symbols), gzip, "
-                    "256 x 8 MiB buffers (128 Zipf text + 128 64-byte records), one call per buffer",
-        "level": 6,
-        "format": "gzip",
-        "chunk": 524288,
+        "workload": workload,
+        "level": level,
+        "format": ["raw", "zlib", "gzip"][fmt],
         "buffers": len(bufs),
         "input_bytes": n,
+        "tuned": tuned,
     })
     rec.update({
-        "p50_ms": round(pct(lat_ms, 0.5), 3),
-        "p99_ms": round(pct(lat_ms, 0.99), 3),
+        "p50_ms": round(pct(lat, 0.5), 3),
+        "p99_ms": round(pct(lat, 0.99), 3),
         "compressed_bytes": clen,
         "ratio": round(clen / n, 5),
         "inflate_ok": ok,
-        "stage_ms_mean": [round(statistics.mean(x[1][k] for x in per), 4) for k in range(4)],
+        "plain_path_gbps": round(n / plain_t / 1e9, 3),
+        "speedup_vs_plain": round((n * args.steps / total) / (n / plain_t), 2),
+        "alone_equals_in_mix": mix_equal,
+        "size_vs_default_config": round(clen / plain_clen, 5),
+        "autotune_s": round(t_tune, 3),
+        "stage_ms_tuned": [round(x, 4) for x in stages],
         "roofline": {
             "bound": "hbm",
-            "kernel": "l6 front (pred + match + parse + block analysis)",
+            "kernel": kernel,
             "achieved": round(b_alg / front / 1e9, 2),
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -380,23 +445,40 @@ def bench_l6(args) -> dict:
         },
     })
     if not args.no_cpu:
-        base, orot_sizes = cpu_multi(bufs, 6, 2)
+        base, orot_sizes = cpu_multi(bufs, level, fmt)
         rec["cpu_baseline"] = base
         m = len(orot_sizes)
         by_kind = {}
         for i in range(m):
-            k = "text" if i % 2 == 0 else "records"
+            k = cpu_kinds(i)
             g, o = by_kind.get(k, (0, 0))
-            by_kind[k] = (g + clens[i], o + orot_sizes[i])
+            by_kind[k] = (g + int(out_sizes[i]), o + orot_sizes[i])
         rec["ratio_vs_orot"] = {
             "buffers": m,
-            "gpu_over_orot": round(sum(clens[:m]) / max(1, sum(orot_sizes)), 5),
+            "gpu_over_orot": round(int(np.sum(out_sizes[:m])) / max(1, sum(orot_sizes)), 5),
             "per_kind": {k: round(g / max(1, o), 5) for k, (g, o) in by_kind.items()},
         }
-    pc.close()
     d_in.free()
     d_out.free()
     return rec
+
+
+def bench_l6(args) -> dict:
+    """Level 6, gzip: 256 x 8 MiB buffers (text / records)."""
+    specs = l6_specs()
+    return bench_multi(args, specs, 6, 2,
+                       "L6-class lazy parse, dynamic Huffman blocks (15-bit codes, <= 16384 symbols), gzip, "
+                       "256 x 8 MiB buffers (128 Zipf text + 128 64-byte records), batched",
+                       lambda i: specs[i][0])
+
+
+def bench_batch(args) -> dict:
+    """Level 1, zlib: 4096 buffers of 4 KiB - 4 MiB, mixed content."""
+    specs = batch_specs()
+    return bench_multi(args, specs, 1, 1,
+                       "batched L1-class greedy, zlib, 4096 buffers log-uniform in [4 KiB, 4 MiB], content "
+                       "uniform over zeros / Zipf text / random / records, one graph-submitted batch, autotuned",
+                       lambda i: specs[i][0])
 
 
 def main() -> int:
@@ -404,9 +486,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("l1", "l6"), default="l1",
+    ap.add_argument("--workload", choices=("l1", "l6", "batch"), default="l1",
                     help="l1: 256 MiB four-segment buffer at level 1 (default); "
-                         "l6: 256 x 8 MiB text/records buffers at level 6, gzip")
+                         "l6: 256 x 8 MiB text/records buffers at level 6, gzip, batched; "
+                         "batch: 4096 mixed buffers of 4 KiB - 4 MiB at level 1, zlib, batched")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale HBM counters")
@@ -417,7 +500,7 @@ def main() -> int:
         print("bench.py measures one GPU only: every box here holds a single MI355X; "
               "batches shard across GPUs with no exchange (see the design notes).", file=sys.stderr)
         return 2
-    rec = bench_l6(args) if args.workload == "l6" else bench_l1(args)
+    rec = {"l1": bench_l1, "l6": bench_l6, "batch": bench_batch}[args.workload](args)
     try:
         rec["device"] = orot_amd.device_info()[0]
     except Exception:

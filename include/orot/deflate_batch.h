@@ -63,7 +63,8 @@ int deflate_batch_set_config(deflate_batch_ctx* ctx, const deflate_batch_config*
 
 /* Times each candidate configuration on (a prefix of up to 256 MiB of) the
  * given batch and keeps the fastest whose compressed size stays within 1.5%
- * of the level's default configuration.  Outputs go to d_out as in
+ * (levels 1-3, whose default is orot's own output) or 0.5% (levels 4-12) of
+ * the level's default configuration.  Outputs go to d_out as in
  * deflate_batch_compress_device.  Returns 0 on success. */
 int deflate_batch_autotune(deflate_batch_ctx* ctx, const void* d_in, void* d_out, const deflate_batch_item* items,
                            size_t count);

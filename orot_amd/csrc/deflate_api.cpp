@@ -178,13 +178,17 @@ int autotune(Engine& eng, const uint8_t* d_in, uint8_t* d_out, const deflate_bat
     double base_t = 0;
     uint64_t base_sz = 0;
     if (!measure(base, base_t, base_sz)) return -1;
+    // Levels 1-3 default to orot's own bytes, so the tuned size may grow by
+    // 1.5%; the L6 class's default is already above orot's size (by up to
+    // 2.5% on the measured corpora), so tuning may only add 0.5%.
+    const double tol = lv <= 3 ? 1.015 : 1.005;
     EngineConfig best = base;
     double best_t = base_t;
     for (const EngineConfig& c : tune_candidates(base)) {
         double t = 0;
         uint64_t sz = 0;
         if (!measure(c, t, sz)) continue;
-        if ((double)sz <= (double)base_sz * 1.015 && t < best_t) {
+        if ((double)sz <= (double)base_sz * tol && t < best_t) {
             best = c;
             best_t = t;
         }
