@@ -46,6 +46,15 @@ EXPORTED_SYMBOLS = (
     "deflate_batch_autotune",
     "deflate_batch_set_timing",
     "deflate_batch_stage_times",
+    "deflate_server_new",
+    "deflate_server_free",
+    "deflate_server_submit_device",
+    "deflate_server_submit",
+    "deflate_server_poll",
+    "deflate_server_output",
+    "deflate_server_read",
+    "deflate_server_release",
+    "deflate_server_stats",
 )
 
 BATCH_GRAPH = 1
@@ -57,6 +66,21 @@ class BatchItem(ctypes.Structure):
 
     _fields_ = [("in_offset", ctypes.c_uint64), ("in_size", ctypes.c_uint64),
                 ("out_offset", ctypes.c_uint64), ("out_capacity", ctypes.c_uint64)]
+
+
+class ServerConfig(ctypes.Structure):
+    """deflate_server_config (include/orot/deflate_server.h)."""
+
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_batch_bytes", ctypes.c_size_t),
+                ("max_batch_requests", ctypes.c_int), ("lanes_per_level", ctypes.c_int)]
+
+
+class Completion(ctypes.Structure):
+    """deflate_server_completion (include/orot/deflate_server.h)."""
+
+    _fields_ = [("id", ctypes.c_uint64), ("in_size", ctypes.c_uint64), ("out_size", ctypes.c_uint64),
+                ("status", ctypes.c_int32), ("level", ctypes.c_int32), ("submit_ns", ctypes.c_uint64),
+                ("launch_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -170,6 +194,25 @@ def lib() -> ctypes.CDLL:
     L.deflate_batch_set_timing.argtypes = [vp, ctypes.c_int]
     L.deflate_batch_stage_times.restype = ctypes.c_int
     L.deflate_batch_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+    u64 = ctypes.c_uint64
+    L.deflate_server_new.restype = vp
+    L.deflate_server_new.argtypes = [ctypes.POINTER(ServerConfig)]
+    L.deflate_server_free.restype = None
+    L.deflate_server_free.argtypes = [vp]
+    L.deflate_server_submit_device.restype = u64
+    L.deflate_server_submit_device.argtypes = [vp, vp, sz, ctypes.c_int, ctypes.c_int]
+    L.deflate_server_submit.restype = u64
+    L.deflate_server_submit.argtypes = [vp, vp, sz, ctypes.c_int, ctypes.c_int]
+    L.deflate_server_poll.restype = ctypes.c_int
+    L.deflate_server_poll.argtypes = [vp, ctypes.POINTER(Completion), ctypes.c_int, ctypes.c_int]
+    L.deflate_server_output.restype = vp
+    L.deflate_server_output.argtypes = [vp, u64]
+    L.deflate_server_read.restype = sz
+    L.deflate_server_read.argtypes = [vp, u64, vp, sz]
+    L.deflate_server_release.restype = ctypes.c_int
+    L.deflate_server_release.argtypes = [vp, u64]
+    L.deflate_server_stats.restype = ctypes.c_int
+    L.deflate_server_stats.argtypes = [vp, ctypes.POINTER(u64), ctypes.c_int]
     _lib = L
     return L
 
@@ -388,6 +431,74 @@ class BatchCompressor:
         if self._ctx:
             lib().deflate_batch_free(self._ctx)
             self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching compression server (include/orot/deflate_server.h)."""
+
+    def __init__(self, hbm_budget: int = 0, max_batch_bytes: int = 0, max_batch_requests: int = 0,
+                 lanes_per_level: int = 0):
+        cfg = ServerConfig(hbm_budget, max_batch_bytes, max_batch_requests, lanes_per_level)
+        self._srv = lib().deflate_server_new(ctypes.byref(cfg))
+        if not self._srv:
+            raise Error(Result.MEM_ERROR, f"Server: {last_error()}")
+        self._comp = (Completion * 1024)()
+
+    def submit_device(self, d_in: int, size: int, level: int, fmt: int = Format.GZIP) -> int:
+        rid = lib().deflate_server_submit_device(self._srv, d_in, int(size), int(level), int(fmt))
+        if rid == 0:
+            raise Error(Result.BUF_ERROR, f"deflate_server_submit_device: {last_error()}")
+        return rid
+
+    def submit(self, data, level: int, fmt: int = Format.GZIP) -> int:
+        """Host input; the array must stay alive until the request completes."""
+        a = _as_u8(data)
+        rid = lib().deflate_server_submit(self._srv, a.ctypes.data, a.size, int(level), int(fmt))
+        if rid == 0:
+            raise Error(Result.BUF_ERROR, f"deflate_server_submit: {last_error()}")
+        return rid
+
+    def poll(self, timeout_ms: int = 0) -> list:
+        k = lib().deflate_server_poll(self._srv, self._comp, len(self._comp), int(timeout_ms))
+        return [(c.id, c.in_size, c.out_size, c.status, c.level, c.submit_ns, c.launch_ns, c.done_ns)
+                for c in self._comp[:k]]
+
+    def output_ptr(self, rid: int) -> int:
+        return lib().deflate_server_output(self._srv, rid) or 0
+
+    def read(self, rid: int, size: int) -> bytes:
+        out = np.empty(max(int(size), 1), dtype=np.uint8)
+        n = lib().deflate_server_read(self._srv, rid, out.ctypes.data, out.size)
+        if n != size:
+            raise Error(Result.BUF_ERROR, f"deflate_server_read: {last_error()}")
+        return out[:n].tobytes()
+
+    def release(self, rid: int) -> None:
+        if lib().deflate_server_release(self._srv, rid) != 0:
+            raise Error(Result.PARAM_ERROR, "deflate_server_release: unknown or unfinished request")
+
+    def stats(self) -> dict:
+        v = (ctypes.c_uint64 * 7)()
+        lib().deflate_server_stats(self._srv, v, 7)
+        keys = ("submitted", "completed", "queued", "in_flight", "pool_bytes", "batches", "max_queue")
+        return dict(zip(keys, (int(x) for x in v)))
+
+    def close(self) -> None:
+        if self._srv:
+            lib().deflate_server_free(self._srv)
+            self._srv = None
 
     def __enter__(self):
         return self

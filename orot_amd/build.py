@@ -26,7 +26,7 @@ COMMON = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["deflate_kernels.hip", "deflate_l6.hip", "engine.cpp", "deflate_api.cpp"]
+SOURCES = ["deflate_kernels.hip", "deflate_l6.hip", "engine.cpp", "deflate_api.cpp", "server.cpp"]
 
 
 def _run(cmd, verbose):

@@ -58,8 +58,10 @@ def test_no_gpu_means_no_context():
         "L=orot_amd.lib()\n"
         "ctx=L.deflate_parallel_new(1,0,0,0)\n"
         "bctx=L.deflate_batch_new(1,1,0,0)\n"
+        "cfg=orot_amd.ServerConfig(1<<20,0,0,0)\n"
+        "srv=L.deflate_server_new(cfg)\n"
         "nogpu=L.orot_gpu_synchronize()!=0\n"
-        "sys.exit(0 if (ctx is None) == nogpu and (bctx is None) == nogpu else 1)\n"
+        "sys.exit(0 if (ctx is None) == nogpu and (bctx is None) == nogpu and (srv is None) == nogpu else 1)\n"
     )
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env={**os.environ, "PYTHONPATH": ROOT})
     assert r.returncode == 0
