@@ -481,15 +481,151 @@ def bench_batch(args) -> dict:
                        lambda i: specs[i][0])
 
 
+# Serving workload: Poisson arrivals, sizes log-uniform in [1 KiB, 8 MiB],
+# content uniform over the four generators, level uniform over {1, 6}, gzip,
+# 32 GiB HBM budget.
+SERVE_RATE = 2000.0
+SERVE_LO = 1 << 10
+SERVE_HI = 8 << 20
+SERVE_MASTER = 0x5E7E0004
+
+
+def serve_requests(duration: float):
+    count = int(SERVE_RATE * duration)
+    r = corpus.stream(SERVE_MASTER, 4 * count)
+    u = (r >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    gaps = -np.log(1.0 - u[:count]) / SERVE_RATE
+    arrivals = np.cumsum(gaps)
+    sizes = corpus.log_uniform_sizes(count, SERVE_LO, SERVE_HI, SERVE_MASTER ^ 0x51)
+    kinds = (r[count:2 * count] % np.uint64(4)).astype(np.int64)
+    levels = np.where((r[2 * count:3 * count] & np.uint64(1)) == 0, 1, 6)
+    offs = (u[3 * count:] * (SEGMENT - sizes)).astype(np.int64)
+    return arrivals, sizes, kinds, levels, offs
+
+
+def bench_serve(args) -> dict:
+    """Open-loop serving: a producer thread submits each request at its
+    arrival time (inputs are slices of the device-resident four-segment
+    corpus); the main thread polls completions, checks a sample of outputs
+    and releases every output."""
+    import threading
+
+    data = workload()
+    d_in = orot_amd.DeviceBuffer.from_host(data)
+    arrivals, sizes, kinds, levels, offs = serve_requests(args.duration)
+    count = len(sizes)
+    srv = orot_amd.Server(hbm_budget=32 << 30)
+    ids = np.zeros(count, dtype=np.uint64)
+    index = {}
+    t_start = time.perf_counter()
+
+    def producer():
+        for i in range(count):
+            wait = arrivals[i] - (time.perf_counter() - t_start)
+            if wait > 0.0005:
+                time.sleep(wait - 0.0003)
+            while time.perf_counter() - t_start < arrivals[i]:
+                pass
+            src = int(kinds[i]) * SEGMENT + int(offs[i])
+            rid = srv.submit_device(d_in.ptr + src, int(sizes[i]), int(levels[i]), orot_amd.Format.GZIP)
+            ids[i] = rid
+            index[rid] = i
+
+    th = threading.Thread(target=producer)
+    th.start()
+    lat, queue_ms, done_bytes = [], [], 0
+    lat_by_level = {1: [], 6: []}
+    checked, ok = 0, True
+    first_ns, last_ns = None, 0
+    got = 0
+    deadline = time.perf_counter() + args.duration + 120
+    while got < count and time.perf_counter() < deadline:
+        for c in srv.poll(100):
+            rid, n_in, n_out, status, lv, t_sub, t_launch, t_done = c
+            got += 1
+            if status != 0:
+                ok = False
+                continue
+            lat.append((t_done - t_sub) / 1e6)
+            lat_by_level.setdefault(lv, []).append(lat[-1])
+            queue_ms.append((t_launch - t_sub) / 1e6)
+            done_bytes += n_in
+            first_ns = t_sub if first_ns is None else min(first_ns, t_sub)
+            last_ns = max(last_ns, t_done)
+            while rid not in index:  # the producer records the id right after submit returns
+                time.sleep(0)
+            i = index[rid]
+            if i % 64 == 0:
+                comp = srv.read(rid, n_out)
+                src = int(kinds[i]) * SEGMENT + int(offs[i])
+                ok = ok and zlib.decompress(comp, 31) == data[src:src + int(sizes[i])].tobytes()
+                checked += 1
+            srv.release(rid)
+    th.join()
+    stats = srv.stats()
+    srv.close()
+    span = (last_ns - (first_ns or 0)) / 1e9
+    offered = float(np.sum(sizes)) / float(arrivals[-1])
+    rec = base_record(args, done_bytes / span / 1e9 if span > 0 else 0.0, span / max(1, got) * 1e3, {
+        "workload": f"serving: Poisson arrivals at {SERVE_RATE:.0f} req/s for {args.duration:g} s, sizes "
+                    "log-uniform in [1 KiB, 8 MiB], content uniform over zeros / Zipf text / random / records, "
+                    "levels uniform over {1, 6}, gzip, 32 GiB HBM budget, continuous batching",
+        "requests": count,
+        "format": "gzip",
+    })
+    rec.pop("steps")
+    rec.pop("warmup")
+    rec["ms_per_step"] = None
+    lat_s = sorted(lat)
+    rec.update({
+        "completed": got,
+        "completed_ok": len(lat),
+        "offered_gbps": round(offered / 1e9, 3),
+        "p50_ms": round(pct(lat_s, 0.5), 3),
+        "p99_ms": round(pct(lat_s, 0.99), 3),
+        "p999_ms": round(pct(lat_s, 0.999), 3),
+        "latency_by_level_ms": {str(k): {"p50": round(pct(sorted(v), 0.5), 3), "p99": round(pct(sorted(v), 0.99), 3)}
+                                for k, v in lat_by_level.items() if v},
+        "queue_p50_ms": round(pct(sorted(queue_ms), 0.5), 3),
+        "queue_p99_ms": round(pct(sorted(queue_ms), 0.99), 3),
+        "inflate_ok": ok,
+        "outputs_checked": checked,
+        "server": stats,
+    })
+    if not args.no_cpu:
+        # orot on the host cores over a sample of the same request mix
+        threads = cpu_threads()
+        cpus = {1: OrotCPU(1, 2, 0, threads), 6: OrotCPU(6, 2, 0, threads)}
+        out = np.empty(int(SERVE_HI) * 9 // 8 + 4096, dtype=np.uint8)
+        nb, t0, k = 0, time.perf_counter(), 0
+        while k < count and time.perf_counter() - t0 < 20.0:
+            src = int(kinds[k]) * SEGMENT + int(offs[k])
+            cpus[int(levels[k])].compress(data[src:src + int(sizes[k])], out)
+            nb += int(sizes[k])
+            k += 1
+        dt = time.perf_counter() - t0
+        for c in cpus.values():
+            c.close()
+        rec["cpu_baseline"] = {
+            "value": round(nb / dt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": cpus[1].kind,
+            "sample": f"orot parallel L1/L6 gzip, one call per request, the first {k} requests of the same "
+                      f"stream back to back ({nb >> 20} MiB)",
+        }
+    d_in.free()
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("l1", "l6", "batch"), default="l1",
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
+    ap.add_argument("--workload", choices=("l1", "l6", "batch", "serve"), default="l1",
                     help="l1: 256 MiB four-segment buffer at level 1 (default); "
                          "l6: 256 x 8 MiB text/records buffers at level 6, gzip, batched; "
-                         "batch: 4096 mixed buffers of 4 KiB - 4 MiB at level 1, zlib, batched")
+                         "batch: 4096 mixed buffers of 4 KiB - 4 MiB at level 1, zlib, batched; "
+                         "serve: Poisson request stream through the continuous-batching server")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale HBM counters")
@@ -500,7 +636,7 @@ def main() -> int:
         print("bench.py measures one GPU only: every box here holds a single MI355X; "
               "batches shard across GPUs with no exchange (see the design notes).", file=sys.stderr)
         return 2
-    rec = {"l1": bench_l1, "l6": bench_l6, "batch": bench_batch}[args.workload](args)
+    rec = {"l1": bench_l1, "l6": bench_l6, "batch": bench_batch, "serve": bench_serve}[args.workload](args)
     try:
         rec["device"] = orot_amd.device_info()[0]
     except Exception:
