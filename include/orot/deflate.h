@@ -6,7 +6,8 @@
  * include/orot/deflate.h; each declaration cites the orot line it replaces.
  * Entry points of orot that are not part of the GPU compress path
  * (deflate_decompress, the streaming API, deflate_set_allocator) are not
- * exported by this library.
+ * exported by this library.  deflate_compress is served by the chunked GPU
+ * pipeline (its stream differs from orot's single-block one).
  *
  * Deviations from orot, all deliberate:
  *   - gzip trailers carry the IEEE CRC-32 on every host (orot's x86 build
@@ -57,6 +58,16 @@ typedef enum deflate_format {
  * src/api/deflate_api.cpp:81-98): n + n/4096 + n/16384 + n/2^25 + 13, plus 6
  * (zlib) or 18 (gzip). */
 size_t deflate_compress_bound(size_t in_size, deflate_format format);
+
+/* Whole-buffer compression (orot include/orot/deflate.h:69-77,
+ * src/api/deflate_api.cpp:100-136).  Returns 0 when out_capacity is below
+ * deflate_compress_bound(in_size, format), as orot does; the gzip header's XFL
+ * byte is 2 for levels >= 9, 4 for levels <= 2, else 0
+ * (src/formats/gzip_wrapper.cpp:31).  The stream is produced by the same
+ * chunked GPU pipeline as deflate_parallel_compress (the level's default
+ * chunk size), not as orot's single block. */
+size_t deflate_compress(const void* in, size_t in_size, void* out, size_t out_capacity, int level,
+                        deflate_format format);
 
 /* Parallel compressor (orot include/orot/deflate.h:120-145,
  * src/api/stream_api.cpp:384-406).  block_size 0 selects 128 KiB for levels

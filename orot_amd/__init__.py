@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "liborot_amd.so")
 # Every C symbol declared in include/orot/*.h.
 EXPORTED_SYMBOLS = (
     "deflate_compress_bound",
+    "deflate_compress",
     "deflate_parallel_new",
     "deflate_parallel_free",
     "deflate_parallel_compress",
@@ -144,6 +145,8 @@ def lib() -> ctypes.CDLL:
     vp, sz, u32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32
     L.deflate_compress_bound.restype = sz
     L.deflate_compress_bound.argtypes = [sz, ctypes.c_int]
+    L.deflate_compress.restype = sz
+    L.deflate_compress.argtypes = [vp, sz, vp, sz, ctypes.c_int, ctypes.c_int]
     L.deflate_parallel_new.restype = vp
     L.deflate_parallel_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, sz]
     L.deflate_parallel_free.restype = None
@@ -230,6 +233,17 @@ def _as_u8(data) -> np.ndarray:
 def compress_bound(n: int, fmt: int = Format.GZIP) -> int:
     """deflate_compress_bound (orot include/orot/deflate.h:66)."""
     return int(lib().deflate_compress_bound(n, int(fmt)))
+
+
+def compress(data, level: int = Level.DEFAULT, fmt: int = Format.GZIP) -> bytes:
+    """deflate_compress (orot include/orot/deflate.h:69-77): whole buffer,
+    output capacity from compress_bound."""
+    a = _as_u8(data)
+    out = np.empty(compress_bound(a.size, fmt), dtype=np.uint8)
+    n = lib().deflate_compress(a.ctypes.data, a.size, out.ctypes.data, out.size, int(level), int(fmt))
+    if n == 0:
+        raise Error(Result.BUF_ERROR, f"deflate_compress: {last_error()}")
+    return out[:n].tobytes()
 
 
 class ParallelCompressor:

@@ -273,6 +273,36 @@ int deflate_parallel_stage_times(const deflate_parallel_ctx* ctx, float* ms, int
     return ctx->eng.stage_times(ms, max_values);
 }
 
+// ---- whole buffer ----------------------------------------------------------
+
+// deflate_compress (orot include/orot/deflate.h:69-77, src/api/deflate_api.cpp:
+// 100-136 -> src/formats/{raw_deflate,zlib_wrapper,gzip_wrapper}.cpp): orot's
+// rules for this entry point -- 0 when out_capacity is below the bound, the
+// gzip XFL byte by level (gzip_wrapper.cpp:31) -- over the chunked GPU
+// pipeline with the level's default chunk size.
+size_t deflate_compress(const void* in, size_t n, void* out, size_t cap, int level, deflate_format format) {
+    if (!out || (!in && n)) return 0;
+    if (cap < deflate_compress_bound(n, format)) {
+        set_error_msg("deflate_compress: out_capacity is below deflate_compress_bound");
+        return 0;
+    }
+    const int lv = level < 0 ? 0 : (level > 12 ? 12 : level);
+    const int fmt = (format == DEFLATE_FORMAT_ZLIB || format == DEFLATE_FORMAT_GZIP) ? (int)format : 0;
+    // one context per (level, format), created on first use
+    static std::mutex mu;
+    static deflate_parallel_ctx* ctxs[13][3] = {};
+    deflate_parallel_ctx* ctx;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!ctxs[lv][fmt]) ctxs[lv][fmt] = deflate_parallel_new(lv, (deflate_format)fmt, 0, 0);
+        ctx = ctxs[lv][fmt];
+    }
+    if (!ctx) return 0;
+    const size_t got = deflate_parallel_compress(ctx, in, n, out, cap);
+    if (got && fmt == DEFLATE_FORMAT_GZIP) static_cast<uint8_t*>(out)[8] = lv >= 9 ? 2 : (lv <= 2 ? 4 : 0);
+    return got;
+}
+
 // ---- batches -------------------------------------------------------------
 
 deflate_batch_ctx* deflate_batch_new(int level, deflate_format format, size_t block_size, unsigned flags) {

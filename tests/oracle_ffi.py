@@ -154,3 +154,15 @@ def ref_lz77(data, level: int) -> np.ndarray:
     t = np.zeros(max(1, a.size), dtype=np.uint32)
     n = ref().ref_lz77(a.ctypes.data, a.size, level, t.ctypes.data)
     return t[:n]
+
+
+def ref_compress(data, level: int, fmt: int) -> bytes:
+    """orot's own whole-buffer deflate_compress (src/api/deflate_api.cpp:100-136)."""
+    a = _u8(data)
+    L = ref()
+    L.deflate_compress.restype = _sz
+    L.deflate_compress.argtypes = [_vp, _sz, _vp, _sz, ctypes.c_int, ctypes.c_int]
+    cap = a.size + (a.size >> 12) + (a.size >> 14) + (a.size >> 25) + 13 + 18
+    out = np.zeros(cap, dtype=np.uint8)
+    n = L.deflate_compress(a.ctypes.data, a.size, out.ctypes.data, cap, level, fmt)
+    return out[:n].tobytes()
