@@ -35,58 +35,108 @@ constexpr uint32_t MAX_CHUNK_L1 = 256u * 1024u;  // 12-bit tables: 8 sub-ranges 
 // Checksum partials per chunk.
 // ---------------------------------------------------------------------------
 // CRC-32 register (zero initial value) or Adler sums of one chunk, by 256
-// threads: each thread a contiguous segment, segments combined with the
-// x^(8 * bytes after) shift (CRC) or the weighted Adler sum.
+// threads.  The chunk is read in 64-byte pieces, piece q by thread q mod 256,
+// so every load instruction of a wave covers one contiguous 4 KiB window.
+// Partial results combine linearly:
+//   CRC:   a thread folds its pieces as acc = acc * x^(8*16384) + crc(piece)
+//          (16384 bytes separate its consecutive pieces; the multiply by that
+//          constant is four 256-entry LDS tables), then shifts acc by the
+//          bytes after its last piece;
+//   Adler: s1 = sum b, s2 = sum (len - p) * b over byte positions p, per piece
+//          from the byte sum and the position-weighted byte sum (v_dot4).
+constexpr uint32_t CK_THREADS = 256;
+constexpr uint32_t CK_PIECE = 64;
+constexpr uint32_t CK_STRIDE = CK_PIECE * CK_THREADS;
+
+struct CkTables {
+    uint32_t crc[4][256];  // slicing-by-4
+    uint32_t mx[4][256];   // mx[k][b] = (b << 8k) * x^(8 * CK_STRIDE) mod P
+};
+
+__device__ __forceinline__ void load_ck_tables(CkTables& t) {
+    const uint32_t xs = gf2_x8n(CK_STRIDE);
+    for (int i = threadIdx.x; i < 1024; i += CK_THREADS) {
+        t.crc[i >> 8][i & 255] = c_crc.t[i >> 8][i & 255];
+        t.mx[i >> 8][i & 255] = gf2_mul((uint32_t)(i & 255) << (8 * (i >> 8)), xs);
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t crc_word(uint32_t crc, uint32_t w, const CkTables& t) {
+    crc ^= w;
+    return t.crc[3][crc & 0xFF] ^ t.crc[2][(crc >> 8) & 0xFF] ^ t.crc[1][(crc >> 16) & 0xFF] ^ t.crc[0][crc >> 24];
+}
+
+__device__ __forceinline__ uint32_t mul_stride(uint32_t a, const CkTables& t) {
+    return t.mx[0][a & 0xFF] ^ t.mx[1][(a >> 8) & 0xFF] ^ t.mx[2][(a >> 16) & 0xFF] ^ t.mx[3][a >> 24];
+}
+
 __device__ __forceinline__ void chunk_checksum(const uint8_t* __restrict__ s, uint32_t len, int kind,
-                                               ChunkInfo* __restrict__ ci, const uint32_t (*T)[256],
-                                               uint32_t* r0, uint32_t* r1) {
-    const int tid = threadIdx.x;
-    const uint32_t seg = (((len + 255u) / 256u) + 15u) & ~15u;
-    const uint32_t a = min(len, (uint32_t)tid * seg), b = min(len, a + seg);
-    const uint32_t after = len - b;
+                                               ChunkInfo* __restrict__ ci, const CkTables& t, uint32_t* r0,
+                                               uint32_t* r1) {
+    const uint32_t tid = threadIdx.x;
     if (kind == 0) {
-        uint32_t crc = 0;
-        uint32_t p = a;
-        for (; p + 4 <= b; p += 4) {
-            crc ^= ld32u(s + p);
-            crc = T[3][crc & 0xFF] ^ T[2][(crc >> 8) & 0xFF] ^ T[1][(crc >> 16) & 0xFF] ^ T[0][crc >> 24];
+        uint32_t acc = 0, end = 0;
+        bool any = false;
+        for (uint32_t o = tid * CK_PIECE; o < len; o += CK_STRIDE) {
+            const uint32_t L = min(CK_PIECE, len - o);
+            uint32_t crc = 0;
+            if (L == CK_PIECE) {
+                uint32_t w[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) w[k] = ld32u(s + o + 4 * k);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) crc = crc_word(crc, w[k], t);
+                acc = any ? (mul_stride(acc, t) ^ crc) : crc;
+            } else {
+                for (uint32_t p = o; p < o + L; ++p) crc = (crc >> 8) ^ t.crc[0][(crc ^ s[p]) & 0xFF];
+                acc = any ? (gf2_mul(gf2_x8n(CK_STRIDE - CK_PIECE + L), acc) ^ crc) : crc;
+            }
+            any = true;
+            end = o + L;
         }
-        for (; p < b; ++p) crc = (crc >> 8) ^ T[0][(crc ^ s[p]) & 0xFF];
-        r0[tid] = gf2_mul(gf2_x8n(after), crc);
+        r0[tid] = any ? gf2_mul(gf2_x8n(len - end), acc) : 0u;
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (tid < w) r0[tid] ^= r0[tid + w];
+        for (int w = CK_THREADS / 2; w > 0; w >>= 1) {
+            if ((int)tid < w) r0[tid] ^= r0[tid + w];
             __syncthreads();
         }
         if (tid == 0) ci->crc = r0[0];
     } else {
-        uint32_t s1 = 0, s2 = 0;
-        uint32_t p = a;
-        while (p < b) {
-            const uint32_t e = min(b, p + 4096u);
-            for (; p + 4 <= e; p += 4) {
-                const uint32_t w = ld32u(s + p);
-                s1 += w & 0xFF;
-                s2 += s1;
-                s1 += (w >> 8) & 0xFF;
-                s2 += s1;
-                s1 += (w >> 16) & 0xFF;
-                s2 += s1;
-                s1 += w >> 24;
-                s2 += s1;
+        uint64_t A = 0, B = 0;  // sum b, sum (len - p) * b
+        uint32_t it = 0;
+        for (uint32_t o = tid * CK_PIECE; o < len; o += CK_STRIDE) {
+            const uint32_t L = min(CK_PIECE, len - o);
+            uint32_t S = 0, K = 0;  // byte sum, sum of (p - o) * b
+            if (L == CK_PIECE) {
+                uint32_t w[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) w[k] = ld32u(s + o + 4 * k);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const uint32_t sb = __builtin_amdgcn_udot4(w[k], 0x01010101u, 0u, false);
+                    S += sb;
+                    K += 4u * (uint32_t)k * sb + __builtin_amdgcn_udot4(w[k], 0x03020100u, 0u, false);
+                }
+            } else {
+                for (uint32_t p = o; p < o + L; ++p) {
+                    S += s[p];
+                    K += (p - o) * s[p];
+                }
             }
-            for (; p < e; ++p) {
-                s1 += s[p];
-                s2 += s1;
+            A += S;
+            B += (uint64_t)(len - o) * S - K;
+            if (++it == 4096) {  // keep the 64-bit sums far from overflow on huge chunks
+                A %= ADLER_MOD;
+                B %= ADLER_MOD;
+                it = 0;
             }
-            s1 %= ADLER_MOD;
-            s2 %= ADLER_MOD;
         }
-        r0[tid] = s1;
-        r1[tid] = (uint32_t)(((uint64_t)s1 * (after % ADLER_MOD) + s2) % ADLER_MOD);
+        r0[tid] = (uint32_t)(A % ADLER_MOD);
+        r1[tid] = (uint32_t)(B % ADLER_MOD);
         __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (tid < w) {
+        for (int w = CK_THREADS / 2; w > 0; w >>= 1) {
+            if ((int)tid < w) {
                 r0[tid] = (r0[tid] + r0[tid + w]) % ADLER_MOD;
                 r1[tid] = (r1[tid] + r1[tid + w]) % ADLER_MOD;
             }
@@ -101,29 +151,27 @@ __device__ __forceinline__ void chunk_checksum(const uint8_t* __restrict__ s, ui
 }
 
 // Checksum partials of every chunk of the batch (grid-stride over chunks).
-__global__ __launch_bounds__(256) void checksum_kernel(const uint8_t* __restrict__ src,
-                                                       const ChunkDesc* __restrict__ desc,
-                                                       const BatchHdr* __restrict__ hdr, int kind,
-                                                       ChunkInfo* __restrict__ info) {
-    __shared__ uint32_t T[4][256];
-    __shared__ uint32_t r0[256];
-    __shared__ uint32_t r1[256];
-    for (int i = threadIdx.x; i < 1024; i += 256) T[i >> 8][i & 255] = c_crc.t[i >> 8][i & 255];
-    __syncthreads();
+__global__ __launch_bounds__(CK_THREADS) void checksum_kernel(const uint8_t* __restrict__ src,
+                                                              const ChunkDesc* __restrict__ desc,
+                                                              const BatchHdr* __restrict__ hdr, int kind,
+                                                              ChunkInfo* __restrict__ info) {
+    __shared__ CkTables T;
+    __shared__ uint32_t r0[CK_THREADS];
+    __shared__ uint32_t r1[CK_THREADS];
+    load_ck_tables(T);
     const uint32_t nchunks = hdr->nchunks;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x)
         chunk_checksum(src + desc[c].in_off, desc[c].len, kind, &info[c], T, r0, r1);
 }
 
 // Checksum partials of a plain buffer in uniform chunks (whole-buffer entry points).
-__global__ __launch_bounds__(256) void checksum_uniform_kernel(const uint8_t* __restrict__ src, uint64_t n,
-                                                               uint32_t chunk, int kind,
-                                                               ChunkInfo* __restrict__ info) {
-    __shared__ uint32_t T[4][256];
-    __shared__ uint32_t r0[256];
-    __shared__ uint32_t r1[256];
-    for (int i = threadIdx.x; i < 1024; i += 256) T[i >> 8][i & 255] = c_crc.t[i >> 8][i & 255];
-    __syncthreads();
+__global__ __launch_bounds__(CK_THREADS) void checksum_uniform_kernel(const uint8_t* __restrict__ src, uint64_t n,
+                                                                      uint32_t chunk, int kind,
+                                                                      ChunkInfo* __restrict__ info) {
+    __shared__ CkTables T;
+    __shared__ uint32_t r0[CK_THREADS];
+    __shared__ uint32_t r1[CK_THREADS];
+    load_ck_tables(T);
     const uint32_t c = blockIdx.x;
     const uint64_t base = (uint64_t)c * chunk;
     chunk_checksum(src + base, (uint32_t)min<uint64_t>(chunk, n - base), kind, &info[c], T, r0, r1);
