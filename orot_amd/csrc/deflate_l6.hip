@@ -210,20 +210,34 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
             uint32_t best = 0, bd = 0;
             if (x + 4 <= n) {
                 const uint32_t cap = min(258u, n - x);
-                const uint32_t w = ring_u32(DR, x);
+                // bytes x .. x+15 (beyond the chunk end they are stale or zero,
+                // which only lengthens a comparison that min(l, cap) cuts back)
+                const uint64_t xa = ring_u64(DR, x), xb = ring_u64(DR, x + 8);
+                const uint32_t w = (uint32_t)xa;
                 uint32_t cur = x;
                 for (int k = 0; k < max_chain; ++k) {
                     const uint32_t st = PR[cur % L6M_PRING];
                     if (st == 0) break;
                     const uint32_t q = cur - st;
                     if (x - q > 32768u) break;
-                    if (ring_u32(DR, q) == w) {
-                        uint32_t l = 4;
+                    cur = q;
+                    if (ring_u32(DR, q) != w) continue;
+                    // only a match that also covers byte `best` can be longer
+                    // than best (best < cap here, so these bytes are in the chunk)
+                    if (best >= 4 && ring_u32(DR, q + best - 3) != ring_u32(DR, x + best - 3)) continue;
+                    uint32_t l;
+                    uint64_t df = ring_u64(DR, q) ^ xa;
+                    if (df) {
+                        l = (uint32_t)(__builtin_ctzll(df) >> 3);
+                    } else if ((df = ring_u64(DR, q + 8) ^ xb) != 0) {
+                        l = 8 + (uint32_t)(__builtin_ctzll(df) >> 3);
+                    } else {
+                        l = 16;
                         while (l < cap) {
                             if (l + 8 <= cap) {
-                                const uint64_t df = ring_u64(DR, x + l) ^ ring_u64(DR, q + l);
-                                if (df) {
-                                    l += (uint32_t)(__builtin_ctzll(df) >> 3);
+                                const uint64_t d2 = ring_u64(DR, x + l) ^ ring_u64(DR, q + l);
+                                if (d2) {
+                                    l += (uint32_t)(__builtin_ctzll(d2) >> 3);
                                     break;
                                 }
                                 l += 8;
@@ -232,14 +246,14 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                                 ++l;
                             }
                         }
-                        l = min(l, cap);
-                        if (l > best) {
-                            best = l;
-                            bd = x - q;
-                            if ((int)l >= nice) break;
-                        }
                     }
-                    cur = q;
+                    l = min(l, cap);
+                    if (l > best) {
+                        best = l;
+                        bd = x - q;
+                        // nothing later can be longer than nice (stop rule) or cap
+                        if ((int)l >= nice || l >= cap) break;
+                    }
                 }
             }
             ML[x] = best >= 3 ? (best | (bd << 16)) : 0u;
@@ -460,53 +474,88 @@ __device__ __forceinline__ uint32_t fixed_ll_len(uint32_t sym) {
 // sorted by (freq, index), a leaf precedes a package of equal weight.  One
 // thread; idx / wprev / wcur / leafbits are LDS scratch.
 __device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, uint8_t* lens, uint16_t* idx,
-                                uint32_t* wprev, uint32_t* wcur, uint8_t* leafbits /* [16][576] */) {
-    int n = 0;
-    for (int i = 0; i < nsym; ++i) {
-        lens[i] = 0;
-        if (freq[i]) idx[n++] = (uint16_t)i;
+                                uint32_t* wleaf, uint32_t* wa, uint32_t* wb, uint8_t* leafbits /* [16][576] */,
+                                uint32_t* s_cnt) {
+    // Called by every thread of the workgroup.
+    const int tid = threadIdx.x;
+    if (tid == 0) *s_cnt = 0;
+    for (int i = tid; i < nsym; i += L6B_THREADS) lens[i] = 0;
+    __syncthreads();
+    // active symbols in (freq, symbol) order: each one's rank among the active
+    for (int i = tid; i < nsym; i += L6B_THREADS) {
+        const uint32_t f = freq[i];
+        if (!f) continue;
+        int r = 0;
+        for (int j = 0; j < nsym; ++j) {
+            const uint32_t g = freq[j];
+            r += (g != 0 && (g < f || (g == f && j < i))) ? 1 : 0;
+        }
+        idx[r] = (uint16_t)i;
+        atomicAdd(s_cnt, 1u);
     }
+    __syncthreads();
+    const int n = (int)*s_cnt;
+    __syncthreads();
     if (n == 0) return;
     if (n == 1) {
-        lens[idx[0]] = 1;
+        if (tid == 0) lens[idx[0]] = 1;
+        __syncthreads();
         return;
     }
-    for (int i = 1; i < n; ++i) {
-        const uint16_t v = idx[i];
-        int j = i - 1;
-        while (j >= 0 && freq[idx[j]] > freq[v]) {
-            idx[j + 1] = idx[j];
-            --j;
-        }
-        idx[j + 1] = v;
-    }
-    for (int i = 0; i < n; ++i) {
-        wprev[i] = freq[idx[i]];
+    for (int i = tid; i < n; i += L6B_THREADS) {
+        wleaf[i] = freq[idx[i]];
+        wa[i] = wleaf[i];
         leafbits[max_bits * 576 + i] = 1;
     }
+    __syncthreads();
+    // each level: the sorted leaves merged with the sorted packages (pairs of the
+    // level below); a leaf goes first on equal weight.  Leaf a lands at
+    // a + #{packages lighter than it}, package p at p + #{leaves not heavier}.
+    uint32_t* wprev = wa;
+    uint32_t* wcur = wb;
     int cntp = n;
     for (int l = max_bits - 1; l >= 1; --l) {
         const int np = cntp / 2;
-        int a = 0, p = 0, k = 0;
-        while (a < n || p < np) {
-            const uint32_t pw = p < np ? wprev[2 * p] + wprev[2 * p + 1] : 0xFFFFFFFFu;
-            if (a < n && freq[idx[a]] <= pw) {
-                wcur[k] = freq[idx[a++]];
-                leafbits[l * 576 + k++] = 1;
-            } else {
-                wcur[k] = pw;
-                leafbits[l * 576 + k++] = 0;
-                ++p;
+        for (int a = tid; a < n; a += L6B_THREADS) {
+            const uint32_t f = wleaf[a];
+            int lo = 0, hi = np;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (wprev[2 * mid] + wprev[2 * mid + 1] < f) lo = mid + 1;
+                else hi = mid;
             }
+            wcur[a + lo] = f;
+            leafbits[l * 576 + a + lo] = 1;
         }
-        for (int i = 0; i < k; ++i) wprev[i] = wcur[i];
-        cntp = k;
+        for (int q = tid; q < np; q += L6B_THREADS) {
+            const uint32_t pw = wprev[2 * q] + wprev[2 * q + 1];
+            int lo = 0, hi = n;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (wleaf[mid] <= pw) lo = mid + 1;
+                else hi = mid;
+            }
+            wcur[q + lo] = pw;
+            leafbits[l * 576 + q + lo] = 0;
+        }
+        __syncthreads();
+        uint32_t* t = wprev;
+        wprev = wcur;
+        wcur = t;
+        cntp = n + np;
     }
+    // a leaf's length = the number of levels whose selected prefix holds it
     int take = 2 * n - 2;
     for (int l = 1; l <= max_bits && take > 0; ++l) {
-        int leaves = 0;
-        for (int i = 0; i < take; ++i) leaves += leafbits[l * 576 + i];
-        for (int i = 0; i < leaves; ++i) lens[idx[i]]++;
+        if (tid == 0) *s_cnt = 0;
+        __syncthreads();
+        uint32_t cnt = 0;
+        for (int i = tid; i < take; i += L6B_THREADS) cnt += leafbits[l * 576 + i];
+        if (cnt) atomicAdd(s_cnt, cnt);
+        __syncthreads();
+        const int leaves = (int)*s_cnt;
+        for (int i = tid; i < leaves; i += L6B_THREADS) lens[idx[i]]++;
+        __syncthreads();
         take = 2 * (take - leaves);
     }
 }
@@ -519,9 +568,13 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
     __shared__ uint32_t dst[32];
     __shared__ uint32_t rawsum;
     __shared__ uint16_t idx[288];
-    __shared__ uint32_t wprev[576];
-    __shared__ uint32_t wcur[576];
+    __shared__ uint32_t wleaf[288];
+    __shared__ uint32_t wa[576];
+    __shared__ uint32_t wb[576];
     __shared__ uint8_t leafbits[16 * 576];
+    __shared__ uint32_t clf[19];
+    __shared__ uint32_t s_cnt;
+    __shared__ uint64_t s_extra;
     __shared__ BlockRec R;
     const uint32_t ntok = ntok_in[c];
     const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
@@ -555,24 +608,20 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
         R.t1 = t1;
         R.raw_len = rawsum;
         R.raw_pos = b == 0 ? 0u : blkpos[(uint64_t)c * maxblk + b];
-        // lengths
-        limited_lengths(lit, 288, 15, R.ll, idx, wprev, wcur, leafbits);
-        limited_lengths(dst, 32, 15, R.dl, idx, wprev, wcur, leafbits);
+    }
+    __syncthreads();
+    // lengths
+    limited_lengths(lit, 288, 15, R.ll, idx, wleaf, wa, wb, leafbits, &s_cnt);
+    limited_lengths(dst, 32, 15, R.dl, idx, wleaf, wa, wb, leafbits, &s_cnt);
+    if (tid == 0) {
         // fixed and dynamic sizes (deflate_block.cpp:50-92 for the fixed estimate)
-        uint64_t fixedb = 3, datab = 0;
-        for (int i = 0; i < 288; ++i) {
-            fixedb += (uint64_t)fixed_ll_len(i) * lit[i];
-            datab += (uint64_t)R.ll[i] * lit[i];
-        }
-        for (int i = 0; i < 32; ++i) {
-            fixedb += 5ull * dst[i];
-            datab += (uint64_t)R.dl[i] * dst[i];
-        }
+        uint64_t fixedb = 3;
+        for (int i = 0; i < 288; ++i) fixedb += (uint64_t)fixed_ll_len(i) * lit[i];
+        for (int i = 0; i < 32; ++i) fixedb += 5ull * dst[i];
         uint64_t extra = 0;
         for (int i = 0; i < 29; ++i) extra += (uint64_t)c_len_extra[i] * lit[257 + i];
         for (int i = 0; i < 30; ++i) extra += (uint64_t)c_dist_extra[i] * dst[i];
         fixedb += extra;
-        datab += extra;
         // dynamic header: EOB / distance fix-ups, trimmed counts, code-length RLE
         // (deflate_block.cpp:196-290, 347-370)
         if (!R.ll[256]) R.ll[256] = 1;
@@ -610,10 +659,20 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
                 ++i;
             }
         }
-        uint32_t clf[19];
         for (int i = 0; i < 19; ++i) clf[i] = 0;
         for (int i = 0; i < nr; ++i) clf[R.rle_sym[i]]++;
-        limited_lengths(clf, 19, 7, R.cl, idx, wprev, wcur, leafbits);
+        R.hlit = (uint32_t)hlit;
+        R.hdist = (uint32_t)hdist;
+        R.nrle = (uint32_t)nr;
+        R.body_bits = fixedb;  // parked here until the type is chosen
+        s_extra = extra;
+    }
+    __syncthreads();
+    limited_lengths(clf, 19, 7, R.cl, idx, wleaf, wa, wb, leafbits, &s_cnt);
+    if (tid == 0) {
+        const int nr = (int)R.nrle;
+        const uint64_t extra = s_extra;
+        const uint64_t fixedb = R.body_bits;
         int act = 0;
         for (int i = 0; i < 19; ++i) act += R.cl[i] != 0;
         if (act == 1) {
@@ -636,10 +695,7 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
         for (int i = 0; i < 32; ++i) dyn_data += (uint64_t)R.dl[i] * dst[i];
         const uint64_t dynb = 3ull + hb + dyn_data;
         const uint64_t storedb = ((uint64_t)R.raw_len + 5) * 8;
-        R.hlit = (uint32_t)hlit;
-        R.hdist = (uint32_t)hdist;
         R.hclen = (uint32_t)hclen;
-        R.nrle = (uint32_t)nr;
         R.hdr_bits = hb;
         if (storedb <= fixedb && storedb <= dynb) {
             R.type = 0;
@@ -651,7 +707,6 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
             R.type = 2;
             R.body_bits = dynb;
         }
-        (void)datab;
     }
     __syncthreads();
     // copy the record out
@@ -881,14 +936,42 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
         wsum[L6E_THREADS / 64] = acc;
     }
     __syncthreads();
-    uint64_t at = sym0 + wsum[wv] + incl - mybits;
-    for (uint32_t t = a; t < e; ++t) {
-        uint64_t cd;
-        uint32_t ln;
-        tok_code(TK[t], cd, ln);
-        put_bits_at(stg, at, (uint32_t)cd, min(ln, 32u));
-        if (ln > 32) put_bits_at(stg, at + 32, (uint32_t)(cd >> 32), ln - 32);
-        at += ln;
+    // Pack the slice in a 64-bit register; words wholly inside it are plain
+    // stores, its first and last words (shared with a neighbour, the header
+    // or the EOB) are atomicOr.
+    if (mybits) {
+        const uint64_t at = sym0 + wsum[wv] + incl - mybits;
+        uint64_t wi = at >> 5;
+        uint32_t nacc = (uint32_t)(at & 31);
+        bool partial = nacc != 0;
+        uint64_t acc = 0;
+        auto flush_word = [&]() {
+            if (partial) {
+                atomicOr(&stg[wi], (uint32_t)acc);
+                partial = false;
+            } else {
+                stg[wi] = (uint32_t)acc;
+            }
+            ++wi;
+            acc >>= 32;
+            nacc -= 32;
+        };
+        for (uint32_t t = a; t < e; ++t) {
+            uint64_t cd;
+            uint32_t ln;
+            tok_code(TK[t], cd, ln);
+            // at most 48 bits: add in two halves so the register never overflows
+            const uint32_t l0 = min(ln, 32u);
+            acc |= (cd & ((1ull << l0) - 1ull)) << nacc;
+            nacc += l0;
+            if (nacc >= 32) flush_word();
+            if (ln > 32) {
+                acc |= (cd >> 32) << nacc;
+                nacc += ln - 32;
+                if (nacc >= 32) flush_word();
+            }
+        }
+        if (nacc) atomicOr(&stg[wi], (uint32_t)acc);
     }
     if (tid == 0) put_bits_at(stg, sym0 + wsum[L6E_THREADS / 64], llc[256], lls[256]);  // EOB
 }
