@@ -1076,6 +1076,7 @@ __global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __r
 //   C   block-wide scan, stored / fixed choice (deflate_block.cpp:414-421),
 //       emission into the chunk's staging bitstream.
 // ---------------------------------------------------------------------------
+static_assert(LC_MAX_CHUNK / LC_THREADS <= 128, "a range spans at most four bitmap words");
 constexpr uint32_t LB_BM_OFF = LC_REGION0 / 4;               // words: table 0, then the token bitmap
 constexpr uint32_t LB_T1_OFF = LB_BM_OFF + LC_BM_WORDS;       // table 1 (two sub-ranges), then fb / misc
 constexpr uint32_t LB_FB_OFF = LB_T1_OFF;                     // first barrier per range
@@ -1107,28 +1108,6 @@ __device__ __forceinline__ uint32_t code_step(uint32_t b, const uint32_t* T, uin
 
 // Fixed-Huffman bit count of the tokens whose start bit is set in
 // bm over [a, b); T read 8 tokens at a time.
-__device__ __forceinline__ uint32_t range_bits(const uint32_t* bm, const uint32_t* T, uint32_t a, uint32_t b) {
-    uint32_t bits = 0;
-    uint32_t pk[8];
-    int cnt = 0;
-    for (uint32_t w = a >> 5; w < ((b + 31) >> 5); ++w) {
-        uint32_t x = bm[w];
-        while (x) {
-            pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
-            x &= x - 1;
-            if (cnt == 8) {
-                uint32_t tv[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) bits += tok_bits(tv[k]);
-                cnt = 0;
-            }
-        }
-    }
-    for (int k = 0; k < cnt; ++k) bits += tok_bits(T[pk[k]]);
-    return bits;
-}
 
 template <int W, bool EXCH>
 __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
@@ -1396,8 +1375,55 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     __syncthreads();
 
     // ---- C: per range bit counts, block scan, stored / fixed choice
-    // (deflate_block.cpp:414-421), emission of the range's tokens.
-    const uint32_t mybits = r0 < r1 ? range_bits(bm, T, r0, r1) : 0u;
+    // (deflate_block.cpp:414-421).  The block is assembled in LDS over the
+    // dead length codes (a fixed block is under n + 5 bytes, so it may reach
+    // three words into the bitmap, which every range has copied to registers
+    // by then) and copied to the chunk's staging words in one coalesced pass.
+    const uint32_t w0 = r0 >> 5, nwd = r0 < r1 ? ((r1 + 31) >> 5) - w0 : 0u;  // <= 4 words
+    uint32_t myw[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) myw[k] = (uint32_t)k < nwd ? bm[w0 + k] : 0u;
+    // token words are streamed in range order, 8 positions (two 16-byte
+    // loads) at a time, skipping groups without a token start; the next
+    // group's loads are issued before the current group is used
+    auto group_mask = [&](uint32_t p0) -> uint32_t {
+        const uint32_t k = (p0 >> 5) - w0;
+        const uint32_t wv32 = k == 0 ? myw[0] : (k == 1 ? myw[1] : (k == 2 ? myw[2] : myw[3]));
+        return (wv32 >> (p0 & 31)) & 0xFFu;
+    };
+    auto load_group = [&](uint32_t p0, uint32_t* tv) {
+        if (p0 + 8 <= r1) {
+            const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
+            const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
+            tv[0] = v0.x; tv[1] = v0.y; tv[2] = v0.z; tv[3] = v0.w;
+            tv[4] = v1.x; tv[5] = v1.y; tv[6] = v1.z; tv[7] = v1.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) tv[j] = p0 + j < r1 ? T[p0 + j] : 0u;
+        }
+    };
+    auto next_group = [&](uint32_t p0) -> uint32_t {  // first group at or after p0 with a token start
+        while (p0 < r1 && group_mask(p0) == 0) p0 += 8;
+        return p0;
+    };
+    uint32_t mybits = 0;
+    {
+        uint32_t g = nwd ? next_group(r0) : r1;
+        uint32_t nx[8];
+        if (g < r1) load_group(g, nx);
+        while (g < r1) {
+            uint32_t tv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) tv[j] = nx[j];
+            const uint32_t m = group_mask(g);
+            const uint32_t gn = next_group(g + 8);
+            if (gn < r1) load_group(gn, nx);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if ((m >> j) & 1u) mybits += tok_bits(tv[j]);
+            g = gn;
+        }
+    }
     const uint32_t incl = wave_incl_scan(mybits, lane);
     if (lane == 63) misc[wv] = incl;
     __syncthreads();
@@ -1421,11 +1447,12 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         info[c].is_stored = stored ? 1u : 0u;
     }
     if (stored) return;
-    uint32_t* stg = stage + (uint64_t)c * stage_words;
+    uint32_t* out = smem;
     const uint32_t nw = (uint32_t)((fixed + 31) >> 5) + 1u;
-    for (uint32_t i = tid; i < nw; i += LC_THREADS) stg[i] = 0;
+    __syncthreads();  // bitmap words and the scan totals are in registers
+    for (uint32_t i = tid; i < nw; i += LC_THREADS) out[i] = 0;
     __syncthreads();
-    if (tid == 0) atomicOr(&stg[0], (is_last ? 1u : 0u) | 2u);  // BFINAL, BTYPE = 01
+    if (tid == 0) atomicOr(&out[0], (is_last ? 1u : 0u) | 2u);  // BFINAL, BTYPE = 01
     if (mybits) {
         const uint32_t bitpos = 3u + excl;
         uint32_t wi = bitpos >> 5;
@@ -1441,35 +1468,35 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             nacc += len;
             if (nacc >= 32) {
                 if (partial) {
-                    atomicOr(&stg[wi], (uint32_t)acc);
+                    atomicOr(&out[wi], (uint32_t)acc);
                     partial = false;
                 } else {
-                    stg[wi] = (uint32_t)acc;
+                    out[wi] = (uint32_t)acc;
                 }
                 ++wi;
                 acc >>= 32;
                 nacc -= 32;
             }
         };
-        uint32_t pk[8];
-        int cnt = 0;
-        for (uint32_t w = r0 >> 5; w < ((r1 + 31) >> 5); ++w) {
-            uint32_t x = bm[w];
-            while (x) {
-                pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
-                x &= x - 1;
-                if (cnt == 8) {
-                    uint32_t tv[8];
+        uint32_t g = next_group(r0);
+        uint32_t nx[8];
+        if (g < r1) load_group(g, nx);
+        while (g < r1) {
+            uint32_t tv[8];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
-                    for (int k = 0; k < 8; ++k) put(tv[k]);
-                    cnt = 0;
-                }
-            }
+            for (int j = 0; j < 8; ++j) tv[j] = nx[j];
+            const uint32_t m = group_mask(g);
+            const uint32_t gn = next_group(g + 8);
+            if (gn < r1) load_group(gn, nx);
+            for (int j = 0; j < 8; ++j)
+                if ((m >> j) & 1u) put(tv[j]);
+            g = gn;
         }
-        for (int k = 0; k < cnt; ++k) put(T[pk[k]]);
-        if (nacc) atomicOr(&stg[wi], (uint32_t)acc);
+        if (nacc) atomicOr(&out[wi], (uint32_t)acc);
     }
+    __syncthreads();
+    uint32_t* stg = stage + (uint64_t)c * stage_words;
+    for (uint32_t i = tid; i < nw; i += LC_THREADS) stg[i] = out[i];
 }
 
 template <int W, bool EXCH>
