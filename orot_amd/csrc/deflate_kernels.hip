@@ -1770,12 +1770,33 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
         const uint64_t e = has_next ? nv.off : res[bi].payload_bits;
         const uint64_t b0 = (o + 7) >> 3, b1 = (e + 7) >> 3;
         uint8_t* dst = out_arena + bufs[bi].out_off + hdr_bytes;
-        for (uint64_t b = b0 + threadIdx.x; b < b1; b += 256) {
+        auto byte_at = [&](uint64_t b) -> uint32_t {
             const uint64_t r = 8 * b - o;
             const uint32_t k = (uint32_t)min<uint64_t>(8, e - 8 * b);
             uint32_t v = chunk_bits(cv, r, k);
             if (k < 8 && has_next) v |= chunk_bits(nv, 0, 8 - k) << k;
-            dst[b] = (uint8_t)v;
+            return v;
+        };
+        // bytes before the first and after the last 4-byte-aligned output word
+        // one at a time; the aligned words in between with one store each
+        const uint64_t a0 = (uint64_t)(uintptr_t)(dst + b0);
+        const uint64_t wb0 = min(b1, b0 + ((4 - (a0 & 3)) & 3));   // first byte of the aligned run
+        const uint64_t nwords = (b1 - wb0) >> 2;
+        const uint64_t wb1 = wb0 + 4 * nwords;                       // first byte after it
+        for (uint64_t b = b0 + threadIdx.x; b < wb0; b += 256) dst[b] = (uint8_t)byte_at(b);
+        for (uint64_t b = wb1 + threadIdx.x; b < b1; b += 256) dst[b] = (uint8_t)byte_at(b);
+        for (uint64_t w = threadIdx.x; w < nwords; w += 256) {
+            const uint64_t b = wb0 + 4 * w;
+            uint32_t v;
+            if (!cv.stored && 8 * (b + 4) <= e) {
+                const uint64_t r = 8 * b - o;
+                const uint64_t wi = r >> 5;
+                const uint64_t x = (uint64_t)cv.stg[wi] | ((uint64_t)cv.stg[wi + 1] << 32);
+                v = (uint32_t)(x >> (r & 31));
+            } else {
+                v = byte_at(b) | (byte_at(b + 1) << 8) | (byte_at(b + 2) << 16) | (byte_at(b + 3) << 24);
+            }
+            *reinterpret_cast<uint32_t*>(dst + b) = v;
         }
     }
 }
