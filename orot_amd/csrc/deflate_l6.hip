@@ -206,57 +206,86 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
         dfill = max(dfill, dneed);
         for (uint32_t q = x0 + tid; q < xe; q += L6M_THREADS) PR[q % L6M_PRING] = PD[q];
         __syncthreads();
-        for (uint32_t x = x0 + tid; x < xe; x += L6M_THREADS) {
-            uint32_t best = 0, bd = 0;
-            if (x + 4 <= n) {
-                const uint32_t cap = min(258u, n - x);
+        // Two positions per thread, x and x + L6M_THREADS, walk their chains in
+        // step so each step has two independent LDS loads in flight.
+        for (uint32_t xp = x0 + tid; xp < xe; xp += 2 * L6M_THREADS) {
+            struct Walk {
+                uint32_t x, cap, cur, best, bd, w;
+                uint64_t xa, xb;
+                int k;
+                bool live;
+            } m[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const uint32_t x = xp + (uint32_t)i * L6M_THREADS;
+                m[i].x = x;
+                m[i].best = 0;
+                m[i].bd = 0;
+                m[i].k = 0;
+                m[i].cur = x;
+                m[i].live = x < xe && x + 4 <= n;
+                m[i].cap = m[i].live ? min(258u, n - x) : 0u;
                 // bytes x .. x+15 (beyond the chunk end they are stale or zero,
                 // which only lengthens a comparison that min(l, cap) cuts back)
-                const uint64_t xa = ring_u64(DR, x), xb = ring_u64(DR, x + 8);
-                const uint32_t w = (uint32_t)xa;
-                uint32_t cur = x;
-                for (int k = 0; k < max_chain; ++k) {
-                    const uint32_t st = PR[cur % L6M_PRING];
-                    if (st == 0) break;
-                    const uint32_t q = cur - st;
-                    if (x - q > 32768u) break;
-                    cur = q;
-                    if (ring_u32(DR, q) != w) continue;
+                m[i].xa = ring_u64(DR, x);
+                m[i].xb = ring_u64(DR, x + 8);
+                m[i].w = (uint32_t)m[i].xa;
+            }
+            while (m[0].live || m[1].live) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    Walk& a = m[i];
+                    if (!a.live) continue;
+                    if (a.k >= max_chain) {
+                        a.live = false;
+                        continue;
+                    }
+                    ++a.k;
+                    const uint32_t st = PR[a.cur % L6M_PRING];
+                    const uint32_t q = a.cur - st;
+                    if (st == 0 || a.x - q > 32768u) {
+                        a.live = false;
+                        continue;
+                    }
+                    a.cur = q;
+                    if (ring_u32(DR, q) != a.w) continue;
                     // only a match that also covers byte `best` can be longer
                     // than best (best < cap here, so these bytes are in the chunk)
-                    if (best >= 4 && ring_u32(DR, q + best - 3) != ring_u32(DR, x + best - 3)) continue;
+                    if (a.best >= 4 && ring_u32(DR, q + a.best - 3) != ring_u32(DR, a.x + a.best - 3)) continue;
                     uint32_t l;
-                    uint64_t df = ring_u64(DR, q) ^ xa;
+                    uint64_t df = ring_u64(DR, q) ^ a.xa;
                     if (df) {
                         l = (uint32_t)(__builtin_ctzll(df) >> 3);
-                    } else if ((df = ring_u64(DR, q + 8) ^ xb) != 0) {
+                    } else if ((df = ring_u64(DR, q + 8) ^ a.xb) != 0) {
                         l = 8 + (uint32_t)(__builtin_ctzll(df) >> 3);
                     } else {
                         l = 16;
-                        while (l < cap) {
-                            if (l + 8 <= cap) {
-                                const uint64_t d2 = ring_u64(DR, x + l) ^ ring_u64(DR, q + l);
+                        while (l < a.cap) {
+                            if (l + 8 <= a.cap) {
+                                const uint64_t d2 = ring_u64(DR, a.x + l) ^ ring_u64(DR, q + l);
                                 if (d2) {
                                     l += (uint32_t)(__builtin_ctzll(d2) >> 3);
                                     break;
                                 }
                                 l += 8;
                             } else {
-                                if (ring_u8(DR, x + l) != ring_u8(DR, q + l)) break;
+                                if (ring_u8(DR, a.x + l) != ring_u8(DR, q + l)) break;
                                 ++l;
                             }
                         }
                     }
-                    l = min(l, cap);
-                    if (l > best) {
-                        best = l;
-                        bd = x - q;
+                    l = min(l, a.cap);
+                    if (l > a.best) {
+                        a.best = l;
+                        a.bd = a.x - q;
                         // nothing later can be longer than nice (stop rule) or cap
-                        if ((int)l >= nice || l >= cap) break;
+                        if ((int)l >= nice || l >= a.cap) a.live = false;
                     }
                 }
             }
-            ML[x] = best >= 3 ? (best | (bd << 16)) : 0u;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                if (m[i].x < xe) ML[m[i].x] = m[i].best >= 3 ? (m[i].best | (m[i].bd << 16)) : 0u;
         }
         __syncthreads();
     }
