@@ -1297,20 +1297,17 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     }
     __syncthreads();
 
-    // ---- K: length codes over the bytes; bitmap cleared.
-    for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
-        if (p0 + 8 <= r1) {
-            const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
-            const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
-            D[p0 >> 2] = len_code8(tok_len(v0.x)) | (len_code8(tok_len(v0.y)) << 8) |
-                         (len_code8(tok_len(v0.z)) << 16) | (len_code8(tok_len(v0.w)) << 24);
-            D[(p0 >> 2) + 1] = len_code8(tok_len(v1.x)) | (len_code8(tok_len(v1.y)) << 8) |
-                               (len_code8(tok_len(v1.z)) << 16) | (len_code8(tok_len(v1.w)) << 24);
+    // ---- K: length codes over the bytes; bitmap cleared (lane-coalesced).
+    for (uint32_t p0 = (uint32_t)tid * 4u; p0 < n; p0 += LC_THREADS * 4u) {
+        if (p0 + 4 <= n) {
+            const uint4 v = *reinterpret_cast<const uint4*>(T + p0);
+            D[p0 >> 2] = len_code8(tok_len(v.x)) | (len_code8(tok_len(v.y)) << 8) | (len_code8(tok_len(v.z)) << 16) |
+                         (len_code8(tok_len(v.w)) << 24);
         } else {
-            for (uint32_t p = p0; p < r1; ++p) code[p] = (uint8_t)len_code8(tok_len(T[p]));
+            for (uint32_t p = p0; p < n; ++p) code[p] = (uint8_t)len_code8(tok_len(T[p]));
         }
     }
-    for (uint32_t w = r0 >> 5; w < ((r1 + 31) >> 5); ++w) bm[w] = 0;
+    for (uint32_t w = tid; w < ((n + 31) >> 5); w += LC_THREADS) bm[w] = 0;
     __syncthreads();
 
     // ---- P1: first parse barrier of each range.  reach(p) = p + step(p);
