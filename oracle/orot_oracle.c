@@ -398,6 +398,11 @@ void oracle_l1_candidates(const uint8_t* src, size_t n, int hash_bits, uint16_t*
  *            the walk stops at nice_len;
  *   parse    lazy, lazy_depth steps, the probe at pos + 1 being M(pos + 1):
  *            a literal is emitted when M(pos + 1) > best + 1 (orot's rule).
+ * Cross-chunk dictionary: a chunk that is not the first of its buffer sees
+ * the up to 32 KiB before it (the "halo") as positions -H..-1 -- they are in
+ * the chains and matches may start there -- while tokens are made for the
+ * chunk's own positions only.  orot's parallel path starts every chunk
+ * empty (src/parallel/parallel_compressor.cpp:56-70).
  * ====================================================================== */
 
 typedef struct {
@@ -411,21 +416,27 @@ static const l6_cfg L6LEVELS[13] = {
 
 #define L6_HASH_BITS 12
 
-void oracle_l6_match(const uint8_t* src, size_t n, int level, uint32_t* mlen, uint32_t* mdist) {
+#define L6_HALO 32768
+
+/* src points at the halo's first byte; mlen / mdist get n entries, for
+ * positions halo .. halo + n - 1. */
+void oracle_l6_match_h(const uint8_t* src, size_t halo, size_t n, int level, uint32_t* mlen, uint32_t* mdist) {
     const l6_cfg* c = &L6LEVELS[clamp_level(level)];
+    const size_t nx = halo + n;
     uint32_t* head = (uint32_t*)calloc((size_t)1 << L6_HASH_BITS, sizeof(uint32_t));
-    uint32_t* C = (uint32_t*)calloc(n ? n : 1, sizeof(uint32_t));
-    for (size_t p = 0; p < n; ++p) {
+    uint32_t* C = (uint32_t*)calloc(nx ? nx : 1, sizeof(uint32_t));
+    for (size_t p = 0; p < nx; ++p) {
         C[p] = 0;
-        if (p + 4 > n) continue;
+        if (p + 4 > nx) continue;
         const uint32_t h = hash4(src + p, L6_HASH_BITS);
         C[p] = head[h];
         head[h] = (uint32_t)p + 1;
     }
-    for (size_t x = 0; x < n; ++x) {
+    for (size_t i = 0; i < n; ++i) {
+        const size_t x = halo + i;
         uint32_t best = 0, bd = 0;
-        if (x + 4 <= n) {
-            const int cap = n - x < 258 ? (int)(n - x) : 258;
+        if (x + 4 <= nx) {
+            const int cap = nx - x < 258 ? (int)(nx - x) : 258;
             uint32_t cur = C[x];
             for (int k = 0; k < c->max_chain && cur; ++k) {
                 const uint32_t q = cur - 1;
@@ -441,29 +452,34 @@ void oracle_l6_match(const uint8_t* src, size_t n, int level, uint32_t* mlen, ui
                 cur = C[q];
             }
         }
-        mlen[x] = best >= 3 ? best : 0;
-        mdist[x] = best >= 3 ? bd : 0;
+        mlen[i] = best >= 3 ? best : 0;
+        mdist[i] = best >= 3 ? bd : 0;
     }
     free(C);
     free(head);
 }
 
-size_t oracle_l6_lz77(const uint8_t* src, size_t n, int level, uint32_t* tokens) {
+void oracle_l6_match(const uint8_t* src, size_t n, int level, uint32_t* mlen, uint32_t* mdist) {
+    oracle_l6_match_h(src, 0, n, level, mlen, mdist);
+}
+
+size_t oracle_l6_lz77_h(const uint8_t* src, size_t halo, size_t n, int level, uint32_t* tokens) {
     const l6_cfg* c = &L6LEVELS[clamp_level(level)];
     if (n == 0) return 0;
+    const uint8_t* s = src + halo;
     uint32_t* ml = (uint32_t*)calloc(2 * n, sizeof(uint32_t));
     uint32_t* md = ml + n;
-    oracle_l6_match(src, n, level, ml, md);
+    oracle_l6_match_h(src, halo, n, level, ml, md);
     size_t nt = 0, pos = 0;
     while (pos < n) {
         if (pos + 3 > n || ml[pos] < 3) {
-            tokens[nt++] = src[pos++];
+            tokens[nt++] = s[pos++];
             continue;
         }
         uint32_t b = ml[pos], d = md[pos];
         int k = c->lazy_depth;
         while (k > 0 && pos + b + 1 < n && pos + 1 + 3 <= n && ml[pos + 1] > b + 1) {
-            tokens[nt++] = src[pos++];
+            tokens[nt++] = s[pos++];
             b = ml[pos];
             d = md[pos];
             --k;
@@ -473,6 +489,10 @@ size_t oracle_l6_lz77(const uint8_t* src, size_t n, int level, uint32_t* tokens)
     }
     free(ml);
     return nt;
+}
+
+size_t oracle_l6_lz77(const uint8_t* src, size_t n, int level, uint32_t* tokens) {
+    return oracle_l6_lz77_h(src, 0, n, level, tokens);
 }
 
 /* ======================================================================
@@ -812,6 +832,7 @@ void oracle_encode_blocks_dyn(const uint32_t* tokens, size_t ntok, const uint8_t
 typedef struct {
     const uint8_t* src;
     size_t len;
+    size_t halo;  /* L6 class: bytes before src that are the chunk's dictionary */
     uint32_t* tokens;
     size_t ntok;
     int level;
@@ -833,7 +854,7 @@ static void* worker(void* arg) {
         pthread_mutex_unlock(&q->mu);
         if (i >= q->njobs) break;
         chunk_job* j = &q->jobs[i];
-        j->ntok = j->l6gpu ? oracle_l6_lz77(j->src, j->len, j->level, j->tokens)
+        j->ntok = j->l6gpu ? oracle_l6_lz77_h(j->src - j->halo, j->halo, j->len, j->level, j->tokens)
                            : oracle_lz77(j->src, j->len, j->level, j->tokens);
     }
     return NULL;
@@ -872,6 +893,7 @@ static size_t parallel_compress_impl(int level, int format, int threads, size_t 
         jobs[i].len = n ? (n - i * block_size < block_size ? n - i * block_size : block_size) : 0;
         jobs[i].level = level;
         jobs[i].l6gpu = l6gpu;
+        jobs[i].halo = l6gpu ? (i * block_size < L6_HALO ? i * block_size : L6_HALO) : 0;
         jobs[i].tokens = (uint32_t*)calloc(jobs[i].len ? jobs[i].len : 1, sizeof(uint32_t));
         if (!jobs[i].tokens) ok = 0;
     }

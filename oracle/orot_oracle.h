@@ -58,10 +58,16 @@ size_t oracle_parallel_compress(int level, int format, int threads, size_t block
  * match length / distance per position, and the lazy token stream. */
 void oracle_l6_match(const uint8_t* src, size_t n, int level, uint32_t* mlen, uint32_t* mdist);
 size_t oracle_l6_lz77(const uint8_t* src, size_t n, int level, uint32_t* tokens);
+/* The same with a cross-chunk dictionary: src points at the halo bytes that
+ * precede the chunk (halo <= 32768), the chunk follows them; results are for
+ * the chunk's n positions. */
+void oracle_l6_match_h(const uint8_t* src, size_t halo, size_t n, int level, uint32_t* mlen, uint32_t* mdist);
+size_t oracle_l6_lz77_h(const uint8_t* src, size_t halo, size_t n, int level, uint32_t* tokens);
 
 /* What the HIP path computes: levels 0..3 as oracle_parallel_compress
  * (orot's bytes); levels 4..12 with the L6-class matcher above, orot's chunk
- * sizes (src/parallel/parallel_compressor.cpp:16-20), and
+ * sizes (src/parallel/parallel_compressor.cpp:16-20), each chunk after the
+ * first with the up to 32 KiB before it as dictionary, and
  * oracle_encode_blocks_dyn. */
 size_t oracle_gpu_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src,
                                     size_t n, uint8_t* dst, size_t cap);

@@ -121,6 +121,7 @@ hipError_t launch_parallel_compress(const Plan& plan);
 // L6-class path (levels 4..12): LZ77 + block analysis, then emission.
 L6Params l6_level_params(int level);
 uint32_t l6_max_blocks(uint32_t chunk);
+uint64_t l6_position_stride(uint32_t chunk);  // per-chunk stride of d_tok / d_pd / d_aux (chunk + halo)
 uint32_t l6_max_chunk();
 size_t l6_block_record_bytes();
 hipError_t launch_l6_front(const Plan& plan);

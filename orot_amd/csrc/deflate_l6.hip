@@ -52,6 +52,17 @@ constexpr uint32_t L6R_BM_WORDS = L6_MAX_CHUNK / 32;
 constexpr uint32_t L6R_LDS = (L6R_BM_WORDS + 3 * L6R_THREADS + 64) * 4;
 
 constexpr uint32_t BLK_SYMS = 16384;
+
+// Cross-chunk dictionary: a chunk that is not the first of its buffer sees up
+// to L6_HALO bytes before it as positions -H..-1 (in its chains and as match
+// sources); its tokens cover its own bytes only.  Per-position arrays of the
+// L6 class are laid out with room for the halo.
+constexpr uint32_t L6_HALO = 32768;
+__host__ __device__ constexpr uint64_t l6_stride(uint32_t chunk) { return ((uint64_t)chunk + L6_HALO + 7u) & ~7ull; }
+__device__ __forceinline__ uint32_t l6_halo(uint32_t c, const ChunkDesc* desc, const BufDesc* bufs) {
+    const uint64_t rel = desc[c].in_off - bufs[desc[c].buf & ~CHUNK_LAST].in_off;
+    return (uint32_t)min<uint64_t>(rel, L6_HALO);
+}
 constexpr int L6B_THREADS = 256;
 
 
@@ -63,11 +74,14 @@ __device__ __forceinline__ uint32_t ml_dist(uint32_t m) { return m >> 16; }
 // ---------------------------------------------------------------------------
 template <bool EXCH>
 __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
-                                             const ChunkDesc* __restrict__ desc, uint32_t chunk,
-                                             uint32_t* __restrict__ tokw, uint16_t* __restrict__ pd) {
-    const uint32_t n = desc[c].len;
-    const uint8_t* s = src + desc[c].in_off;
-    const uint64_t cp = (chunk + 7u) & ~7u;
+                                             const ChunkDesc* __restrict__ desc, const BufDesc* __restrict__ bufs,
+                                             uint32_t chunk, uint32_t* __restrict__ tokw,
+                                             uint16_t* __restrict__ pd) {
+    // positions 0 .. n-1 are the halo (H bytes) followed by the chunk
+    const uint32_t H = l6_halo(c, desc, bufs);
+    const uint32_t n = desc[c].len + H;
+    const uint8_t* s = src + desc[c].in_off - H;
+    const uint64_t cp = l6_stride(chunk);
     uint32_t* T = tokw + (uint64_t)c * cp;
     uint16_t* PD = pd + (uint64_t)c * cp;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -144,6 +158,7 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const u
 template <bool EXCH>
 __global__ __launch_bounds__(L6P_THREADS) void l6_pred_kernel(const uint8_t* __restrict__ src,
                                                               const ChunkDesc* __restrict__ desc,
+                                                              const BufDesc* __restrict__ bufs,
                                                               const uint32_t* __restrict__ order,
                                                               BatchHdr* __restrict__ hdr, uint32_t chunk,
                                                               uint32_t* __restrict__ tokw, uint16_t* __restrict__ pd) {
@@ -151,7 +166,7 @@ __global__ __launch_bounds__(L6P_THREADS) void l6_pred_kernel(const uint8_t* __r
     for (;;) {
         const uint32_t k = next_item(&hdr->queue[Q_PRED], hdr->nchunks, smem);
         if (k == NO_ITEM) break;
-        l6_pred_body<EXCH>(order[k], smem, src, desc, chunk, tokw, pd);
+        l6_pred_body<EXCH>(order[k], smem, src, desc, bufs, chunk, tokw, pd);
         __syncthreads();
     }
 }
@@ -175,14 +190,17 @@ __device__ __forceinline__ uint64_t ring_u64(const uint32_t* R, uint32_t a) {
 }
 
 __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
-                                              const ChunkDesc* __restrict__ desc, uint32_t chunk,
-                                              const uint16_t* __restrict__ pd, uint32_t* __restrict__ mlw,
-                                              const L6Params prm) {
+                                              const ChunkDesc* __restrict__ desc, const BufDesc* __restrict__ bufs,
+                                              uint32_t chunk, const uint16_t* __restrict__ pd,
+                                              uint32_t* __restrict__ mlw, const L6Params prm) {
     uint32_t* DR = smem;                                            // byte ring
     uint16_t* PR = reinterpret_cast<uint16_t*>(smem + L6M_DRING / 4);  // chain-step ring
-    const uint32_t n = desc[c].len;
-    const uint8_t* s = src + desc[c].in_off;
-    const uint64_t cp = (chunk + 7u) & ~7u;
+    // positions 0 .. n-1 are the halo (H bytes) followed by the chunk; M is
+    // computed for the chunk's positions, ML[x - H]
+    const uint32_t H = l6_halo(c, desc, bufs);
+    const uint32_t n = desc[c].len + H;
+    const uint8_t* s = src + desc[c].in_off - H;
+    const uint64_t cp = l6_stride(chunk);
     const uint16_t* PD = pd + (uint64_t)c * cp;
     uint32_t* ML = mlw + (uint64_t)c * cp;
     const int tid = threadIdx.x;
@@ -206,6 +224,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
         dfill = max(dfill, dneed);
         for (uint32_t q = x0 + tid; q < xe; q += L6M_THREADS) PR[q % L6M_PRING] = PD[q];
         __syncthreads();
+        if (xe <= H) continue;  // a tile wholly in the halo only fills the rings
         // Two positions per thread, x and x + L6M_THREADS, walk their chains in
         // step so each step has two independent LDS loads in flight.
         for (uint32_t xp = x0 + tid; xp < xe; xp += 2 * L6M_THREADS) {
@@ -223,7 +242,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                 m[i].bd = 0;
                 m[i].k = 0;
                 m[i].cur = x;
-                m[i].live = x < xe && x + 4 <= n;
+                m[i].live = x < xe && x >= H && x + 4 <= n;
                 m[i].cap = m[i].live ? min(258u, n - x) : 0u;
                 // bytes x .. x+15 (beyond the chunk end they are stale or zero,
                 // which only lengthens a comparison that min(l, cap) cuts back)
@@ -285,7 +304,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
             }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                if (m[i].x < xe) ML[m[i].x] = m[i].best >= 3 ? (m[i].best | (m[i].bd << 16)) : 0u;
+                if (m[i].x < xe && m[i].x >= H) ML[m[i].x - H] = m[i].best >= 3 ? (m[i].best | (m[i].bd << 16)) : 0u;
         }
         __syncthreads();
     }
@@ -293,6 +312,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
 
 __global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __restrict__ src,
                                                                const ChunkDesc* __restrict__ desc,
+                                                               const BufDesc* __restrict__ bufs,
                                                                const uint32_t* __restrict__ order,
                                                                BatchHdr* __restrict__ hdr, uint32_t chunk,
                                                                const uint16_t* __restrict__ pd,
@@ -301,7 +321,7 @@ __global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __
     for (;;) {
         const uint32_t k = next_item(&hdr->queue[Q_MATCH], hdr->nchunks, smem);
         if (k == NO_ITEM) break;
-        l6_match_body(order[k], smem, src, desc, chunk, pd, mlw, prm);
+        l6_match_body(order[k], smem, src, desc, bufs, chunk, pd, mlw, prm);
         __syncthreads();
     }
 }
@@ -343,7 +363,7 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
     uint32_t* misc = rcnt + L6R_THREADS;
     const uint32_t n = desc[c].len;
     const uint8_t* s = src + desc[c].in_off;
-    const uint64_t cp = (chunk + 7u) & ~7u;
+    const uint64_t cp = l6_stride(chunk);
     const uint32_t* ML = mlw + (uint64_t)c * cp;
     uint16_t* CODE = codew + (uint64_t)c * cp;
     uint32_t* TK = toks + (uint64_t)c * cp;
@@ -608,7 +628,7 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
     const uint32_t ntok = ntok_in[c];
     const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
     if (b >= nblk) return;
-    const uint64_t cp = (chunk + 7u) & ~7u;
+    const uint64_t cp = l6_stride(chunk);
     const uint32_t* TK = toks + (uint64_t)c * cp;
     const uint32_t t0 = b * BLK_SYMS, t1 = min(ntok, t0 + BLK_SYMS);
     const int tid = threadIdx.x;
@@ -830,7 +850,7 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const BlockRec* RR = recs + (uint64_t)c * maxblk;
     const BlockRec& R = RR[b];
-    const uint64_t cp = (chunk + 7u) & ~7u;
+    const uint64_t cp = l6_stride(chunk);
     const uint32_t* TK = toks + (uint64_t)c * cp;
     const uint8_t* s = src + desc[c].in_off;
     uint32_t* stg = stage + (uint64_t)c * stage_words;
@@ -1039,7 +1059,7 @@ template <bool EXCH>
 hipError_t launch_pred(const Plan& p) {
     const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6P_LDS, L6P_THREADS));
     hipLaunchKernelGGL((l6_pred_kernel<EXCH>), dim3(g), dim3(L6P_THREADS), L6P_LDS, p.stream, p.d_in, p.d_desc,
-                       p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_pd);
+                       p.d_bufs, p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_pd);
     return hipSuccess;
 }
 
@@ -1064,6 +1084,7 @@ L6Params l6_level_params(int level) {
     return t[level < 0 ? 0 : (level > 12 ? 12 : level)];
 }
 uint32_t l6_max_blocks(uint32_t chunk) { return (chunk + BLK_SYMS - 1) / BLK_SYMS; }
+uint64_t l6_position_stride(uint32_t chunk) { return l6_stride(chunk); }
 uint32_t l6_max_chunk() { return L6_MAX_CHUNK; }
 size_t l6_block_record_bytes() { return sizeof(BlockRec); }
 
@@ -1073,8 +1094,8 @@ hipError_t launch_l6_front(const Plan& p) {
     hipError_t e = p.exch_ok ? launch_pred<true>(p) : launch_pred<false>(p);
     if (e != hipSuccess) return e;
     const unsigned gm = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6M_LDS, L6M_THREADS));
-    hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.d_desc, p.d_order,
-                       p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6);
+    hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.d_desc, p.d_bufs,
+                       p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6);
     const unsigned gr = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6R_LDS, L6R_THREADS));
     hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), L6R_LDS, p.stream, p.d_in, p.d_desc, p.d_order,
                        p.d_hdr, p.chunk, p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.l6);
