@@ -38,7 +38,10 @@ SEEDS = {"zeros": 0, "text": 0x5EED0001, "random": 0x5EED0002, "records": 0x5EED
 HBM_PEAK = 8.0e12
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the dominant kernel on the
 # default workload, scaled by the same counters on a plain copy kernel.
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "l1_barrier_kernel_256MiB_pmc.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "l1_lz_stage_256MiB_pmc.txt")
+# the L1-class LZ stage at 12-bit hashes and chunks up to 128 KiB: candidate
+# scan, then match / parse / emission
+L1_STAGE = "l1_candidate_kernel + l1_barrier_kernel"
 
 
 def measured_traffic(path: str = TRAFFIC_SUMMARY):
@@ -150,11 +153,12 @@ class OrotCPU:
             self.ctx = None
 
 
-def cpu_baseline(data: np.ndarray, budget_s: float = 15.0):
-    """orot parallel L1, raw, 128 KiB chunks on the host cores: median of up to
-    10 runs over the whole buffer."""
+def cpu_baseline(data: np.ndarray, level: int = 1, fmt: int = 0, chunk: int = 131072, sample: str = "",
+                 budget_s: float = 15.0):
+    """orot's parallel compressor on the host cores: median of up to 10 runs
+    over the whole buffer after one warm-up."""
     threads = cpu_threads()
-    cpu = OrotCPU(1, 0, 131072, threads)
+    cpu = OrotCPU(level, fmt, chunk, threads)
     out = np.empty(data.size + data.size // 8 + 4096, dtype=np.uint8)
     cpu.compress(data, out)  # warm-up
     times = []
@@ -170,8 +174,7 @@ def cpu_baseline(data: np.ndarray, budget_s: float = 15.0):
         "unit": "GB/s",
         "cores": threads,
         "kind": cpu.kind,
-        "sample": f"orot parallel L1 raw, 128 KiB chunks, the full {data.size >> 20} MiB 4-segment buffer, "
-                  f"median of {len(times)} runs after 1 warm-up ({clen} B out)",
+        "sample": f"{sample}, median of {len(times)} runs after 1 warm-up ({clen} B out)",
     }
 
 
@@ -229,15 +232,15 @@ def base_record(args, value, ms_per_step, config):
     }
 
 
-def bench_l1(args) -> dict:
-    """Default: one 256 MiB buffer (4 x 64 MiB segments), level 1, raw,
-    128 KiB chunks; a step is one deflate_parallel_compress_device call."""
-    data = workload()
+def bench_single(args, data: np.ndarray, level: int, fmt: int, chunk: int, config: dict, golden: dict,
+                 cpu_sample: str, traffic) -> dict:
+    """One buffer resident in HBM; a step is one
+    deflate_parallel_compress_device call over it."""
     n = data.size
     d_in = orot_amd.DeviceBuffer.from_host(data)
-    cap = orot_amd.compress_bound(n, orot_amd.Format.RAW) + (1 << 20)
+    cap = orot_amd.compress_bound(n, fmt) + (1 << 20)
     d_out = orot_amd.DeviceBuffer(cap)
-    pc = orot_amd.ParallelCompressor(1, orot_amd.Format.RAW, 0, 131072)
+    pc = orot_amd.ParallelCompressor(level, fmt, 0, chunk)
     pc.set_timing(True)
 
     # correctness (untimed): zlib inflate and orot's bytes
@@ -245,13 +248,8 @@ def bench_l1(args) -> dict:
     if clen == 0:
         raise SystemExit(f"compress failed: {orot_amd.last_error()}")
     comp = d_out.download(clen).tobytes()
-    inflated_ok = zlib.decompress(comp, -15) == data.tobytes()
+    inflated_ok = zlib.decompress(comp, {0: -15, 1: 15, 2: 31}[fmt]) == data.tobytes()
     digest = hashlib.sha256(comp).hexdigest()
-    golden = None
-    gpath = os.path.join(ROOT, "tests", "golden", "bench_digest.json")
-    if os.path.exists(gpath):
-        with open(gpath) as f:
-            golden = json.load(f)
     matches_orot = golden is not None and golden.get("sha256") == digest and golden.get("size") == clen
     del comp
 
@@ -273,14 +271,7 @@ def bench_l1(args) -> dict:
     b_alg = n + clen
     achieved = b_alg / t_kernel / 1e9
 
-    rec = base_record(args, n * args.steps / total / 1e9, total / args.steps * 1e3, {
-        "workload": "plain GPU path, L1 greedy, raw DEFLATE, 256 MiB = 4 x 64 MiB segments "
-                    "(zeros, Zipf text, random, 64-byte records), 128 KiB chunks",
-        "level": 1,
-        "format": "raw",
-        "chunk": 131072,
-        "input_bytes": n,
-    })
+    rec = base_record(args, n * args.steps / total / 1e9, total / args.steps * 1e3, config)
     rec.update({
         "p50_ms": round(pct(lat_ms, 0.5), 3),
         "p99_ms": round(pct(lat_ms, 0.99), 3),
@@ -291,20 +282,60 @@ def bench_l1(args) -> dict:
         "stage_ms_last": [round(x, 4) for x in pc.stage_times()],
         "roofline": {
             "bound": "hbm",
-            "kernel": "l1_barrier_kernel",
+            "kernel": L1_STAGE,
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved * 1e9 / HBM_PEAK, 5),
-            "traffic": measured_traffic(),
+            "traffic": traffic,
         },
     })
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(data)
+        rec["cpu_baseline"] = cpu_baseline(data, level, fmt, chunk, cpu_sample)
     pc.close()
     d_in.free()
     d_out.free()
     return rec
+
+
+def bench_l1(args) -> dict:
+    """Default: one 256 MiB buffer (4 x 64 MiB segments), level 1, raw,
+    128 KiB chunks."""
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "bench_digest.json")
+    if os.path.exists(gpath):
+        with open(gpath) as f:
+            golden = json.load(f)
+    data = workload()
+    rec = bench_single(args, data, 1, 0, 131072, {
+        "workload": "plain GPU path, L1 greedy, raw DEFLATE, 256 MiB = 4 x 64 MiB segments "
+                    "(zeros, Zipf text, random, 64-byte records), 128 KiB chunks",
+        "level": 1,
+        "format": "raw",
+        "chunk": 131072,
+        "input_bytes": data.size,
+    }, golden, f"orot parallel L1 raw, 128 KiB chunks, the full {data.size >> 20} MiB 4-segment buffer",
+        measured_traffic())
+    return rec
+
+
+def bench_text16(args) -> dict:
+    """orot's own CPU configuration: one 16 MiB Zipf-text buffer, level 1,
+    gzip, 128 KiB chunks (orot's bytes for it are a golden fixture)."""
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "orot_fixtures.json")
+    if os.path.exists(gpath):
+        with open(gpath) as f:
+            golden = json.load(f).get("text16_l1_gzip")
+    data = corpus.generate("text", 16 << 20, 0xC0F1)
+    return bench_single(args, data, 1, 2, 131072, {
+        "workload": "orot's CPU configuration on the GPU: one 16 MiB Zipf-word text buffer, L1 greedy, gzip, "
+                    "128 KiB chunks",
+        "level": 1,
+        "format": "gzip",
+        "chunk": 131072,
+        "input_bytes": data.size,
+    }, golden, "orot parallel L1 gzip, 128 KiB chunks, the same 16 MiB text buffer", None)
 
 
 # Batch workload: 4096 buffers, sizes log-uniform in [4 KiB, 4 MiB], content
@@ -408,7 +439,7 @@ def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds) -> 
         stages = bt.stage_times()
     front = stages[1] * 1e-3
     b_alg = n + clen
-    kernel = "l1_barrier_kernel" if level <= 3 else "l6 front (pred + match + parse + block analysis)"
+    kernel = L1_STAGE if level <= 3 else "l6 front (pred + match + parse + block analysis)"
     if level <= 3 and tuned["block_size"] > 131072:
         kernel = "l1_lz77_kernel"
     if level <= 3 and tuned["hash_bits"] == 14:
@@ -621,8 +652,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
-    ap.add_argument("--workload", choices=("l1", "l6", "batch", "serve"), default="l1",
+    ap.add_argument("--workload", choices=("l1", "text16", "l6", "batch", "serve"), default="l1",
                     help="l1: 256 MiB four-segment buffer at level 1 (default); "
+                         "text16: 16 MiB Zipf text at level 1, gzip (orot's own CPU configuration); "
                          "l6: 256 x 8 MiB text/records buffers at level 6, gzip, batched; "
                          "batch: 4096 mixed buffers of 4 KiB - 4 MiB at level 1, zlib, batched; "
                          "serve: Poisson request stream through the continuous-batching server")
@@ -636,7 +668,8 @@ def main() -> int:
         print("bench.py measures one GPU only: every box here holds a single MI355X; "
               "batches shard across GPUs with no exchange (see the design notes).", file=sys.stderr)
         return 2
-    rec = {"l1": bench_l1, "l6": bench_l6, "batch": bench_batch, "serve": bench_serve}[args.workload](args)
+    rec = {"l1": bench_l1, "text16": bench_text16, "l6": bench_l6, "batch": bench_batch,
+           "serve": bench_serve}[args.workload](args)
     try:
         rec["device"] = orot_amd.device_info()[0]
     except Exception:

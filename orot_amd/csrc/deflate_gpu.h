@@ -112,6 +112,10 @@ hipError_t check_exchange_order(hipStream_t stream, bool* ok);
 // Largest chunk the L1-class kernels accept for the given hash width.
 uint32_t max_chunk_for_hash_bits(int hash_bits);
 
+// Per-chunk stride (32-bit words) of the L1-class token words: the chunk
+// plus one partial range of the interleaved layout.
+inline uint64_t l1_position_stride(uint32_t chunk) { return (((uint64_t)chunk + 7) & ~7ull) + 256; }
+
 // Stage capacity (32-bit words) needed per chunk of `chunk` bytes.
 uint64_t stage_words_for_chunk(uint32_t chunk);
 

@@ -1055,13 +1055,19 @@ __global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __r
 }
 
 // ---------------------------------------------------------------------------
-// L1-class kernel, 12-bit hashes, chunks up to 128 KiB (orot's default L1
-// chunk): one 1024-thread workgroup per chunk.
-//   0   the chunk's bytes are staged in LDS.
-//   A1  wave 0: latest earlier same-hash position, one LDS exchange per
+// L1-class path, 12-bit hashes, chunks up to 128 KiB (orot's default L1
+// chunk): two launches per batch.
+//   l1_candidate_kernel, one 128-thread workgroup per chunk, 40 KiB of LDS
+//   (four chunks resident per CU):
+//   A1  each wave scans one half of the chunk's hash positions in order: the
+//       latest earlier same-hash position of its half, one LDS exchange per
 //       position (EXCH: same-address exchanges of one wave instruction take
 //       effect in lane order, checked at start-up by exchange_order_kernel)
-//       or a ballot match-any; T[p] = C + 1.
+//       or a ballot match-any; T[p] = C + 1, 0 if none.  A position of the
+//       second half with no earlier same-hash position in that half takes the
+//       last one of the first half (the first half's final table).
+//   l1_barrier_kernel, one 1024-thread workgroup per chunk:
+//   0   the chunk's bytes are staged in LDS.
 //   A2  per position: candidate distance (C == 0 empty, 15-bit masked,
 //       first byte compared at C: lz77.cpp:178-184) and greedy match length
 //       (lz77.cpp:163-191), the verified extent carried along a diagonal,
@@ -1076,16 +1082,167 @@ __global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __r
 //   C   block-wide scan, stored / fixed choice (deflate_block.cpp:414-421),
 //       emission into the chunk's staging bitstream.
 // ---------------------------------------------------------------------------
-static_assert(LC_MAX_CHUNK / LC_THREADS <= 128, "a range spans at most four bitmap words");
-constexpr uint32_t LB_BM_OFF = LC_REGION0 / 4;               // words: table 0, then the token bitmap
-constexpr uint32_t LB_T1_OFF = LB_BM_OFF + LC_BM_WORDS;       // table 1 (two sub-ranges), then fb / misc
-constexpr uint32_t LB_FB_OFF = LB_T1_OFF;                     // first barrier per range
+static_assert(LC_MAX_CHUNK / LC_THREADS <= 128, "ranges of at most 136 positions span at most five bitmap words");
+constexpr uint32_t LB_BM_OFF = LC_REGION0 / 4;               // words: the token bitmap
+constexpr uint32_t LB_FB_OFF = LB_BM_OFF + LC_BM_WORDS;       // first barrier per range
 constexpr uint32_t LB_MISC_OFF = LB_FB_OFF + LC_THREADS;
-template <int W>
-struct LBCfg {
-    static constexpr uint32_t LDS_BYTES = W == 2 ? (LB_T1_OFF + 4096u) * 4u : (LB_MISC_OFF + 64u) * 4u;
+constexpr uint32_t LB_LDS_BYTES = (LB_MISC_OFF + 64u) * 4u;
+
+constexpr int LA_THREADS = 128;  // candidate kernel: one wave per half chunk
+constexpr uint32_t LA_TAB = 1u << 12;
+constexpr uint16_t LA_NONE = 0xFFFFu;
+
+// Ranges and token-word layout of the barrier path.  The chunk is cut into R
+// ranges of S positions, one per thread of l1_barrier_kernel.  S is 8 times
+// an odd number (136 for 128 KiB chunks): the lanes of a wave then start
+// their ranges on LDS words 2 * odd apart, which spreads them over the banks
+// in every phase that walks the ranges in order, and each range is whole
+// groups of 8 positions starting on a byte of its bitmap word.  Group g of
+// range t is stored at (g * R + t) * 8, so those phases read 64 consecutive
+// groups per wave instruction.  t = p / S is umulhi(p, ceil(2^32 / S)),
+// exact for every p < 2^17 + 256 and S <= 136.
+struct LbLayout {
+    uint32_t S, R, m;
+    __device__ explicit LbLayout(uint32_t n)
+        : S(8u * ((((n + LC_THREADS - 1) / LC_THREADS + 7u) >> 3) | 1u)),
+          R((n + S - 1) / S),
+          m(0xFFFFFFFFu / S + 1u) {}
+    __device__ uint32_t range_of(uint32_t p) const { return __umulhi(p, m); }
+    __device__ uint32_t index(uint32_t p) const {
+        const uint32_t t = range_of(p), q = p - t * S;
+        return (((q >> 3) * R + t) << 3) | (q & 7u);
+    }
+    // first word of group g of range t
+    __device__ uint32_t group(uint32_t t, uint32_t g) const { return (g * R + t) << 3; }
 };
-static_assert(LB_MISC_OFF + 64 <= LB_T1_OFF + 4096, "fb / misc must fit under table 1");
+constexpr uint32_t lb_tok_stride(uint32_t chunk) { return ((chunk + 7u) & ~7u) + 256u; }
+
+template <bool EXCH>
+__global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t* __restrict__ src,
+    const ChunkDesc* __restrict__ desc, const BatchHdr* __restrict__ hdr, uint32_t chunk,
+    uint32_t* __restrict__ tokw) {
+    constexpr int HB = 12;
+    __shared__ uint32_t tab[2 * LA_TAB];  // per half: latest position + 1 of each hash
+    __shared__ uint16_t first1[LA_TAB];   // second half: offset of each hash's first position
+    __shared__ uint32_t sink[LA_THREADS];  // exchange target of lanes that store nothing
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t nchunks = hdr->nchunks;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint32_t n = desc[c].len;
+        const uint8_t* s = src + desc[c].in_off;
+        uint32_t* T = tokw + (uint64_t)c * lb_tok_stride(chunk);
+        const LbLayout L(n);
+        const uint32_t nh = n >= 4 ? n - 3 : 0;
+        // halves of 64-position groups: the second starts at SR <= 65536, so
+        // its offsets fit 16 bits
+        const uint32_t SR = (((nh + 1) / 2) + 63u) & ~63u;
+        for (uint32_t i = tid; i < 2 * LA_TAB; i += LA_THREADS) tab[i] = 0;
+        for (uint32_t i = tid; i < LA_TAB; i += LA_THREADS) first1[i] = LA_NONE;
+        __syncthreads();
+        uint32_t* my = tab + (uint32_t)wv * LA_TAB;
+        const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
+        // the 4-byte windows of 512 positions come from global memory; three
+        // groups of loads (clamped into [a, b)) stay in flight ahead of the
+        // exchanges
+        auto load8 = [&](uint32_t q0, uint32_t* w) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) w[j] = ld32u(s + min(q0 + 64u * j + (uint32_t)lane, b - 1));
+        };
+        auto step = [&](uint32_t p0, const uint32_t* w) {
+            uint32_t h[8], res[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) h[j] = hash4<HB>(w[j]);
+            if (EXCH) {
+                // A run of lanes with equal hashes makes one exchange: its
+                // first lane stores the run's last position, the others take
+                // the position just before them.  The hash of the lane below
+                // comes from a DPP wave shift (lane 0 gets an impossible
+                // hash), and every lane exchanges, the ones that store
+                // nothing on a word of their own, so the eight exchanges of
+                // a step are issued back to back.
+                uint32_t* addr[8];
+                uint32_t val[8];
+                bool same[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                    const bool valid = p < b;
+                    const uint32_t hp =
+                        (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)h[j], 0x138, 0xF, 0xF, false);
+                    same[j] = valid && hp == h[j];
+                    const uint64_t E = ballot(same[j]);
+                    const uint32_t extra = (uint32_t)__builtin_ctzll(~((E >> lane) >> 1));
+                    const bool head = valid && !same[j];
+                    addr[j] = head ? &my[h[j]] : &sink[tid];
+                    val[j] = p + extra + 1u;
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) res[j] = atomicExch(addr[j], val[j]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                    if (same[j]) res[j] = p;
+                    if (p >= b) res[j] = 0;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const uint32_t q0 = p0 + 64u * j;
+                    const uint32_t p = q0 + (uint32_t)lane;
+                    const bool valid = p < b;
+                    uint64_t m = ballot(valid);
+#pragma unroll
+                    for (int k = 0; k < HB; ++k) {
+                        const uint32_t bit = (h[j] >> k) & 1u;
+                        const uint64_t bk = ballot(bit != 0);
+                        m &= bk ^ ((uint64_t)bit - 1ull);
+                    }
+                    res[j] = 0;
+                    if (valid) {
+                        const uint64_t below = m & ((1ull << lane) - 1ull);
+                        if (below) res[j] = q0 + (uint32_t)(63 - __builtin_clzll(below)) + 1u;
+                        else res[j] = my[h[j]];
+                        if ((m >> lane) == 1ull) my[h[j]] = p + 1u;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t p = p0 + 64u * j + (uint32_t)lane;
+                if (p < b) {
+                    if (wv == 1 && res[j] == 0) first1[h[j]] = (uint16_t)(p - a);  // one per hash
+                    else T[L.index(p)] = res[j];
+                }
+            }
+        };
+        uint32_t w0[8], w1[8], w2[8], w3[8];
+        if (a < b) {
+            load8(a, w0);
+            load8(a + 512u, w1);
+            load8(a + 1024u, w2);
+        }
+        for (uint32_t p0 = a; p0 < b; p0 += 2048u) {
+            load8(p0 + 1536u, w3);
+            step(p0, w0);
+            if (p0 + 512u >= b) break;
+            load8(p0 + 2048u, w0);
+            step(p0 + 512u, w1);
+            if (p0 + 1024u >= b) break;
+            load8(p0 + 2560u, w1);
+            step(p0 + 1024u, w2);
+            if (p0 + 1536u >= b) break;
+            load8(p0 + 3072u, w2);
+            step(p0 + 1536u, w3);
+        }
+        __syncthreads();
+        // first positions of the second half: the first half's latest position
+        for (uint32_t i = tid; i < LA_TAB; i += LA_THREADS) {
+            const uint32_t f = first1[i];
+            if (f != LA_NONE) T[L.index(SR + f)] = tab[i];
+        }
+        __syncthreads();
+    }
+}
 
 __device__ __forceinline__ uint32_t lds_u32(const uint32_t* L, uint32_t off) {
     const uint32_t i = off >> 2, k = off & 3u;
@@ -1099,25 +1256,23 @@ __device__ __forceinline__ uint64_t lds_u64(const uint32_t* L, uint32_t off) {
 __device__ __forceinline__ uint32_t lds_u8(const uint32_t* L, uint32_t off) {
     return (L[off >> 2] >> (8 * (off & 3u))) & 0xFFu;
 }
-__device__ __forceinline__ uint32_t code_step(uint32_t b, const uint32_t* T, uint32_t pos) {
+__device__ __forceinline__ uint32_t code_step(uint32_t b, const uint32_t* T, const LbLayout& L, uint32_t pos) {
     if (b == 0) return 1;
     if (b < 254) return b + 2;
     if (b == 254) return 258;
-    return tok_len(T[pos]);
+    return tok_len(T[L.index(pos)]);
 }
 
 // Fixed-Huffman bit count of the tokens whose start bit is set in
 // bm over [a, b); T read 8 tokens at a time.
 
-template <int W, bool EXCH>
 __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, uint32_t chunk, uint32_t* __restrict__ tokw,
     uint32_t* __restrict__ stage, uint64_t stage_words, ChunkInfo* __restrict__ info) {
-    constexpr int HB = 12;
-    constexpr uint32_t TAB = 1u << HB;
     const uint32_t n = desc[c].len;
     const uint8_t* s = src + desc[c].in_off;
-    uint32_t* T = tokw + (uint64_t)c * ((chunk + 7u) & ~7u);
+    uint32_t* T = tokw + (uint64_t)c * lb_tok_stride(chunk);  // C + 1 from l1_candidate_kernel, LbLayout order
+    const LbLayout L(n);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     uint32_t* D = smem;                         // chunk bytes, then length codes
     uint8_t* code = reinterpret_cast<uint8_t*>(smem);
@@ -1125,7 +1280,8 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     uint32_t* fb = smem + LB_FB_OFF;
     uint32_t* misc = smem + LB_MISC_OFF;
 
-    // ---- 0: stage the chunk in LDS (plus zeroed padding words), clear tables
+    // ---- 0: stage the chunk in LDS, plus zeroed padding words (which may
+    // reach into the bitmap; K clears it)
     for (uint32_t i = (uint32_t)tid * 16u; i < n; i += LC_THREADS * 16u) {
         if (i + 16 <= n) {
             const uint64_t a = ld64u(s + i), b = ld64u(s + i + 8);
@@ -1142,80 +1298,32 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             }
         }
     }
-    for (uint32_t i = tid; i < W * TAB; i += LC_THREADS) smem[LB_BM_OFF + i] = 0;
-    __syncthreads();
-    if (tid < 4) D[((n + 3) >> 2) + tid] = 0;  // after the table clear: may overlap table 0
+    if (tid < 4) D[((n + 3) >> 2) + tid] = 0;  // past the staged words
     __syncthreads();
 
-    // ---- A1 (waves 0..W-1, one sub-range each): T[p] = C + 1 for the latest
-    // earlier position C of the sub-range with the same hash, 0 if none.
+    // ---- A2: candidate distance (C == 0 empty, 15-bit masked, first byte
+    // compared at C: lz77.cpp:178-184) and greedy match length
+    // (lz77.cpp:163-191).
     const uint32_t nh = n >= 4 ? n - 3 : 0;
-    const uint32_t SR = W == 1 ? nh : ((((nh + 1) / 2) + 63u) & ~63u);
-    if (wv < W) {
-        uint32_t* tab = smem + LB_BM_OFF + (uint32_t)wv * TAB;
-        const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
-        for (uint32_t p0 = a; p0 < b; p0 += 512) {
-            uint32_t wd[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) wd[j] = lds_u32(D, min(p0 + 64u * j + (uint32_t)lane, b - 1));
-            uint32_t res[8];
-            if (EXCH) {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const uint32_t p = p0 + 64u * j + (uint32_t)lane;
-                    res[j] = p < b ? atomicExch(&tab[hash4<HB>(wd[j])], p + 1u) : 0u;
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const uint32_t q0 = p0 + 64u * j;
-                    const uint32_t p = q0 + (uint32_t)lane;
-                    const bool valid = p < b;
-                    const uint32_t h = hash4<HB>(wd[j]);
-                    uint64_t m = ballot(valid);
-#pragma unroll
-                    for (int k = 0; k < HB; ++k) {
-                        const uint32_t bit = (h >> k) & 1u;
-                        const uint64_t bk = ballot(bit != 0);
-                        m &= bk ^ ((uint64_t)bit - 1ull);
-                    }
-                    res[j] = 0;
-                    if (valid) {
-                        const uint64_t below = m & ((1ull << lane) - 1ull);
-                        if (below) res[j] = q0 + (uint32_t)(63 - __builtin_clzll(below)) + 1u;
-                        else res[j] = tab[h];
-                        if ((m >> lane) == 1ull) tab[h] = p + 1u;
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t p = p0 + 64u * j + (uint32_t)lane;
-                if (p < b) T[p] = res[j];
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- A2: candidate distance (first occurrences of sub-range 1 resolved
-    // against table 0; C == 0 empty, 15-bit masked, first byte compared at C:
-    // lz77.cpp:178-184) and greedy match length (lz77.cpp:163-191).
-    const uint32_t S = max(32u, (((n + LC_THREADS - 1) / LC_THREADS) + 31u) & ~31u);
+    const uint32_t S = L.S;
     const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
+    // this range's group holding position q (r0 <= q < r1, q - r0 a multiple of 8)
+    auto tg = [&](uint32_t q) { return T + L.group((uint32_t)tid, (q - r0) >> 3); };
     {
         const uint32_t lim = min(n, r1 + 257u);
         uint32_t cd = 0, ce = 0;
         bool cx = false;
         uint32_t nx[8];
         auto load_loc = [&](uint32_t q0, uint32_t* dst) {
+            const uint32_t* g = tg(q0);
             if (q0 + 8 <= nh) {
-                const uint4 v0 = *reinterpret_cast<const uint4*>(T + q0);
-                const uint4 v1 = *reinterpret_cast<const uint4*>(T + q0 + 4);
+                const uint4 v0 = *reinterpret_cast<const uint4*>(g);
+                const uint4 v1 = *reinterpret_cast<const uint4*>(g + 4);
                 dst[0] = v0.x; dst[1] = v0.y; dst[2] = v0.z; dst[3] = v0.w;
                 dst[4] = v1.x; dst[5] = v1.y; dst[6] = v1.z; dst[7] = v1.w;
             } else {
 #pragma unroll
-                for (int k = 0; k < 8; ++k) dst[k] = (q0 + k < nh) ? T[q0 + k] : 0u;
+                for (int k = 0; k < 8; ++k) dst[k] = (q0 + k < nh) ? g[k] : 0u;
             }
         };
         if (r0 < r1) load_loc(r0, nx);
@@ -1230,9 +1338,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             for (int k = 0; k < 8; ++k) {
                 const uint32_t p = p0 + k;
                 const uint32_t lit = win_byte(w0, w1, k);
-                uint32_t C1 = loc[k];
-                if (W == 2 && C1 == 0 && p >= SR && p < nh)
-                    C1 = smem[LB_BM_OFF + hash4<HB>((uint32_t)(win_u64(w0, w1, k) & 0xFFFFFFFFull))];
+                const uint32_t C1 = loc[k];
                 uint32_t d = 0, m1 = 0;
                 if (p < r1 && C1 > 1) {
                     const uint32_t C = C1 - 1u;
@@ -1285,13 +1391,14 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 }
                 outw[k] = lit | (m1 << 8) | ((m1 ? d : 0u) << 17);
             }
+            uint32_t* g = tg(p0);
             if (p0 + 8 <= r1) {
-                *reinterpret_cast<uint4*>(T + p0) = make_uint4(outw[0], outw[1], outw[2], outw[3]);
-                *reinterpret_cast<uint4*>(T + p0 + 4) = make_uint4(outw[4], outw[5], outw[6], outw[7]);
+                *reinterpret_cast<uint4*>(g) = make_uint4(outw[0], outw[1], outw[2], outw[3]);
+                *reinterpret_cast<uint4*>(g + 4) = make_uint4(outw[4], outw[5], outw[6], outw[7]);
             } else {
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (p0 + k < r1) T[p0 + k] = outw[k];
+                    if (p0 + k < r1) g[k] = outw[k];
             }
         }
     }
@@ -1299,12 +1406,12 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
 
     // ---- K: length codes over the bytes; bitmap cleared (lane-coalesced).
     for (uint32_t p0 = (uint32_t)tid * 4u; p0 < n; p0 += LC_THREADS * 4u) {
-        if (p0 + 4 <= n) {
-            const uint4 v = *reinterpret_cast<const uint4*>(T + p0);
+        if (p0 + 4 <= n) {  // four positions of one group (p0 is a multiple of 4)
+            const uint4 v = *reinterpret_cast<const uint4*>(T + L.index(p0));
             D[p0 >> 2] = len_code8(tok_len(v.x)) | (len_code8(tok_len(v.y)) << 8) | (len_code8(tok_len(v.z)) << 16) |
                          (len_code8(tok_len(v.w)) << 24);
         } else {
-            for (uint32_t p = p0; p < n; ++p) code[p] = (uint8_t)len_code8(tok_len(T[p]));
+            for (uint32_t p = p0; p < n; ++p) code[p] = (uint8_t)len_code8(tok_len(T[L.index(p)]));
         }
     }
     for (uint32_t w = tid; w < ((n + 31) >> 5); w += LC_THREADS) bm[w] = 0;
@@ -1319,7 +1426,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             const uint32_t cw = D[p >> 2];
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (p + k < r1) runmax = max(runmax, p + k + code_step((cw >> (8 * k)) & 0xFFu, T, p + k));
+                if (p + k < r1) runmax = max(runmax, p + k + code_step((cw >> (8 * k)) & 0xFFu, T, L, p + k));
         }
         uint32_t v = runmax;
 #pragma unroll
@@ -1348,7 +1455,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             for (int k = 0; k < 4; ++k) {
                 if (p + k < r1 && first == 0xFFFFFFFFu) {
                     if (carry <= p + k) first = p + k;
-                    else carry = max(carry, p + k + code_step((cw >> (8 * k)) & 0xFFu, T, p + k));
+                    else carry = max(carry, p + k + code_step((cw >> (8 * k)) & 0xFFu, T, L, p + k));
                 }
             }
         }
@@ -1363,9 +1470,9 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         if (pstart != 0xFFFFFFFFu) {
             uint32_t pos = pstart;
             while (pos < n) {
-                if (pos >= r1 && fb[pos / S] == pos) break;
+                if (pos >= r1 && fb[L.range_of(pos)] == pos) break;
                 atomicOr(&bm[pos >> 5], 1u << (pos & 31));
-                pos += code_step(code[pos], T, pos);
+                pos += code_step(code[pos], T, L, pos);
             }
         }
     }
@@ -1376,27 +1483,29 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     // dead length codes (a fixed block is under n + 5 bytes, so it may reach
     // three words into the bitmap, which every range has copied to registers
     // by then) and copied to the chunk's staging words in one coalesced pass.
-    const uint32_t w0 = r0 >> 5, nwd = r0 < r1 ? ((r1 + 31) >> 5) - w0 : 0u;  // <= 4 words
-    uint32_t myw[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) myw[k] = (uint32_t)k < nwd ? bm[w0 + k] : 0u;
+    // The range's bitmap words (<= 5; the first and last may hold bits of
+    // the neighbouring ranges, which no group of this range selects).
+    const uint32_t w0 = r0 >> 5, nwd = r0 < r1 ? ((r1 + 31) >> 5) - w0 : 0u;
+    const uint32_t bw0 = nwd > 0 ? bm[w0] : 0u, bw1 = nwd > 1 ? bm[w0 + 1] : 0u, bw2 = nwd > 2 ? bm[w0 + 2] : 0u,
+                   bw3 = nwd > 3 ? bm[w0 + 3] : 0u, bw4 = nwd > 4 ? bm[w0 + 4] : 0u;
     // token words are streamed in range order, 8 positions (two 16-byte
     // loads) at a time, skipping groups without a token start; the next
     // group's loads are issued before the current group is used
     auto group_mask = [&](uint32_t p0) -> uint32_t {
         const uint32_t k = (p0 >> 5) - w0;
-        const uint32_t wv32 = k == 0 ? myw[0] : (k == 1 ? myw[1] : (k == 2 ? myw[2] : myw[3]));
+        const uint32_t wv32 = k == 0 ? bw0 : (k == 1 ? bw1 : (k == 2 ? bw2 : (k == 3 ? bw3 : bw4)));
         return (wv32 >> (p0 & 31)) & 0xFFu;
     };
     auto load_group = [&](uint32_t p0, uint32_t* tv) {
+        const uint32_t* g = tg(p0);
         if (p0 + 8 <= r1) {
-            const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
-            const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
+            const uint4 v0 = *reinterpret_cast<const uint4*>(g);
+            const uint4 v1 = *reinterpret_cast<const uint4*>(g + 4);
             tv[0] = v0.x; tv[1] = v0.y; tv[2] = v0.z; tv[3] = v0.w;
             tv[4] = v1.x; tv[5] = v1.y; tv[6] = v1.z; tv[7] = v1.w;
         } else {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) tv[j] = p0 + j < r1 ? T[p0 + j] : 0u;
+            for (int j = 0; j < 8; ++j) tv[j] = p0 + j < r1 ? g[j] : 0u;
         }
     };
     auto next_group = [&](uint32_t p0) -> uint32_t {  // first group at or after p0 with a token start
@@ -1496,7 +1605,6 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     for (uint32_t i = tid; i < nw; i += LC_THREADS) stg[i] = out[i];
 }
 
-template <int W, bool EXCH>
 __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
     uint32_t chunk, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
@@ -1505,7 +1613,7 @@ __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* _
     for (;;) {
         const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
         if (k == NO_ITEM) break;
-        l1_barrier_body<W, EXCH>(order[k], smem, src, desc, chunk, tokw, stage, stage_words, info);
+        l1_barrier_body(order[k], smem, src, desc, chunk, tokw, stage, stage_words, info);
         __syncthreads();
     }
 }
@@ -1868,11 +1976,14 @@ hipError_t launch_chunk(const Plan& p) {
     return hipSuccess;
 }
 
-template <int W, bool EXCH>
+template <bool EXCH>
 hipError_t launch_barrier(const Plan& p) {
-    const size_t smem = LBCfg<W>::LDS_BYTES;
-    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, smem, LC_THREADS));
-    hipLaunchKernelGGL((l1_barrier_kernel<W, EXCH>), dim3(g), dim3(LC_THREADS), smem, p.stream, p.d_in, p.d_desc,
+    const unsigned gc = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, sizeof(uint32_t) * 2 * LA_TAB +
+                                                                           sizeof(uint16_t) * LA_TAB, LA_THREADS));
+    hipLaunchKernelGGL((l1_candidate_kernel<EXCH>), dim3(gc), dim3(LA_THREADS), 0, p.stream, p.d_in, p.d_desc,
+                       p.d_hdr, p.chunk, p.d_tok);
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, LB_LDS_BYTES, LC_THREADS));
+    hipLaunchKernelGGL(l1_barrier_kernel, dim3(g), dim3(LC_THREADS), LB_LDS_BYTES, p.stream, p.d_in, p.d_desc,
                        p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_stage, p.stage_words, p.d_info);
     return hipSuccess;
 }
@@ -1888,11 +1999,7 @@ hipError_t launch_wide(const Plan& p) {
 
 hipError_t launch_l1_class(const Plan& p) {
     if (p.hash_bits == 12) {
-        if (p.chunk <= LC_MAX_CHUNK) {
-            if (p.lds_max >= LBCfg<2>::LDS_BYTES)
-                return p.exch_ok ? launch_barrier<2, true>(p) : launch_barrier<2, false>(p);
-            return p.exch_ok ? launch_barrier<1, true>(p) : launch_barrier<1, false>(p);
-        }
+        if (p.chunk <= LC_MAX_CHUNK) return p.exch_ok ? launch_barrier<true>(p) : launch_barrier<false>(p);
         if (p.chunk <= MAX_CHUNK_L1) return launch_wide<12>(p);
         return hipErrorInvalidValue;
     }
@@ -1925,10 +2032,7 @@ hipError_t check_exchange_order(hipStream_t stream, bool* ok) {
 hipError_t set_kernel_attributes_l1(size_t lds_max) {
     hipError_t e = allow_lds(&l1_chunk_kernel<14, true>, LC_LDS_BYTES);
     if (e == hipSuccess) e = allow_lds(&l1_chunk_kernel<14, false>, LC_LDS_BYTES);
-    if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel<1, true>, LBCfg<1>::LDS_BYTES);
-    if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel<1, false>, LBCfg<1>::LDS_BYTES);
-    if (e == hipSuccess && lds_max >= LBCfg<2>::LDS_BYTES) e = allow_lds(&l1_barrier_kernel<2, true>, LBCfg<2>::LDS_BYTES);
-    if (e == hipSuccess && lds_max >= LBCfg<2>::LDS_BYTES) e = allow_lds(&l1_barrier_kernel<2, false>, LBCfg<2>::LDS_BYTES);
+    if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel, LB_LDS_BYTES);
     const size_t wide = L1Cfg<12>::TAB_BYTES > L1_PHASE_B_BYTES ? L1Cfg<12>::TAB_BYTES : L1_PHASE_B_BYTES;
     if (e == hipSuccess) e = allow_lds(&l1_lz77_kernel<12>, wide);
     return e;

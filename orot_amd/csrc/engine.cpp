@@ -204,8 +204,7 @@ bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     h_res_ = nullptr;
     chunk_cap_ = buf_cap_ = 0;
     const uint32_t chunk = cfg_.chunk;
-    const size_t cp = ((size_t)chunk + 7) & ~(size_t)7;
-    const size_t per_pos = (size_t)ncap * (l6 ? l6_position_stride(chunk) : cp) + 8;
+    const size_t per_pos = (size_t)ncap * (l6 ? l6_position_stride(chunk) : l1_position_stride(chunk)) + 8;
     bool ok = true;
     if (l1) {
         ok = ok && dev_alloc(d_tok_, per_pos, "hipMalloc(token words)") &&

@@ -68,10 +68,11 @@ def main() -> int:
                     out["large"].append({"kind": kind, "n": n, "seed": seed, "level": level, "format": fmt,
                                          "block": block, "size": len(comp), "sha256": sha(comp)})
     # the CPU reference configuration: 16 MiB Zipf text, L1, gzip, 128 KiB chunks
-    cfg1 = corpus.generate("text", 16 << 20, 0xC0F1)
-    comp = orc.ref_parallel_compress(cfg1, 1, 2, 131072, threads=8)
-    out["cfg1"] = {"kind": "text", "n": 16 << 20, "seed": 0xC0F1, "level": 1, "format": 2, "block": 131072,
-                   "size": len(comp), "sha256": sha(comp), "zlib_ok": zlib.decompress(comp, 31) == cfg1.tobytes()}
+    text16 = corpus.generate("text", 16 << 20, 0xC0F1)
+    comp = orc.ref_parallel_compress(text16, 1, 2, 131072, threads=8)
+    out["text16_l1_gzip"] = {"kind": "text", "n": 16 << 20, "seed": 0xC0F1, "level": 1, "format": 2,
+                             "block": 131072, "size": len(comp), "sha256": sha(comp),
+                             "zlib_ok": zlib.decompress(comp, 31) == text16.tobytes()}
     # 3. LZ77 token streams, levels 1-9 (one chunk, fresh tables)
     for kind in corpus.GENERATORS:
         for n in TOKEN_SIZES:

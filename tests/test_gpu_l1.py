@@ -55,3 +55,37 @@ def test_l1_wide_chunks(amd, kind, block):
     """Chunks above 128 KiB take the wide kernel (level 1 only)."""
     data = corpus.generate(kind, 700_000, 91)
     _check(amd, data, 1, 0, block)
+
+
+def _orot_fixtures():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "orot_fixtures.json")) as f:
+        return json.load(f)
+
+
+_FIX = _orot_fixtures()
+
+
+def _check_fixture(amd, e):
+    import hashlib
+
+    data = corpus.generate(e["kind"], e["n"], e["seed"])
+    out = np.empty(amd.compress_bound(len(data), e["format"]) + 4096, dtype=np.uint8)
+    with amd.ParallelCompressor(e["level"], e["format"], 0, e["block"]) as pc:
+        n = pc.compress_into(data, out)
+    assert n == e["size"], (n, e)
+    assert hashlib.sha256(out[:n].tobytes()).hexdigest() == e["sha256"], e
+
+
+@pytest.mark.parametrize("e", _FIX["large"], ids=lambda e: f'{e["kind"]}-{e["n"]}-L{e["level"]}-f{e["format"]}-b{e["block"]}')
+def test_orot_bytes_large(amd, e):
+    """GPU output equals the bytes orot's own parallel compressor wrote for
+    the same input (golden fixtures, gzip trailers with the IEEE CRC-32)."""
+    _check_fixture(amd, e)
+
+
+def test_orot_bytes_text16(amd):
+    """16 MiB Zipf text, level 1, gzip, 128 KiB chunks: orot's bytes."""
+    _check_fixture(amd, _FIX["text16_l1_gzip"])

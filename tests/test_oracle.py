@@ -186,8 +186,8 @@ def test_golden_large_digests(fixtures):
         assert len(got) == e["size"] and hashlib.sha256(got).hexdigest() == e["sha256"], e
 
 
-def test_golden_cfg1(fixtures):
-    e = fixtures["cfg1"]
+def test_golden_text16_l1_gzip(fixtures):
+    e = fixtures["text16_l1_gzip"]
     data = corpus.generate(e["kind"], e["n"], e["seed"])
     got = orc.parallel_compress(data, e["level"], e["format"], e["block"], threads=8)
     assert len(got) == e["size"] and hashlib.sha256(got).hexdigest() == e["sha256"]
