@@ -90,7 +90,7 @@ struct Plan {
     L6Params l6;
     int format;             // 0 raw, 1 zlib, 2 gzip
     uint32_t* d_tok;        // per position (chunk-padded): token word / match word
-    uint16_t* d_pd;         // L6 class: per position chain step, then decision code
+    uint16_t* d_pd;         // per position: L1 class candidate distance code, L6 class chain step then decision
     uint32_t* d_aux;        // L6 class: tokens in order (chunk-padded)
     uint32_t* d_ntok;       // L6 class: tokens per chunk
     uint32_t* d_blkpos;     // L6 class: raw position of each block's first token

@@ -1060,16 +1060,18 @@ __global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __r
 //   l1_candidate_kernel, one 128-thread workgroup per chunk, 40 KiB of LDS
 //   (four chunks resident per CU):
 //   A1  each wave scans one half of the chunk's hash positions in order: the
-//       latest earlier same-hash position of its half, one LDS exchange per
+//       latest earlier same-hash position C of its half, one LDS exchange per
 //       position (EXCH: same-address exchanges of one wave instruction take
 //       effect in lane order, checked at start-up by exchange_order_kernel)
-//       or a ballot match-any; T[p] = C + 1, 0 if none.  A position of the
-//       second half with no earlier same-hash position in that half takes the
-//       last one of the first half (the first half's final table).
+//       or a ballot match-any.  A position of the second half with no earlier
+//       same-hash position in that half takes the last one of the first half
+//       (the first half's final table).  Output per position, 16 bits:
+//       cand_code(p, C).
 //   l1_barrier_kernel, one 1024-thread workgroup per chunk:
 //   0   the chunk's bytes are staged in LDS.
-//   A2  per position: candidate distance (C == 0 empty, 15-bit masked,
-//       first byte compared at C: lz77.cpp:178-184) and greedy match length
+//   A2  per position: candidate distance from its code (far candidates:
+//       C recovered and first byte compared at C, lz77.cpp:178-184) and
+//       greedy match length
 //       (lz77.cpp:163-191), the verified extent carried along a diagonal,
 //       all compares in LDS; T[p] = literal | length << 8 | distance << 17.
 //   K   the LDS bytes are replaced by u8 length codes (0 literal,
@@ -1117,20 +1119,34 @@ struct LbLayout {
 };
 constexpr uint32_t lb_tok_stride(uint32_t chunk) { return ((chunk + 7u) & ~7u) + 256u; }
 
+// Candidate code of position p whose latest earlier same-hash position is
+// C (C1 = C + 1, 0 if none).  orot takes head 0 as empty, masks the distance
+// to 15 bits and, when the masked distance differs from the true one,
+// accepts the candidate only if the bytes at C and p agree
+// (lz77.cpp:178-184).  Codes: 0 no candidate; 1..32767 the distance;
+// 0x8000 | dd a candidate 32 KiB or more back with masked distance dd != 0.
+// For the last, C is the nearest of p - dd - 32768 k (k >= 1) whose hash is
+// p's: no same-hash position lies between C and p.
+__device__ __forceinline__ uint32_t cand_code(uint32_t p, uint32_t C1) {
+    if (C1 <= 1) return 0;
+    const uint32_t dist = p - (C1 - 1u), dd = dist & 0x7FFFu;
+    if (dist == dd) return dd;
+    return dd ? (0x8000u | dd) : 0u;
+}
+
 template <bool EXCH>
 __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, const BatchHdr* __restrict__ hdr, uint32_t chunk,
-    uint32_t* __restrict__ tokw) {
+    uint16_t* __restrict__ cand) {
     constexpr int HB = 12;
     __shared__ uint32_t tab[2 * LA_TAB];  // per half: latest position + 1 of each hash
     __shared__ uint16_t first1[LA_TAB];   // second half: offset of each hash's first position
-    __shared__ uint32_t sink[LA_THREADS];  // exchange target of lanes that store nothing
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t nchunks = hdr->nchunks;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const uint32_t n = desc[c].len;
         const uint8_t* s = src + desc[c].in_off;
-        uint32_t* T = tokw + (uint64_t)c * lb_tok_stride(chunk);
+        uint16_t* Q = cand + (uint64_t)c * lb_tok_stride(chunk);  // LbLayout order
         const LbLayout L(n);
         const uint32_t nh = n >= 4 ? n - 3 : 0;
         // halves of 64-position groups: the second starts at SR <= 65536, so
@@ -1141,6 +1157,7 @@ __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t*
         __syncthreads();
         uint32_t* my = tab + (uint32_t)wv * LA_TAB;
         const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
+        const uint32_t spare = lb_tok_stride(chunk) - 1u;  // past every index of the layout
         // the 4-byte windows of 512 positions come from global memory; three
         // groups of loads (clamped into [a, b)) stay in flight ahead of the
         // exchanges
@@ -1157,12 +1174,10 @@ __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t*
                 // first lane stores the run's last position, the others take
                 // the position just before them.  The hash of the lane below
                 // comes from a DPP wave shift (lane 0 gets an impossible
-                // hash), and every lane exchanges, the ones that store
-                // nothing on a word of their own, so the eight exchanges of
-                // a step are issued back to back.
-                uint32_t* addr[8];
+                // hash); with no LDS instruction before them, the eight
+                // masked exchanges of a step are issued back to back.
+                bool head[8], same[8];
                 uint32_t val[8];
-                bool same[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const uint32_t p = p0 + 64u * j + (uint32_t)lane;
@@ -1172,12 +1187,11 @@ __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t*
                     same[j] = valid && hp == h[j];
                     const uint64_t E = ballot(same[j]);
                     const uint32_t extra = (uint32_t)__builtin_ctzll(~((E >> lane) >> 1));
-                    const bool head = valid && !same[j];
-                    addr[j] = head ? &my[h[j]] : &sink[tid];
+                    head[j] = valid && !same[j];
                     val[j] = p + extra + 1u;
                 }
 #pragma unroll
-                for (int j = 0; j < 8; ++j) res[j] = atomicExch(addr[j], val[j]);
+                for (int j = 0; j < 8; ++j) res[j] = head[j] ? atomicExch(&my[h[j]], val[j]) : 0u;
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const uint32_t p = p0 + 64u * j + (uint32_t)lane;
@@ -1206,13 +1220,17 @@ __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t*
                     }
                 }
             }
+            // Every lane stores (lanes past the half on a spare slot beyond
+            // the layout), so the compiler counts the stores exactly and the
+            // waits for the prefetched loads do not wait for them.  A first
+            // position of the second half stores 0 here and its code after
+            // the barrier.
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const uint32_t p = p0 + 64u * j + (uint32_t)lane;
-                if (p < b) {
-                    if (wv == 1 && res[j] == 0) first1[h[j]] = (uint16_t)(p - a);  // one per hash
-                    else T[L.index(p)] = res[j];
-                }
+                const bool valid = p < b;
+                if (valid && wv == 1 && res[j] == 0) first1[h[j]] = (uint16_t)(p - a);  // one per hash
+                Q[valid ? L.index(p) : spare] = (uint16_t)cand_code(p, res[j]);
             }
         };
         uint32_t w0[8], w1[8], w2[8], w3[8];
@@ -1238,7 +1256,7 @@ __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t*
         // first positions of the second half: the first half's latest position
         for (uint32_t i = tid; i < LA_TAB; i += LA_THREADS) {
             const uint32_t f = first1[i];
-            if (f != LA_NONE) T[L.index(SR + f)] = tab[i];
+            if (f != LA_NONE) Q[L.index(SR + f)] = (uint16_t)cand_code(SR + f, tab[i]);
         }
         __syncthreads();
     }
@@ -1267,11 +1285,14 @@ __device__ __forceinline__ uint32_t code_step(uint32_t b, const uint32_t* T, con
 // bm over [a, b); T read 8 tokens at a time.
 
 __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
-    const ChunkDesc* __restrict__ desc, uint32_t chunk, uint32_t* __restrict__ tokw,
-    uint32_t* __restrict__ stage, uint64_t stage_words, ChunkInfo* __restrict__ info) {
+    const ChunkDesc* __restrict__ desc, uint32_t chunk, const uint16_t* __restrict__ cand,
+    uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
+    ChunkInfo* __restrict__ info) {
     const uint32_t n = desc[c].len;
     const uint8_t* s = src + desc[c].in_off;
-    uint32_t* T = tokw + (uint64_t)c * lb_tok_stride(chunk);  // C + 1 from l1_candidate_kernel, LbLayout order
+    // candidate codes from l1_candidate_kernel and token words, LbLayout order
+    const uint16_t* Q = cand + (uint64_t)c * lb_tok_stride(chunk);
+    uint32_t* T = tokw + (uint64_t)c * lb_tok_stride(chunk);
     const LbLayout L(n);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     uint32_t* D = smem;                         // chunk bytes, then length codes
@@ -1301,9 +1322,9 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     if (tid < 4) D[((n + 3) >> 2) + tid] = 0;  // past the staged words
     __syncthreads();
 
-    // ---- A2: candidate distance (C == 0 empty, 15-bit masked, first byte
-    // compared at C: lz77.cpp:178-184) and greedy match length
-    // (lz77.cpp:163-191).
+    // ---- A2: candidate distance from the candidate code (cand_code; for a
+    // far candidate C is recovered by hash and its first byte compared,
+    // lz77.cpp:178-184) and greedy match length (lz77.cpp:163-191).
     const uint32_t nh = n >= 4 ? n - 3 : 0;
     const uint32_t S = L.S;
     const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
@@ -1313,37 +1334,44 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         const uint32_t lim = min(n, r1 + 257u);
         uint32_t cd = 0, ce = 0;
         bool cx = false;
-        uint32_t nx[8];
-        auto load_loc = [&](uint32_t q0, uint32_t* dst) {
-            const uint32_t* g = tg(q0);
-            if (q0 + 8 <= nh) {
-                const uint4 v0 = *reinterpret_cast<const uint4*>(g);
-                const uint4 v1 = *reinterpret_cast<const uint4*>(g + 4);
-                dst[0] = v0.x; dst[1] = v0.y; dst[2] = v0.z; dst[3] = v0.w;
-                dst[4] = v1.x; dst[5] = v1.y; dst[6] = v1.z; dst[7] = v1.w;
-            } else {
+        uint4 nx;  // the next group's eight 16-bit candidate codes
+        auto load_loc = [&](uint32_t q0) -> uint4 {
+            const uint16_t* g = Q + L.group((uint32_t)tid, (q0 - r0) >> 3);
+            if (q0 + 8 <= nh) return *reinterpret_cast<const uint4*>(g);
+            uint32_t v[4] = {0, 0, 0, 0};
 #pragma unroll
-                for (int k = 0; k < 8; ++k) dst[k] = (q0 + k < nh) ? g[k] : 0u;
-            }
+            for (int k = 0; k < 8; ++k)
+                if (q0 + k < nh) v[k >> 1] |= (uint32_t)g[k] << (16 * (k & 1));
+            return make_uint4(v[0], v[1], v[2], v[3]);
         };
-        if (r0 < r1) load_loc(r0, nx);
+        if (r0 < r1) nx = load_loc(r0);
         for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
-            uint32_t loc[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) loc[k] = nx[k];
-            if (p0 + 8 < r1) load_loc(p0 + 8, nx);
+            const uint4 lv = nx;
+            const uint32_t lw[4] = {lv.x, lv.y, lv.z, lv.w};
+            if (p0 + 8 < r1) nx = load_loc(p0 + 8);
             const uint64_t w0 = lds_u64(D, p0), w1 = lds_u64(D, p0 + 8);
             uint32_t outw[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const uint32_t p = p0 + k;
                 const uint32_t lit = win_byte(w0, w1, k);
-                const uint32_t C1 = loc[k];
+                const uint32_t e = (lw[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
                 uint32_t d = 0, m1 = 0;
-                if (p < r1 && C1 > 1) {
-                    const uint32_t C = C1 - 1u;
-                    const uint32_t dd = (p - C) & 0x7FFFu;
-                    if (dd && (p - C == dd || lds_u8(D, C) == lit)) d = dd;
+                if (p < r1 && e) {
+                    if (!(e & 0x8000u)) {
+                        d = e;
+                    } else {
+                        // the nearest same-hash position among p - dd - 32768 k
+                        const uint32_t dd = e & 0x7FFFu;
+                        const uint32_t hp = hash4<12>((uint32_t)(win_u64(w0, w1, k) & 0xFFFFFFFFull));
+                        for (uint32_t back = dd + 32768u; back <= p; back += 32768u) {
+                            const uint32_t C = p - back;
+                            if (hash4<12>(lds_u32(D, C)) == hp) {
+                                if (lds_u8(D, C) == lit) d = dd;
+                                break;
+                            }
+                        }
+                    }
                 }
                 if (d) {
                     const uint32_t cap = min(258u, n - p);
@@ -1607,13 +1635,13 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
 
 __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
-    uint32_t chunk, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
-    ChunkInfo* __restrict__ info) {
+    uint32_t chunk, const uint16_t* __restrict__ cand, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage,
+    uint64_t stage_words, ChunkInfo* __restrict__ info) {
     extern __shared__ uint32_t smem[];
     for (;;) {
         const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
         if (k == NO_ITEM) break;
-        l1_barrier_body(order[k], smem, src, desc, chunk, tokw, stage, stage_words, info);
+        l1_barrier_body(order[k], smem, src, desc, chunk, cand, tokw, stage, stage_words, info);
         __syncthreads();
     }
 }
@@ -1981,10 +2009,10 @@ hipError_t launch_barrier(const Plan& p) {
     const unsigned gc = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, sizeof(uint32_t) * 2 * LA_TAB +
                                                                            sizeof(uint16_t) * LA_TAB, LA_THREADS));
     hipLaunchKernelGGL((l1_candidate_kernel<EXCH>), dim3(gc), dim3(LA_THREADS), 0, p.stream, p.d_in, p.d_desc,
-                       p.d_hdr, p.chunk, p.d_tok);
+                       p.d_hdr, p.chunk, p.d_pd);
     const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, LB_LDS_BYTES, LC_THREADS));
     hipLaunchKernelGGL(l1_barrier_kernel, dim3(g), dim3(LC_THREADS), LB_LDS_BYTES, p.stream, p.d_in, p.d_desc,
-                       p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_stage, p.stage_words, p.d_info);
+                       p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.d_stage, p.stage_words, p.d_info);
     return hipSuccess;
 }
 

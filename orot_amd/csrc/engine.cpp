@@ -208,12 +208,12 @@ bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     bool ok = true;
     if (l1) {
         ok = ok && dev_alloc(d_tok_, per_pos, "hipMalloc(token words)") &&
+             dev_alloc(d_pd_, per_pos, "hipMalloc(candidates / chain steps)") &&
              dev_alloc(d_stage_, (size_t)ncap * stage_words_for_chunk(chunk), "hipMalloc(staging)");
     }
     if (l6) {
         const size_t mb = l6_max_blocks(chunk);
-        ok = ok && dev_alloc(d_pd_, per_pos, "hipMalloc(chain steps)") &&
-             dev_alloc(d_aux_, per_pos, "hipMalloc(tokens)") && dev_alloc(d_ntok_, ncap, "hipMalloc(token counts)") &&
+        ok = ok && dev_alloc(d_aux_, per_pos, "hipMalloc(tokens)") && dev_alloc(d_ntok_, ncap, "hipMalloc(token counts)") &&
              dev_alloc(d_blkpos_, (size_t)ncap * mb, "hipMalloc(block starts)") &&
              dev_alloc(d_blocks_, (size_t)ncap * mb * l6_block_record_bytes(), "hipMalloc(block records)");
     }
