@@ -1090,7 +1090,6 @@ constexpr uint32_t LB_FB_OFF = LB_BM_OFF + LC_BM_WORDS;       // first barrier p
 constexpr uint32_t LB_MISC_OFF = LB_FB_OFF + LC_THREADS;
 constexpr uint32_t LB_LDS_BYTES = (LB_MISC_OFF + 64u) * 4u;
 
-constexpr int LA_THREADS = 128;  // candidate kernel: one wave per half chunk
 constexpr uint32_t LA_TAB = 1u << 12;
 constexpr uint16_t LA_NONE = 0xFFFFu;
 
@@ -1134,13 +1133,21 @@ __device__ __forceinline__ uint32_t cand_code(uint32_t p, uint32_t C1) {
     return dd ? (0x8000u | dd) : 0u;
 }
 
-template <bool EXCH>
-__global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t* __restrict__ src,
+// LDS of the candidate kernel with W waves: a table per sub-range (latest
+// position + 1 of each hash) and, for sub-ranges 1.., the offset of each
+// hash's first position.
+template <int W>
+constexpr size_t la_lds_bytes() { return (size_t)W * LA_TAB * 4u + (size_t)(W - 1) * LA_TAB * 2u; }
+
+template <int W, bool EXCH>
+__global__ __launch_bounds__(64 * W) void l1_candidate_kernel(const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, const BatchHdr* __restrict__ hdr, uint32_t chunk,
     uint16_t* __restrict__ cand) {
     constexpr int HB = 12;
-    __shared__ uint32_t tab[2 * LA_TAB];  // per half: latest position + 1 of each hash
-    __shared__ uint16_t first1[LA_TAB];   // second half: offset of each hash's first position
+    constexpr int NT = 64 * W;
+    extern __shared__ uint32_t la_smem[];
+    uint32_t* tab = la_smem;                                         // W x 4096
+    uint16_t* first = reinterpret_cast<uint16_t*>(la_smem + W * LA_TAB);  // (W - 1) x 4096
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t nchunks = hdr->nchunks;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -1149,11 +1156,13 @@ __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t*
         uint16_t* Q = cand + (uint64_t)c * lb_tok_stride(chunk);  // LbLayout order
         const LbLayout L(n);
         const uint32_t nh = n >= 4 ? n - 3 : 0;
-        // halves of 64-position groups: the second starts at SR <= 65536, so
-        // its offsets fit 16 bits
-        const uint32_t SR = (((nh + 1) / 2) + 63u) & ~63u;
-        for (uint32_t i = tid; i < 2 * LA_TAB; i += LA_THREADS) tab[i] = 0;
-        for (uint32_t i = tid; i < LA_TAB; i += LA_THREADS) first1[i] = LA_NONE;
+        // W sub-ranges of 64-position groups; every offset inside a
+        // sub-range other than the first is below 65535 (two sub-ranges of
+        // a 128 KiB chunk: at most 65532), so it fits 16 bits beside LA_NONE
+        const uint32_t SR = ((nh + W - 1) / W + 63u) & ~63u;
+        for (uint32_t i = tid; i < W * LA_TAB; i += NT) tab[i] = 0;
+        for (uint32_t i = tid; i < (W - 1) * LA_TAB / 2; i += NT)
+            reinterpret_cast<uint32_t*>(first)[i] = ((uint32_t)LA_NONE << 16) | LA_NONE;
         __syncthreads();
         uint32_t* my = tab + (uint32_t)wv * LA_TAB;
         const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
@@ -1229,7 +1238,7 @@ __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t*
             for (int j = 0; j < 8; ++j) {
                 const uint32_t p = p0 + 64u * j + (uint32_t)lane;
                 const bool valid = p < b;
-                if (valid && wv == 1 && res[j] == 0) first1[h[j]] = (uint16_t)(p - a);  // one per hash
+                if (valid && wv >= 1 && res[j] == 0) first[(wv - 1) * LA_TAB + h[j]] = (uint16_t)(p - a);  // one per hash
                 Q[valid ? L.index(p) : spare] = (uint16_t)cand_code(p, res[j]);
             }
         };
@@ -1253,10 +1262,17 @@ __global__ __launch_bounds__(LA_THREADS) void l1_candidate_kernel(const uint8_t*
             step(p0 + 1536u, w3);
         }
         __syncthreads();
-        // first positions of the second half: the first half's latest position
-        for (uint32_t i = tid; i < LA_TAB; i += LA_THREADS) {
-            const uint32_t f = first1[i];
-            if (f != LA_NONE) Q[L.index(SR + f)] = (uint16_t)cand_code(SR + f, tab[i]);
+        // a hash's first position in sub-range j takes its latest position
+        // in the sub-ranges before j
+        for (uint32_t i = tid; i < LA_TAB; i += NT) {
+            uint32_t last = tab[i];
+#pragma unroll
+            for (int j = 1; j < W; ++j) {
+                const uint32_t f = first[(j - 1) * LA_TAB + i];
+                if (f != LA_NONE) Q[L.index(j * SR + f)] = (uint16_t)cand_code(j * SR + f, last);
+                const uint32_t t = tab[j * LA_TAB + i];
+                if (t) last = t;
+            }
         }
         __syncthreads();
     }
@@ -2004,12 +2020,20 @@ hipError_t launch_chunk(const Plan& p) {
     return hipSuccess;
 }
 
+template <int W, bool EXCH>
+void launch_candidates(const Plan& p) {
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, la_lds_bytes<W>(), 64 * W));
+    hipLaunchKernelGGL((l1_candidate_kernel<W, EXCH>), dim3(g), dim3(64 * W), la_lds_bytes<W>(), p.stream, p.d_in,
+                       p.d_desc, p.d_hdr, p.chunk, p.d_pd);
+}
+
 template <bool EXCH>
 hipError_t launch_barrier(const Plan& p) {
-    const unsigned gc = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, sizeof(uint32_t) * 2 * LA_TAB +
-                                                                           sizeof(uint16_t) * LA_TAB, LA_THREADS));
-    hipLaunchKernelGGL((l1_candidate_kernel<EXCH>), dim3(gc), dim3(LA_THREADS), 0, p.stream, p.d_in, p.d_desc,
-                       p.d_hdr, p.chunk, p.d_pd);
+    // Few chunks: six waves per chunk, one chunk per CU, for the shortest
+    // scan per chunk.  Many: two waves per chunk, four chunks per CU, where
+    // the scan is bound by its memory traffic.  Both give the same codes.
+    if (p.nchunks <= 2u * (uint32_t)p.cus) launch_candidates<6, EXCH>(p);
+    else launch_candidates<2, EXCH>(p);
     const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, LB_LDS_BYTES, LC_THREADS));
     hipLaunchKernelGGL(l1_barrier_kernel, dim3(g), dim3(LC_THREADS), LB_LDS_BYTES, p.stream, p.d_in, p.d_desc,
                        p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.d_stage, p.stage_words, p.d_info);
@@ -2061,6 +2085,10 @@ hipError_t set_kernel_attributes_l1(size_t lds_max) {
     hipError_t e = allow_lds(&l1_chunk_kernel<14, true>, LC_LDS_BYTES);
     if (e == hipSuccess) e = allow_lds(&l1_chunk_kernel<14, false>, LC_LDS_BYTES);
     if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel, LB_LDS_BYTES);
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<2, true>, la_lds_bytes<2>());
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<2, false>, la_lds_bytes<2>());
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<6, true>, la_lds_bytes<6>());
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<6, false>, la_lds_bytes<6>());
     const size_t wide = L1Cfg<12>::TAB_BYTES > L1_PHASE_B_BYTES ? L1Cfg<12>::TAB_BYTES : L1_PHASE_B_BYTES;
     if (e == hipSuccess) e = allow_lds(&l1_lz77_kernel<12>, wide);
     return e;

@@ -89,3 +89,13 @@ def test_orot_bytes_large(amd, e):
 def test_orot_bytes_text16(amd):
     """16 MiB Zipf text, level 1, gzip, 128 KiB chunks: orot's bytes."""
     _check_fixture(amd, _FIX["text16_l1_gzip"])
+
+
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+@pytest.mark.parametrize("block", [4096, 65536])
+def test_many_and_few_chunks(amd, kind, block):
+    """3 MiB at 4 KiB chunks (768 chunks: two-wave candidate scan, four
+    chunks per CU) and at 64 KiB chunks (48 chunks: six-wave scan); both
+    scans give the same bytes."""
+    data = corpus.generate(kind, 3 << 20, 4242)
+    _check(amd, data, 1, 2, block)
