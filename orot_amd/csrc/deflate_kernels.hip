@@ -1395,35 +1395,17 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                     if (d == cd && p <= ce && (cx || ce >= p + cap)) {
                         L = ce - p;
                     } else {
-                        uint32_t e;
-                        bool x;
-                        if (p + 8 <= n) {
-                            const uint64_t diff = win_u64(w0, w1, k) ^ lds_u64(D, p - d);
-                            if (diff) {
-                                e = p + (uint32_t)(__builtin_ctzll(diff) >> 3);
-                                x = true;
-                            } else {
-                                e = p + 8;
-                                x = e >= n;
-                            }
-                        } else {
-                            e = p;
-                            while (e < n && lds_u8(D, e) == lds_u8(D, e - d)) ++e;
-                            x = true;
-                        }
+                        // 8-byte compares only: every compare starts below
+                        // lim <= n, so it reads at most 11 bytes past n, in
+                        // the zero padding; a match that runs into it only
+                        // overshoots n, and the cap below cuts it back.
+                        const uint64_t diff = win_u64(w0, w1, k) ^ lds_u64(D, p - d);
+                        uint32_t e = p + (diff ? (uint32_t)(__builtin_ctzll(diff) >> 3) : 8u);
+                        bool x = diff != 0;
                         while (!x && e < lim) {
-                            if (e + 8 <= n) {
-                                const uint64_t df = lds_u64(D, e) ^ lds_u64(D, e - d);
-                                if (df) {
-                                    e += (uint32_t)(__builtin_ctzll(df) >> 3);
-                                    x = true;
-                                } else {
-                                    e += 8;
-                                }
-                            } else {
-                                while (e < n && lds_u8(D, e) == lds_u8(D, e - d)) ++e;
-                                x = true;
-                            }
+                            const uint64_t df = lds_u64(D, e) ^ lds_u64(D, e - d);
+                            e += df ? (uint32_t)(__builtin_ctzll(df) >> 3) : 8u;
+                            x = df != 0;
                         }
                         cd = d;
                         ce = e;
