@@ -1850,6 +1850,30 @@ __device__ __forceinline__ uint32_t stored_byte(const ChunkView& cv, uint64_t j)
     return cv.raw[dbase + q - 5u];
 }
 
+// Stored chunk: the four output bytes j..j+3 of the byte-aligned part
+// (after the first block header's bits and pad) when they are all data
+// bytes of one piece (pieces of up to 65535 bytes, each after a 5-byte
+// header, or 4 bytes for the first, whose header byte shares the bits).
+__device__ __forceinline__ bool stored_word(const ChunkView& cv, uint32_t j, uint32_t& v) {
+    const uint32_t L = cv.len;
+    const uint32_t L0 = min(L, 65535u);
+    uint32_t q, Lk, dbase;
+    if (j < 4u + L0) {
+        q = j + 1u;
+        Lk = L0;
+        dbase = 0;
+    } else {
+        const uint32_t r = j - 4u - L0;
+        const uint32_t k = 1u + r / 65540u;
+        q = r - (k - 1u) * 65540u;
+        Lk = min(L - k * 65535u, 65535u);
+        dbase = k * 65535u;
+    }
+    if (q < 5u || q + 4u > 5u + Lk) return false;
+    v = ld32u(cv.raw + dbase + q - 5u);
+    return true;
+}
+
 // k (<= 8) bits of the chunk's bitstream starting at relative bit r.
 __device__ __forceinline__ uint32_t chunk_bits(const ChunkView& cv, uint64_t r, uint32_t k) {
     const uint32_t mask = (1u << k) - 1u;
@@ -1916,6 +1940,9 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
         const uint64_t wb1 = wb0 + 4 * nwords;                       // first byte after it
         for (uint64_t b = b0 + threadIdx.x; b < wb0; b += 256) dst[b] = (uint8_t)byte_at(b);
         for (uint64_t b = wb1 + threadIdx.x; b < b1; b += 256) dst[b] = (uint8_t)byte_at(b);
+        // a stored chunk's bytes are byte-aligned after its first header
+        // bits and pad: output byte b is byte b - jb0 of that part
+        const uint64_t jb0 = (o + 3 + ((8 - ((o + 3) & 7)) & 7)) >> 3;
         for (uint64_t w = threadIdx.x; w < nwords; w += 256) {
             const uint64_t b = wb0 + 4 * w;
             uint32_t v;
@@ -1924,7 +1951,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
                 const uint64_t wi = r >> 5;
                 const uint64_t x = (uint64_t)cv.stg[wi] | ((uint64_t)cv.stg[wi + 1] << 32);
                 v = (uint32_t)(x >> (r & 31));
-            } else {
+            } else if (!(cv.stored && 8 * (b + 4) <= e && b >= jb0 && stored_word(cv, (uint32_t)(b - jb0), v))) {
                 v = byte_at(b) | (byte_at(b + 1) << 8) | (byte_at(b + 2) << 16) | (byte_at(b + 3) << 24);
             }
             *reinterpret_cast<uint32_t*>(dst + b) = v;
