@@ -229,7 +229,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
         // step so each step has two independent LDS loads in flight.
         for (uint32_t xp = x0 + tid; xp < xe; xp += 2 * L6M_THREADS) {
             struct Walk {
-                uint32_t x, cap, cur, best, bd, w;
+                uint32_t x, cap, cur, ri, best, bd, w, xbw;
                 uint64_t xa, xb;
                 int k;
                 bool live;
@@ -242,6 +242,8 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                 m[i].bd = 0;
                 m[i].k = 0;
                 m[i].cur = x;
+                m[i].ri = x % L6M_PRING;  // ring slot of cur, kept as cur moves back
+                m[i].xbw = 0;             // x's word at best - 3 (best >= 4)
                 m[i].live = x < xe && x >= H && x + 4 <= n;
                 m[i].cap = m[i].live ? min(258u, n - x) : 0u;
                 // bytes x .. x+15 (beyond the chunk end they are stale or zero,
@@ -260,17 +262,18 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                         continue;
                     }
                     ++a.k;
-                    const uint32_t st = PR[a.cur % L6M_PRING];
+                    const uint32_t st = PR[a.ri];
                     const uint32_t q = a.cur - st;
                     if (st == 0 || a.x - q > 32768u) {
                         a.live = false;
                         continue;
                     }
                     a.cur = q;
+                    a.ri = a.ri >= st ? a.ri - st : a.ri + L6M_PRING - st;  // st <= 32768 < L6M_PRING
                     if (ring_u32(DR, q) != a.w) continue;
                     // only a match that also covers byte `best` can be longer
                     // than best (best < cap here, so these bytes are in the chunk)
-                    if (a.best >= 4 && ring_u32(DR, q + a.best - 3) != ring_u32(DR, a.x + a.best - 3)) continue;
+                    if (a.best >= 4 && ring_u32(DR, q + a.best - 3) != a.xbw) continue;
                     uint32_t l;
                     uint64_t df = ring_u64(DR, q) ^ a.xa;
                     if (df) {
@@ -297,6 +300,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                     if (l > a.best) {
                         a.best = l;
                         a.bd = a.x - q;
+                        if (l >= 4) a.xbw = ring_u32(DR, a.x + l - 3);
                         // nothing later can be longer than nice (stop rule) or cap
                         if ((int)l >= nice || l >= a.cap) a.live = false;
                     }
