@@ -775,9 +775,13 @@ __global__ __launch_bounds__(L6B_THREADS) void l6_block_kernel(const uint32_t* _
                                                                const uint32_t* __restrict__ blkpos,
                                                                const BatchHdr* __restrict__ hdr, uint32_t chunk,
                                                                uint32_t maxblk, BlockRec* __restrict__ recs) {
-    const uint32_t items = hdr->nchunks * maxblk;
+    // items block-major (block b of every chunk, then b + 1): a chunk has
+    // fewer blocks than maxblk, and a chunk-major order with a grid that is a
+    // multiple of maxblk would hand every workgroup the same block index
+    const uint32_t nc = hdr->nchunks;
+    const uint32_t items = nc * maxblk;
     for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
-        l6_block_body(w / maxblk, w % maxblk, toks, ntok_in, blkpos, chunk, maxblk, recs);
+        l6_block_body(w % nc, w / nc, toks, ntok_in, blkpos, chunk, maxblk, recs);
         __syncthreads();
     }
 }
@@ -1038,9 +1042,11 @@ __global__ __launch_bounds__(L6E_THREADS) void l6_emit_kernel(const uint8_t* __r
                                                               const ChunkInfo* __restrict__ info,
                                                               const Result* __restrict__ res,
                                                               uint32_t* __restrict__ stage, uint64_t stage_words) {
-    const uint32_t items = hdr->nchunks * maxblk;
+    // items block-major, as in l6_block_kernel
+    const uint32_t nc = hdr->nchunks;
+    const uint32_t items = nc * maxblk;
     for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
-        l6_emit_body(w / maxblk, w % maxblk, src, desc, chunk, toks, ntok_in, recs, maxblk, info, res, stage,
+        l6_emit_body(w % nc, w / nc, src, desc, chunk, toks, ntok_in, recs, maxblk, info, res, stage,
                      stage_words);
         __syncthreads();
     }
@@ -1103,7 +1109,7 @@ hipError_t launch_l6_front(const Plan& p) {
     const unsigned gr = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6R_LDS, L6R_THREADS));
     hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), L6R_LDS, p.stream, p.d_in, p.d_desc, p.d_order,
                        p.d_hdr, p.chunk, p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.l6);
-    const unsigned gb = grid_for(p.persistent_grid, p.nchunks * maxblk, p.cus * 4u);
+    const unsigned gb = grid_for(p.persistent_grid, p.nchunks * maxblk, p.cus * 8u);
     hipLaunchKernelGGL(l6_block_kernel, dim3(gb), dim3(L6B_THREADS), 0, p.stream, p.d_aux, p.d_ntok, p.d_blkpos,
                        p.d_hdr, p.chunk, maxblk, reinterpret_cast<BlockRec*>(p.d_blocks));
     const unsigned gs = grid_for(p.persistent_grid, (p.nchunks + 255) / 256, p.cus);
