@@ -44,12 +44,17 @@ constexpr uint32_t L6M_TILE = 4096;
 constexpr uint32_t L6M_DRING = 65536;  // bytes, power of two
 constexpr uint32_t L6M_PRING = 36864;  // chain steps: 32 KiB + one tile
 constexpr uint32_t L6M_LDS = L6M_DRING + L6M_PRING * 2;
+constexpr int L6M_WALKS = 4;  // chain walks in step per thread
 
 constexpr int L6R_THREADS = 1024;
 constexpr int L6R_WAVES = L6R_THREADS / 64;
 constexpr uint32_t L6_MAX_CHUNK = 1u << 20;
 constexpr uint32_t L6R_BM_WORDS = L6_MAX_CHUNK / 32;
 constexpr uint32_t L6R_LDS = (L6R_BM_WORDS + 3 * L6R_THREADS + 64) * 4;
+// the parse kernel's LDS for chunks of at most `chunk` bytes: the token
+// bitmap, then the per-range arrays (512 KiB chunks: two workgroups per CU)
+constexpr uint32_t l6r_bm_words(uint32_t chunk) { return (chunk + 31u) / 32u; }
+constexpr uint32_t l6r_lds(uint32_t chunk) { return (l6r_bm_words(chunk) + 3 * L6R_THREADS + 64) * 4; }
 
 constexpr uint32_t BLK_SYMS = 16384;
 
@@ -225,17 +230,18 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
         for (uint32_t q = x0 + tid; q < xe; q += L6M_THREADS) PR[q % L6M_PRING] = PD[q];
         __syncthreads();
         if (xe <= H) continue;  // a tile wholly in the halo only fills the rings
-        // Two positions per thread, x and x + L6M_THREADS, walk their chains in
-        // step so each step has two independent LDS loads in flight.
-        for (uint32_t xp = x0 + tid; xp < xe; xp += 2 * L6M_THREADS) {
+        // L6M_WALKS positions per thread, x + i * L6M_THREADS, walk their
+        // chains in step, so each step has that many independent LDS loads
+        // in flight.
+        for (uint32_t xp = x0 + tid; xp < xe; xp += L6M_WALKS * L6M_THREADS) {
             struct Walk {
                 uint32_t x, cap, cur, ri, best, bd, w, xbw;
                 uint64_t xa, xb;
                 int k;
                 bool live;
-            } m[2];
+            } m[L6M_WALKS];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < L6M_WALKS; ++i) {
                 const uint32_t x = xp + (uint32_t)i * L6M_THREADS;
                 m[i].x = x;
                 m[i].best = 0;
@@ -252,9 +258,13 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                 m[i].xb = ring_u64(DR, x + 8);
                 m[i].w = (uint32_t)m[i].xa;
             }
-            while (m[0].live || m[1].live) {
+            for (;;) {
+                bool any = false;
 #pragma unroll
-                for (int i = 0; i < 2; ++i) {
+                for (int i = 0; i < L6M_WALKS; ++i) any |= m[i].live;
+                if (!any) break;
+#pragma unroll
+                for (int i = 0; i < L6M_WALKS; ++i) {
                     Walk& a = m[i];
                     if (!a.live) continue;
                     if (a.k >= max_chain) {
@@ -307,7 +317,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                 }
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < L6M_WALKS; ++i)
                 if (m[i].x < xe && m[i].x >= H) ML[m[i].x - H] = m[i].best >= 3 ? (m[i].best | (m[i].bd << 16)) : 0u;
         }
         __syncthreads();
@@ -362,7 +372,7 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
                                               uint32_t* __restrict__ toks, uint32_t* __restrict__ ntok_out,
                                               uint32_t* __restrict__ blkpos, uint32_t maxblk, const L6Params prm) {
     uint32_t* bm = smem;
-    uint32_t* fb = smem + L6R_BM_WORDS;
+    uint32_t* fb = smem + l6r_bm_words(chunk);
     uint32_t* rcnt = fb + L6R_THREADS;
     uint32_t* misc = rcnt + L6R_THREADS;
     const uint32_t n = desc[c].len;
@@ -1106,8 +1116,8 @@ hipError_t launch_l6_front(const Plan& p) {
     const unsigned gm = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6M_LDS, L6M_THREADS));
     hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.d_desc, p.d_bufs,
                        p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6);
-    const unsigned gr = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6R_LDS, L6R_THREADS));
-    hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), L6R_LDS, p.stream, p.d_in, p.d_desc, p.d_order,
+    const unsigned gr = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, l6r_lds(p.chunk), L6R_THREADS));
+    hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), l6r_lds(p.chunk), p.stream, p.d_in, p.d_desc, p.d_order,
                        p.d_hdr, p.chunk, p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.l6);
     const unsigned gb = grid_for(p.persistent_grid, p.nchunks * maxblk, p.cus * 8u);
     hipLaunchKernelGGL(l6_block_kernel, dim3(gb), dim3(L6B_THREADS), 0, p.stream, p.d_aux, p.d_ntok, p.d_blkpos,
