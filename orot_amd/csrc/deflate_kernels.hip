@@ -6,7 +6,10 @@
 //
 //   checksum_kernel   one workgroup per chunk: CRC-32 register or Adler-32
 //                     partial sums of the chunk (lane slices + GF(2) shifts).
-//   l1_lz77_kernel    one workgroup (8 waves) per chunk, levels 1-3:
+//   l1_candidate_kernel + l1_barrier_kernel   levels 1-3, chunks up to
+//                     128 KiB (orot's defaults): see the section below.
+//   l1_lz77_kernel    level 1 with chunks of 128-256 KiB, one workgroup
+//                     (8 waves) per chunk:
 //                     A) candidate distance for every position (orot's
 //                        single-probe head table, lz77.cpp:163-191), built
 //                        by 8 waves over 8 sub-ranges with private LDS
@@ -583,42 +586,13 @@ __global__ __launch_bounds__(L1_THREADS) void l1_lz77_kernel(const uint8_t* __re
 }
 
 // ---------------------------------------------------------------------------
-// L1-class kernel for chunks up to 128 KiB: one 1024-thread workgroup per
-// chunk, the chunk's match lengths resident in LDS.
-//   A1  latest earlier same-hash position per sub-range (8 sub-ranges for
-//       12-bit hashes, 2 for 14-bit), 8 positions per lane in flight; one LDS
-//       exchange per position (EXCH: same-address exchanges of one wave
-//       instruction take effect in lane order, checked at start-up by
-//       exchange_order_kernel) or a ballot match-any.
-//   A2a final candidate distance per position (earlier sub-ranges, aliased
-//       first-byte test).
-//   A2b match length per position with the verified extent carried along a
-//       diagonal; a u8 length code goes to LDS (0 literal, 1..253 = len - 2,
-//       254 = 258, 255 = 256/257 read back from the token word), the token
-//       word (literal | length << 8 | distance << 17) to HBM.
-//   B   speculative greedy parse of 1024 lane segments.
-//   R   serial resolve of segment entries on wave 0, all in LDS.
-//   C   bit counts, block-wide scan, stored / fixed choice, emission.
+// Shared by the L1-class kernels for chunks up to 128 KiB.
 // ---------------------------------------------------------------------------
 constexpr int LC_THREADS = 1024;
 constexpr int LC_WAVES = LC_THREADS / 64;
 constexpr uint32_t LC_MAX_CHUNK = 128u * 1024u;
 constexpr uint32_t LC_BM_WORDS = LC_MAX_CHUNK / 32;
-constexpr uint32_t LC_REGION0 = LC_MAX_CHUNK;  // bytes: tables during A1/A2a, length codes after
-constexpr uint32_t LC_LDS_BYTES = LC_REGION0 + (LC_BM_WORDS + LC_THREADS + 64) * 4;
-
-template <int HB>
-struct LCCfg {
-    static constexpr int W = HB == 12 ? 8 : 2;
-    static constexpr uint32_t TAB = 1u << HB;
-    static_assert(W * TAB * 4u <= LC_REGION0, "tables must fit region 0");
-};
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-    return v;
-}
+constexpr uint32_t LC_REGION0 = LC_MAX_CHUNK;  // bytes: the chunk, then length codes
 
 __device__ __forceinline__ uint32_t len_code8(uint32_t m1) {
     if (m1 == 0) return 0;
@@ -627,446 +601,19 @@ __device__ __forceinline__ uint32_t len_code8(uint32_t m1) {
     return m1 - 2;
 }
 
-// parse step at pos from its length code (escape: exact length from HBM)
-__device__ __forceinline__ uint32_t step_at(const uint8_t* code, const uint32_t* T, uint32_t pos) {
-    const uint32_t b = code[pos];
-    if (b == 0) return 1;
-    if (b < 254) return b + 2;
-    if (b == 254) return 258;
-    return tok_len(T[pos]);
-}
-
-template <int HB, bool EXCH>
-__device__ __forceinline__ void l1_chunk_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
-    const ChunkDesc* __restrict__ desc, uint32_t chunk, uint32_t* __restrict__ tokw,
-    uint32_t* __restrict__ stage, uint64_t stage_words, ChunkInfo* __restrict__ info) {
-    constexpr int W = LCCfg<HB>::W;
-    constexpr uint32_t TAB = LCCfg<HB>::TAB;
-    const uint32_t n = desc[c].len;
-    const uint8_t* s = src + desc[c].in_off;
-    uint32_t* T = tokw + (uint64_t)c * ((chunk + 7u) & ~7u);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    uint32_t* tabs = smem;
-    uint8_t* code = reinterpret_cast<uint8_t*>(smem);
-    uint32_t* bm = smem + LC_REGION0 / 4;
-    uint32_t* exits = bm + LC_BM_WORDS;
-    uint32_t* misc = exits + LC_THREADS;
-
-    // ---- A1
-    const uint32_t nh = n >= 4 ? n - 3 : 0;
-    const uint32_t SR = (((nh + W - 1) / W) + 63u) & ~63u;
-    if (wv < W) {
-        uint32_t* tab = tabs + (uint32_t)wv * TAB;
-        for (uint32_t i = lane; i < TAB; i += 64) tab[i] = 0;
-        const uint32_t a = (uint32_t)wv * SR;
-        const uint32_t b = min(a + SR, nh);
-        for (uint32_t p0 = a; p0 < b; p0 += 512) {
-            uint32_t wd[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) wd[j] = ld32u(s + min(p0 + 64u * j + (uint32_t)lane, b - 1));
-            uint32_t res[8];
-            if (EXCH) {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const uint32_t p = p0 + 64u * j + (uint32_t)lane;
-                    res[j] = p < b ? atomicExch(&tab[hash4<HB>(wd[j])], p + 1u) : 0u;
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const uint32_t q0 = p0 + 64u * j;
-                    const uint32_t p = q0 + (uint32_t)lane;
-                    const bool valid = p < b;
-                    const uint32_t h = hash4<HB>(wd[j]);
-                    uint64_t m = ballot(valid);
-#pragma unroll
-                    for (int k = 0; k < HB; ++k) {
-                        const uint32_t bit = (h >> k) & 1u;
-                        const uint64_t bk = ballot(bit != 0);
-                        m &= bk ^ ((uint64_t)bit - 1ull);
-                    }
-                    res[j] = 0;
-                    if (valid) {
-                        const uint64_t below = m & ((1ull << lane) - 1ull);
-                        if (below) res[j] = q0 + (uint32_t)(63 - __builtin_clzll(below)) + 1u;
-                        else res[j] = tab[h];
-                        if ((m >> lane) == 1ull) tab[h] = p + 1u;
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint32_t p = p0 + 64u * j + (uint32_t)lane;
-                if (p < b) T[p] = res[j];
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- A2a: final candidate distance; the first-byte test of a
-    // non-aliased candidate (C = p - d) is part of the match length.
-    for (uint32_t p0 = (uint32_t)tid * 8u; p0 < nh; p0 += LC_THREADS * 8u) {
-        uint32_t loc[8];
-        uint64_t w0 = 0, w1 = 0;
-        if (p0 + 8 <= nh) {
-            const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
-            const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
-            loc[0] = v0.x; loc[1] = v0.y; loc[2] = v0.z; loc[3] = v0.w;
-            loc[4] = v1.x; loc[5] = v1.y; loc[6] = v1.z; loc[7] = v1.w;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) loc[k] = (p0 + k < nh) ? T[p0 + k] : 0u;
-        }
-        if (p0 + 16 <= n) {
-            w0 = ld64u(s + p0);
-            w1 = ld64u(s + p0 + 8);
-        } else {
-            for (uint32_t j = 0; j < 16 && p0 + j < n; ++j) {
-                const uint64_t bv = s[p0 + j];
-                if (j < 8) w0 |= bv << (8 * j);
-                else w1 |= bv << (8 * (j - 8));
-            }
-        }
-        uint32_t outv[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t p = p0 + k;
-            uint32_t C1 = loc[k];
-            outv[k] = 0;
-            if (p >= nh) continue;
-            if (C1 == 0) {
-                const uint32_t w = p / SR;
-                if (w > 0) {
-                    const uint32_t h = hash4<HB>((uint32_t)(win_u64(w0, w1, k) & 0xFFFFFFFFull));
-                    for (int w2 = (int)w - 1; w2 >= 0; --w2) {
-                        const uint32_t u = tabs[(uint32_t)w2 * TAB + h];
-                        if (u) {
-                            C1 = u;
-                            break;
-                        }
-                    }
-                }
-            }
-            if (C1 > 1) {
-                const uint32_t C = C1 - 1u;
-                const uint32_t d = (p - C) & 0x7FFFu;
-                if (d && (p - C == d || s[C] == win_byte(w0, w1, k))) outv[k] = d;
-            }
-        }
-        if (p0 + 8 <= nh) {
-            *reinterpret_cast<uint4*>(T + p0) = make_uint4(outv[0], outv[1], outv[2], outv[3]);
-            *reinterpret_cast<uint4*>(T + p0 + 4) = make_uint4(outv[4], outv[5], outv[6], outv[7]);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (p0 + k < nh) T[p0 + k] = outv[k];
-        }
-    }
-    __syncthreads();
-
-    // ---- A2b: match lengths (orot match_find_fast + match_len,
-    // lz77.cpp:163-191) over the thread's run of positions.
-    const uint32_t S = max(32u, (((n + LC_THREADS - 1) / LC_THREADS) + 31u) & ~31u);
-    {
-        const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
-        const uint32_t lim = min(n, r1 + 257u);
-        uint32_t cd = 0, ce = 0;
-        bool cx = false;
-        for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
-            uint32_t dv[8];
-            if (p0 + 8 <= nh) {
-                const uint4 v0 = *reinterpret_cast<const uint4*>(T + p0);
-                const uint4 v1 = *reinterpret_cast<const uint4*>(T + p0 + 4);
-                dv[0] = v0.x; dv[1] = v0.y; dv[2] = v0.z; dv[3] = v0.w;
-                dv[4] = v1.x; dv[5] = v1.y; dv[6] = v1.z; dv[7] = v1.w;
-            } else {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) dv[k] = (p0 + k < nh) ? T[p0 + k] : 0u;
-            }
-            uint64_t w0 = 0, w1 = 0;
-            if (p0 + 16 <= n) {
-                w0 = ld64u(s + p0);
-                w1 = ld64u(s + p0 + 8);
-            } else {
-                for (uint32_t j = 0; j < 16 && p0 + j < n; ++j) {
-                    const uint64_t bv = s[p0 + j];
-                    if (j < 8) w0 |= bv << (8 * j);
-                    else w1 |= bv << (8 * (j - 8));
-                }
-            }
-            uint64_t cw[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) cw[k] = (dv[k] && p0 + k + 8 <= n) ? ld64u(s + p0 + k - dv[k]) : 0ull;
-            uint32_t outw[8];
-            uint32_t cod[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t p = p0 + k;
-                const uint32_t lit = win_byte(w0, w1, k);
-                const uint32_t d = dv[k];
-                uint32_t m1 = 0;
-                if (d && p < r1) {
-                    const uint32_t cap = min(258u, n - p);
-                    uint32_t L;
-                    if (d == cd && p <= ce && (cx || ce >= p + cap)) {
-                        L = ce - p;
-                    } else {
-                        uint32_t e;
-                        bool x;
-                        if (p + 8 <= n) {
-                            const uint64_t diff = win_u64(w0, w1, k) ^ cw[k];
-                            if (diff) {
-                                e = p + (uint32_t)(__builtin_ctzll(diff) >> 3);
-                                x = true;
-                            } else {
-                                e = p + 8;
-                                x = e >= n;
-                            }
-                        } else {
-                            e = p;
-                            while (e < n && s[e] == s[e - d]) ++e;
-                            x = true;
-                        }
-                        while (!x && e < lim) {
-                            if (e + 64 <= n) {
-                                uint64_t a8[8], b8[8];
-#pragma unroll
-                                for (int q = 0; q < 8; ++q) {
-                                    a8[q] = ld64u(s + e + 8 * q);
-                                    b8[q] = ld64u(s + e - d + 8 * q);
-                                }
-                                uint32_t adv = 64;
-#pragma unroll
-                                for (int q = 7; q >= 0; --q) {
-                                    const uint64_t df = a8[q] ^ b8[q];
-                                    if (df) adv = 8u * q + (uint32_t)(__builtin_ctzll(df) >> 3);
-                                }
-                                e += adv;
-                                if (adv < 64) x = true;
-                            } else {
-                                while (e < n && s[e] == s[e - d]) ++e;
-                                x = true;
-                            }
-                        }
-                        cd = d;
-                        ce = e;
-                        cx = x;
-                        L = e - p;
-                    }
-                    L = min(L, cap);
-                    m1 = L >= 3 ? L : 0u;
-                }
-                outw[k] = lit | (m1 << 8) | ((m1 ? d : 0u) << 17);
-                cod[k] = len_code8(m1);
-            }
-            if (p0 + 8 <= r1) {
-                *reinterpret_cast<uint4*>(T + p0) = make_uint4(outw[0], outw[1], outw[2], outw[3]);
-                *reinterpret_cast<uint4*>(T + p0 + 4) = make_uint4(outw[4], outw[5], outw[6], outw[7]);
-                reinterpret_cast<uint32_t*>(code)[p0 >> 2] = cod[0] | (cod[1] << 8) | (cod[2] << 16) | (cod[3] << 24);
-                reinterpret_cast<uint32_t*>(code)[(p0 >> 2) + 1] = cod[4] | (cod[5] << 8) | (cod[6] << 16) | (cod[7] << 24);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (p0 + k < r1) {
-                        T[p0 + k] = outw[k];
-                        code[p0 + k] = (uint8_t)cod[k];
-                    }
-            }
-        }
-    }
-    // bitmap cleared by the owners of its words (segments are 32-aligned)
-    {
-        const uint32_t st = (uint32_t)tid * S, en = min(st + S, n);
-        for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) bm[w] = 0;
-    }
-    __syncthreads();
-
-    // ---- B: speculative greedy parse (lz77.cpp:378-406) of each segment.
-    const uint32_t NS = (n + S - 1) / S;
-    if ((uint32_t)tid < NS) {
-        uint32_t pos = (uint32_t)tid * S;
-        const uint32_t end = min(pos + S, n);
-        while (pos < end) {
-            bm[pos >> 5] |= 1u << (pos & 31);
-            pos += step_at(code, T, pos);
-        }
-        exits[tid] = pos;
-    }
-    __syncthreads();
-
-    // ---- R: serial resolve of the segment entries (wave 0).
-    if (wv == 0 && NS > 1) {
-        uint32_t e = exits[0];
-        uint32_t si = 1;
-        while (si < NS) {
-            if (e == si * S) {
-                const uint32_t j = si + (uint32_t)lane;
-                const bool mm = (j >= NS) || (exits[j] != (j + 1) * S);
-                const uint64_t bmask = ballot(mm);
-                if (bmask == 0) {
-                    si += 64;
-                    e = si * S;
-                    continue;
-                }
-                const uint32_t jj = si + (uint32_t)__builtin_ctzll(bmask);
-                if (jj >= NS) break;
-                e = exits[jj];
-                si = jj + 1;
-                continue;
-            }
-            const uint32_t st = si * S, en = min(st + S, n);
-            if (e >= en) {
-                clear_bits(bm, st, en, lane);
-                ++si;
-                continue;
-            }
-            clear_bits(bm, st, e, lane);
-            uint32_t pos = e, ne;
-            for (;;) {
-                if (bm[pos >> 5] & (1u << (pos & 31))) {
-                    ne = exits[si];
-                    break;
-                }
-                if (lane == 0) bm[pos >> 5] |= 1u << (pos & 31);
-                const uint32_t np = pos + step_at(code, T, pos);
-                clear_bits(bm, pos + 1, min(np, en), lane);
-                if (np >= en) {
-                    ne = np;
-                    break;
-                }
-                pos = np;
-            }
-            e = ne;
-            ++si;
-        }
-    }
-    __syncthreads();
-
-    // ---- C: bit counts, scan, block choice (deflate_block.cpp:414-421), emission.
-    const uint32_t st = (uint32_t)tid * S;
-    const uint32_t en = (uint32_t)tid < NS ? min(st + S, n) : st;
-    uint32_t mybits = 0;
-    {
-        uint32_t pk[8];
-        int cnt = 0;
-        for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) {
-            uint32_t x = bm[w];
-            while (x) {
-                pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
-                x &= x - 1;
-                if (cnt == 8) {
-                    uint32_t tv[8];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) mybits += tok_bits(tv[k]);
-                    cnt = 0;
-                }
-            }
-        }
-        for (int k = 0; k < cnt; ++k) mybits += tok_bits(T[pk[k]]);
-    }
-    const uint32_t incl = wave_incl_scan(mybits, lane);
-    if (lane == 63) misc[wv] = incl;
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < LC_WAVES; ++i) {
-            const uint32_t t = misc[i];
-            misc[i] = acc;
-            acc += t;
-        }
-        misc[LC_WAVES] = acc;
-    }
-    __syncthreads();
-    const uint32_t excl = misc[wv] + incl - mybits;
-    const uint64_t fixed = 3ull + misc[LC_WAVES] + 7ull;
-    const bool is_last = (desc[c].buf & CHUNK_LAST) != 0;
-    const bool stored = (uint64_t)(n + 5) * 8ull <= fixed;
-    if (tid == 0) {
-        info[c].fixed_bits = fixed;
-        info[c].raw_len = n;
-        info[c].is_stored = stored ? 1u : 0u;
-    }
-    if (stored) return;
-    uint32_t* stg = stage + (uint64_t)c * stage_words;
-    const uint32_t nw = (uint32_t)((fixed + 31) >> 5) + 1u;
-    for (uint32_t i = tid; i < nw; i += LC_THREADS) stg[i] = 0;
-    __syncthreads();
-    if (tid == 0) atomicOr(&stg[0], (is_last ? 1u : 0u) | 2u);  // BFINAL, BTYPE = 01
-    if (mybits) {
-        const uint32_t bitpos = 3u + excl;
-        uint32_t wi = bitpos >> 5;
-        uint32_t nacc = bitpos & 31u;
-        bool partial = nacc != 0;
-        uint64_t acc = 0;
-        auto put = [&](uint32_t tkn) {
-            uint32_t v, len;
-            const uint32_t l = tok_len(tkn);
-            if (l) fixed_match(l, tok_dist(tkn), v, len);
-            else fixed_lit(tkn & 0xFFu, v, len);
-            acc |= (uint64_t)v << nacc;
-            nacc += len;
-            if (nacc >= 32) {
-                if (partial) {
-                    atomicOr(&stg[wi], (uint32_t)acc);
-                    partial = false;
-                } else {
-                    stg[wi] = (uint32_t)acc;
-                }
-                ++wi;
-                acc >>= 32;
-                nacc -= 32;
-            }
-        };
-        uint32_t pk[8];
-        int cnt = 0;
-        for (uint32_t w = st >> 5; w < ((en + 31) >> 5); ++w) {
-            uint32_t x = bm[w];
-            while (x) {
-                pk[cnt++] = (w << 5) + (uint32_t)__builtin_ctz(x);
-                x &= x - 1;
-                if (cnt == 8) {
-                    uint32_t tv[8];
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) tv[k] = T[pk[k]];
-                    for (int k = 0; k < 8; ++k) put(tv[k]);
-                    cnt = 0;
-                }
-            }
-        }
-        for (int k = 0; k < cnt; ++k) put(T[pk[k]]);
-        if (nacc) atomicOr(&stg[wi], (uint32_t)acc);
-    }
-}
-
-template <int HB, bool EXCH>
-__global__ __launch_bounds__(LC_THREADS) void l1_chunk_kernel(const uint8_t* __restrict__ src,
-    const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
-    uint32_t chunk, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
-    ChunkInfo* __restrict__ info) {
-    extern __shared__ uint32_t smem[];
-    for (;;) {
-        const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
-        if (k == NO_ITEM) break;
-        l1_chunk_body<HB, EXCH>(order[k], smem, src, desc, chunk, tokw, stage, stage_words, info);
-        __syncthreads();
-    }
-}
-
 // ---------------------------------------------------------------------------
-// L1-class path, 12-bit hashes, chunks up to 128 KiB (orot's default L1
-// chunk): two launches per batch.
-//   l1_candidate_kernel, one 128-thread workgroup per chunk, 40 KiB of LDS
-//   (four chunks resident per CU):
-//   A1  each wave scans one half of the chunk's hash positions in order: the
-//       latest earlier same-hash position C of its half, one LDS exchange per
-//       position (EXCH: same-address exchanges of one wave instruction take
-//       effect in lane order, checked at start-up by exchange_order_kernel)
-//       or a ballot match-any.  A position of the second half with no earlier
-//       same-hash position in that half takes the last one of the first half
-//       (the first half's final table).  Output per position, 16 bits:
-//       cand_code(p, C).
+// L1-class path, 12-bit (level 1) or 14-bit (levels 2-3) hashes, chunks up
+// to 128 KiB (orot's defaults): two launches per batch.
+//   l1_candidate_kernel<HB, W>, one W-wave workgroup per chunk (12 bits:
+//   W = 2 with four chunks per CU, or W = 6 for small batches; 14 bits:
+//   W = 2, one chunk per CU):
+//   A1  each wave scans one of W sub-ranges of the chunk's hash positions in
+//       order: the latest earlier same-hash position C in its sub-range, one
+//       LDS exchange per position (EXCH: same-address exchanges of one wave
+//       instruction take effect in lane order, checked at start-up by
+//       exchange_order_kernel) or a ballot match-any.  A hash's first
+//       position in a later sub-range takes its latest position in the
+//       earlier ones.  Output per position, 16 bits: cand_code(p, C).
 //   l1_barrier_kernel, one 1024-thread workgroup per chunk:
 //   0   the chunk's bytes are staged in LDS.
 //   A2  per position: candidate distance from its code (far candidates:
@@ -1090,7 +637,6 @@ constexpr uint32_t LB_FB_OFF = LB_BM_OFF + LC_BM_WORDS;       // first barrier p
 constexpr uint32_t LB_MISC_OFF = LB_FB_OFF + LC_THREADS;
 constexpr uint32_t LB_LDS_BYTES = (LB_MISC_OFF + 64u) * 4u;
 
-constexpr uint32_t LA_TAB = 1u << 12;
 constexpr uint16_t LA_NONE = 0xFFFFu;
 
 // Ranges and token-word layout of the barrier path.  The chunk is cut into R
@@ -1133,21 +679,22 @@ __device__ __forceinline__ uint32_t cand_code(uint32_t p, uint32_t C1) {
     return dd ? (0x8000u | dd) : 0u;
 }
 
-// LDS of the candidate kernel with W waves: a table per sub-range (latest
-// position + 1 of each hash) and, for sub-ranges 1.., the offset of each
-// hash's first position.
-template <int W>
-constexpr size_t la_lds_bytes() { return (size_t)W * LA_TAB * 4u + (size_t)(W - 1) * LA_TAB * 2u; }
+// LDS of the candidate kernel with HB-bit hashes and W waves: a table per
+// sub-range (latest position + 1 of each hash) and, for sub-ranges 1.., the
+// offset of each hash's first position.  12 bits: 40 KiB at W = 2, 136 KiB
+// at W = 6; 14 bits (levels 2-3): 160 KiB at W = 2.
+template <int HB, int W>
+constexpr size_t la_lds_bytes() { return (size_t)W * (4u << HB) + (size_t)(W - 1) * (2u << HB); }
 
-template <int W, bool EXCH>
+template <int HB, int W, bool EXCH>
 __global__ __launch_bounds__(64 * W) void l1_candidate_kernel(const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, const BatchHdr* __restrict__ hdr, uint32_t chunk,
     uint16_t* __restrict__ cand) {
-    constexpr int HB = 12;
+    constexpr uint32_t LA_TAB = 1u << HB;
     constexpr int NT = 64 * W;
     extern __shared__ uint32_t la_smem[];
     uint32_t* tab = la_smem;                                         // W x 4096
-    uint16_t* first = reinterpret_cast<uint16_t*>(la_smem + W * LA_TAB);  // (W - 1) x 4096
+    uint16_t* first = reinterpret_cast<uint16_t*>(la_smem + W * LA_TAB);  // (W - 1) x LA_TAB
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t nchunks = hdr->nchunks;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -1300,6 +847,7 @@ __device__ __forceinline__ uint32_t code_step(uint32_t b, const uint32_t* T, con
 // Fixed-Huffman bit count of the tokens whose start bit is set in
 // bm over [a, b); T read 8 tokens at a time.
 
+template <int HB>
 __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, uint32_t chunk, const uint16_t* __restrict__ cand,
     uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
@@ -1379,10 +927,10 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                     } else {
                         // the nearest same-hash position among p - dd - 32768 k
                         const uint32_t dd = e & 0x7FFFu;
-                        const uint32_t hp = hash4<12>((uint32_t)(win_u64(w0, w1, k) & 0xFFFFFFFFull));
+                        const uint32_t hp = hash4<HB>((uint32_t)(win_u64(w0, w1, k) & 0xFFFFFFFFull));
                         for (uint32_t back = dd + 32768u; back <= p; back += 32768u) {
                             const uint32_t C = p - back;
-                            if (hash4<12>(lds_u32(D, C)) == hp) {
+                            if (hash4<HB>(lds_u32(D, C)) == hp) {
                                 if (lds_u8(D, C) == lit) d = dd;
                                 break;
                             }
@@ -1631,6 +1179,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     for (uint32_t i = tid; i < nw; i += LC_THREADS) stg[i] = out[i];
 }
 
+template <int HB>
 __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
     uint32_t chunk, const uint16_t* __restrict__ cand, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage,
@@ -1639,7 +1188,7 @@ __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* _
     for (;;) {
         const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
         if (k == NO_ITEM) break;
-        l1_barrier_body(order[k], smem, src, desc, chunk, cand, tokw, stage, stage_words, info);
+        l1_barrier_body<HB>(order[k], smem, src, desc, chunk, cand, tokw, stage, stage_words, info);
         __syncthreads();
     }
 }
@@ -2021,30 +1570,29 @@ hipError_t allow_lds(K* kernel, size_t smem) {
                                (int)smem);
 }
 
-template <int HB, bool EXCH>
-hipError_t launch_chunk(const Plan& p) {
-    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, LC_LDS_BYTES, LC_THREADS));
-    hipLaunchKernelGGL((l1_chunk_kernel<HB, EXCH>), dim3(g), dim3(LC_THREADS), LC_LDS_BYTES, p.stream, p.d_in,
-                       p.d_desc, p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_stage, p.stage_words, p.d_info);
-    return hipSuccess;
-}
-
-template <int W, bool EXCH>
+template <int HB, int W, bool EXCH>
 void launch_candidates(const Plan& p) {
-    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, la_lds_bytes<W>(), 64 * W));
-    hipLaunchKernelGGL((l1_candidate_kernel<W, EXCH>), dim3(g), dim3(64 * W), la_lds_bytes<W>(), p.stream, p.d_in,
-                       p.d_desc, p.d_hdr, p.chunk, p.d_pd);
+    constexpr size_t smem = la_lds_bytes<HB, W>();
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, smem, 64 * W));
+    hipLaunchKernelGGL((l1_candidate_kernel<HB, W, EXCH>), dim3(g), dim3(64 * W), smem, p.stream, p.d_in, p.d_desc,
+                       p.d_hdr, p.chunk, p.d_pd);
 }
 
-template <bool EXCH>
+template <int HB, bool EXCH>
 hipError_t launch_barrier(const Plan& p) {
-    // Few chunks: six waves per chunk, one chunk per CU, for the shortest
-    // scan per chunk.  Many: two waves per chunk, four chunks per CU, where
-    // the scan is bound by its memory traffic.  Both give the same codes.
-    if (p.nchunks <= 2u * (uint32_t)p.cus) launch_candidates<6, EXCH>(p);
-    else launch_candidates<2, EXCH>(p);
+    if (HB == 14) {
+        // 64 KiB tables: two waves per chunk, one chunk per CU
+        launch_candidates<14, 2, EXCH>(p);
+    } else {
+        // Few chunks: six waves per chunk, one chunk per CU, for the shortest
+        // scan per chunk.  Many: two waves per chunk, four chunks per CU,
+        // where the scan is bound by its memory traffic.  Both give the same
+        // codes.
+        if (p.nchunks <= 2u * (uint32_t)p.cus) launch_candidates<12, 6, EXCH>(p);
+        else launch_candidates<12, 2, EXCH>(p);
+    }
     const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, LB_LDS_BYTES, LC_THREADS));
-    hipLaunchKernelGGL(l1_barrier_kernel, dim3(g), dim3(LC_THREADS), LB_LDS_BYTES, p.stream, p.d_in, p.d_desc,
+    hipLaunchKernelGGL((l1_barrier_kernel<HB>), dim3(g), dim3(LC_THREADS), LB_LDS_BYTES, p.stream, p.d_in, p.d_desc,
                        p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.d_stage, p.stage_words, p.d_info);
     return hipSuccess;
 }
@@ -2060,12 +1608,12 @@ hipError_t launch_wide(const Plan& p) {
 
 hipError_t launch_l1_class(const Plan& p) {
     if (p.hash_bits == 12) {
-        if (p.chunk <= LC_MAX_CHUNK) return p.exch_ok ? launch_barrier<true>(p) : launch_barrier<false>(p);
+        if (p.chunk <= LC_MAX_CHUNK) return p.exch_ok ? launch_barrier<12, true>(p) : launch_barrier<12, false>(p);
         if (p.chunk <= MAX_CHUNK_L1) return launch_wide<12>(p);
         return hipErrorInvalidValue;
     }
     if (p.hash_bits == 14 && p.chunk <= LC_MAX_CHUNK)
-        return p.exch_ok ? launch_chunk<14, true>(p) : launch_chunk<14, false>(p);
+        return p.exch_ok ? launch_barrier<14, true>(p) : launch_barrier<14, false>(p);
     return hipErrorInvalidValue;
 }
 
@@ -2091,13 +1639,16 @@ hipError_t check_exchange_order(hipStream_t stream, bool* ok) {
 // LDS limits of the L1-class kernels; set once per process before any launch
 // (never while a stream is being captured).
 hipError_t set_kernel_attributes_l1(size_t lds_max) {
-    hipError_t e = allow_lds(&l1_chunk_kernel<14, true>, LC_LDS_BYTES);
-    if (e == hipSuccess) e = allow_lds(&l1_chunk_kernel<14, false>, LC_LDS_BYTES);
-    if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel, LB_LDS_BYTES);
-    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<2, true>, la_lds_bytes<2>());
-    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<2, false>, la_lds_bytes<2>());
-    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<6, true>, la_lds_bytes<6>());
-    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<6, false>, la_lds_bytes<6>());
+    hipError_t e = allow_lds(&l1_barrier_kernel<12>, LB_LDS_BYTES);
+    if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel<14>, LB_LDS_BYTES);
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 2, true>, la_lds_bytes<12, 2>());
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 2, false>, la_lds_bytes<12, 2>());
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 6, true>, la_lds_bytes<12, 6>());
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 6, false>, la_lds_bytes<12, 6>());
+    if (e == hipSuccess && lds_max >= la_lds_bytes<14, 2>())
+        e = allow_lds(&l1_candidate_kernel<14, 2, true>, la_lds_bytes<14, 2>());
+    if (e == hipSuccess && lds_max >= la_lds_bytes<14, 2>())
+        e = allow_lds(&l1_candidate_kernel<14, 2, false>, la_lds_bytes<14, 2>());
     const size_t wide = L1Cfg<12>::TAB_BYTES > L1_PHASE_B_BYTES ? L1Cfg<12>::TAB_BYTES : L1_PHASE_B_BYTES;
     if (e == hipSuccess) e = allow_lds(&l1_lz77_kernel<12>, wide);
     return e;
