@@ -622,7 +622,8 @@ __device__ __forceinline__ uint32_t len_code8(uint32_t m1) {
 //       (lz77.cpp:163-191), the verified extent carried along a diagonal,
 //       all compares in LDS; T[p] = literal | length << 8 | distance << 17.
 //   K   the LDS bytes are replaced by u8 length codes (0 literal,
-//       1..253 = len - 2, 254 = 258, 255 = 256/257 read back from T).
+//       1..253 = len - 2, 254 = 258, 255 = 256/257 read back from T), which
+//       A2 left in each group's slot of the candidate codes.
 //   P   parse barriers: q is a barrier when no token starting before q ends
 //       after q (max over p < q of p + step(p) == q); every greedy parse
 //       visits every barrier.  Lane j parses exactly from the first barrier
@@ -849,13 +850,14 @@ __device__ __forceinline__ uint32_t code_step(uint32_t b, const uint32_t* T, con
 
 template <int HB>
 __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
-    const ChunkDesc* __restrict__ desc, uint32_t chunk, const uint16_t* __restrict__ cand,
+    const ChunkDesc* __restrict__ desc, uint32_t chunk, uint16_t* __restrict__ cand,
     uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
     ChunkInfo* __restrict__ info) {
     const uint32_t n = desc[c].len;
     const uint8_t* s = src + desc[c].in_off;
-    // candidate codes from l1_candidate_kernel and token words, LbLayout order
-    const uint16_t* Q = cand + (uint64_t)c * lb_tok_stride(chunk);
+    // candidate codes from l1_candidate_kernel (then, per group, the group's
+    // length codes) and token words, LbLayout order
+    uint16_t* Q = cand + (uint64_t)c * lb_tok_stride(chunk);
     uint32_t* T = tokw + (uint64_t)c * lb_tok_stride(chunk);
     const LbLayout L(n);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -974,20 +976,21 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 for (int k = 0; k < 8; ++k)
                     if (p0 + k < r1) g[k] = outw[k];
             }
+            // the group's u8 length codes over its (already read) candidate
+            // codes, for K
+            uint32_t cw[2] = {0, 0};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) cw[k >> 2] |= len_code8(tok_len(outw[k])) << (8 * (k & 3));
+            *reinterpret_cast<uint2*>(Q + L.group((uint32_t)tid, (p0 - r0) >> 3)) = make_uint2(cw[0], cw[1]);
         }
     }
     __syncthreads();
 
-    // ---- K: length codes over the bytes; bitmap cleared (lane-coalesced).
-    for (uint32_t p0 = (uint32_t)tid * 4u; p0 < n; p0 += LC_THREADS * 4u) {
-        if (p0 + 4 <= n) {  // four positions of one group (p0 is a multiple of 4)
-            const uint4 v = *reinterpret_cast<const uint4*>(T + L.index(p0));
-            D[p0 >> 2] = len_code8(tok_len(v.x)) | (len_code8(tok_len(v.y)) << 8) | (len_code8(tok_len(v.z)) << 16) |
-                         (len_code8(tok_len(v.w)) << 24);
-        } else {
-            for (uint32_t p = p0; p < n; ++p) code[p] = (uint8_t)len_code8(tok_len(T[L.index(p)]));
-        }
-    }
+    // ---- K: the length codes A2 left per group, to LDS over the bytes, 8 at
+    // a time in range order (codes of a last group's positions past n land
+    // below LC_REGION0 and are never read); bitmap cleared.
+    for (uint32_t p0 = r0; p0 < r1; p0 += 8)
+        *reinterpret_cast<uint2*>(code + p0) = *reinterpret_cast<const uint2*>(Q + L.group((uint32_t)tid, (p0 - r0) >> 3));
     for (uint32_t w = tid; w < ((n + 31) >> 5); w += LC_THREADS) bm[w] = 0;
     __syncthreads();
 
@@ -1182,7 +1185,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
 template <int HB>
 __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
-    uint32_t chunk, const uint16_t* __restrict__ cand, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage,
+    uint32_t chunk, uint16_t* __restrict__ cand, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage,
     uint64_t stage_words, ChunkInfo* __restrict__ info) {
     extern __shared__ uint32_t smem[];
     for (;;) {
