@@ -1073,40 +1073,55 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         const uint32_t wv32 = k == 0 ? bw0 : (k == 1 ? bw1 : (k == 2 ? bw2 : (k == 3 ? bw3 : bw4)));
         return (wv32 >> (p0 & 31)) & 0xFFu;
     };
-    auto load_group = [&](uint32_t p0, uint32_t* tv) {
-        const uint32_t* g = tg(p0);
-        if (p0 + 8 <= r1) {
-            const uint4 v0 = *reinterpret_cast<const uint4*>(g);
-            const uint4 v1 = *reinterpret_cast<const uint4*>(g + 4);
-            tv[0] = v0.x; tv[1] = v0.y; tv[2] = v0.z; tv[3] = v0.w;
-            tv[4] = v1.x; tv[5] = v1.y; tv[6] = v1.z; tv[7] = v1.w;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) tv[j] = p0 + j < r1 ? g[j] : 0u;
-        }
-    };
     auto next_group = [&](uint32_t p0) -> uint32_t {  // first group at or after p0 with a token start
         while (p0 < r1 && group_mask(p0) == 0) p0 += 8;
         return p0;
     };
-    uint32_t mybits = 0;
-    {
-        uint32_t g = nwd ? next_group(r0) : r1;
-        uint32_t nx[8];
-        if (g < r1) load_group(g, nx);
-        while (g < r1) {
-            uint32_t tv[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) tv[j] = nx[j];
-            const uint32_t m = group_mask(g);
-            const uint32_t gn = next_group(g + 8);
-            if (gn < r1) load_group(gn, nx);
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if ((m >> j) & 1u) mybits += tok_bits(tv[j]);
-            g = gn;
+    // The groups with token starts, in order, with the token words of two
+    // more in flight.  Every lane loads whole 8-word slots (a group past n
+    // reads slot words whose positions have no start bit) and a lane with
+    // no such group loads slot 0, so the loads carry no branch and the
+    // waits for them are exact.  Unrolled by three over fixed buffers.
+    auto stream = [&](auto&& body) {
+        auto load8 = [&](uint32_t g, uint32_t* v) {
+            const uint32_t* a = g < r1 ? tg(g) : T;
+            const uint4 x = *reinterpret_cast<const uint4*>(a), y = *reinterpret_cast<const uint4*>(a + 4);
+            v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+            v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+        };
+        uint32_t A[8], B[8], C[8];
+        uint32_t ga = nwd ? next_group(r0) : r1;
+        uint32_t gb = next_group(ga + 8);
+        load8(ga, A);
+        load8(gb, B);
+        for (;;) {
+            if (ga >= r1) break;
+            uint32_t gc = next_group(gb + 8);
+            load8(gc, C);
+            body(ga, A);
+            ga = gb;
+            gb = gc;
+            if (ga >= r1) break;
+            gc = next_group(gb + 8);
+            load8(gc, A);
+            body(ga, B);
+            ga = gb;
+            gb = gc;
+            if (ga >= r1) break;
+            gc = next_group(gb + 8);
+            load8(gc, B);
+            body(ga, C);
+            ga = gb;
+            gb = gc;
         }
-    }
+    };
+    uint32_t mybits = 0;
+    stream([&](uint32_t g, const uint32_t* tv) {
+        const uint32_t m = group_mask(g);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if ((m >> j) & 1u) mybits += tok_bits(tv[j]);
+    });
     const uint32_t incl = wave_incl_scan(mybits, lane);
     if (lane == 63) misc[wv] = incl;
     __syncthreads();
@@ -1161,20 +1176,11 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 nacc -= 32;
             }
         };
-        uint32_t g = next_group(r0);
-        uint32_t nx[8];
-        if (g < r1) load_group(g, nx);
-        while (g < r1) {
-            uint32_t tv[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) tv[j] = nx[j];
+        stream([&](uint32_t g, const uint32_t* tv) {
             const uint32_t m = group_mask(g);
-            const uint32_t gn = next_group(g + 8);
-            if (gn < r1) load_group(gn, nx);
             for (int j = 0; j < 8; ++j)
                 if ((m >> j) & 1u) put(tv[j]);
-            g = gn;
-        }
+        });
         if (nacc) atomicOr(&out[wi], (uint32_t)acc);
     }
     __syncthreads();
