@@ -347,14 +347,16 @@ __global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __
 // up to lazy_depth literals while M(q + 1) > best + 1, then the match.
 // code[p] = step (next decision - p) | lazy literals << 12.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t l6_decide(const uint32_t* ML, uint32_t n, uint32_t p, int lazy) {
+// m0 = ML[p] and m1 = ML[p + 1] (when p + 1 < n), loaded by the caller.
+__device__ __forceinline__ uint32_t l6_decide(const uint32_t* ML, uint32_t n, uint32_t p, int lazy, uint32_t m0,
+                                              uint32_t m1) {
     if (p + 3 > n) return 1;
-    uint32_t b = ml_len(ML[p]);
+    uint32_t b = ml_len(m0);
     if (b < 3) return 1;
     uint32_t q = p;
     int k = lazy;
     while (k > 0 && q + b + 1 < n && q + 4 <= n) {
-        const uint32_t nb = ml_len(ML[q + 1]);
+        const uint32_t nb = ml_len(q == p ? m1 : ML[q + 1]);
         if (nb > b + 1) {
             ++q;
             b = nb;
@@ -388,14 +390,45 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
     // bitmap words owned by this range (S is a multiple of 32; an empty range
     // owns none, even when r0 falls inside the last range's final word)
     const uint32_t w0 = r0 >> 5, w1 = r0 < r1 ? (r1 + 31) >> 5 : w0;
-    // decisions (coalesced over positions) and bitmap clear
-    for (uint32_t p = tid; p < n; p += L6R_THREADS) CODE[p] = (uint16_t)l6_decide(ML, n, p, lazy);
+    // decisions (coalesced over positions), eight per thread per pass with
+    // their match words loaded first (addresses clamped into the chunk), and
+    // bitmap clear
+    for (uint32_t p0 = tid; p0 < n; p0 += 8 * L6R_THREADS) {
+        uint32_t m0[8], m1[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t p = p0 + (uint32_t)k * L6R_THREADS;
+            m0[k] = ML[min(p, n - 1)];
+            m1[k] = ML[min(p + 1, n - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t p = p0 + (uint32_t)k * L6R_THREADS;
+            if (p < n) CODE[p] = (uint16_t)l6_decide(ML, n, p, lazy, m0[k], m1[k]);
+        }
+    }
     for (uint32_t w = w0; w < w1; ++w) bm[w] = 0;
     __syncthreads();
+    // the range's codes 32 at a time (r0 is a multiple of 32, so the 16-byte
+    // loads are aligned; codes past n read the chunk's slack and are masked)
+    auto codes32 = [&](uint32_t p, uint32_t* w) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint4 v = *reinterpret_cast<const uint4*>(CODE + p + 8 * i);
+            w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+        }
+    };
+    auto code_of = [](const uint32_t* w, int k) -> uint32_t { return (w[k >> 1] >> (16 * (k & 1))) & 0xFFFFu; };
     // first parse barrier per range: max over p < q of (p + step(p)) == q
     {
         uint32_t runmax = 0;
-        for (uint32_t p = r0; p < r1; ++p) runmax = max(runmax, p + (CODE[p] & 0xFFFu));
+        for (uint32_t p0 = r0; p0 < r1; p0 += 32) {
+            uint32_t w[16];
+            codes32(p0, w);
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                if (p0 + k < r1) runmax = max(runmax, p0 + k + (code_of(w, k) & 0xFFFu));
+        }
         uint32_t v = runmax;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -417,12 +450,17 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
         uint32_t carry = max(misc[wv], lane ? prev_lane : 0u);
         if (tid == 0) carry = 0;
         uint32_t first = 0xFFFFFFFFu;
-        for (uint32_t p = r0; p < r1; ++p) {
-            if (carry <= p) {
-                first = p;
-                break;
+        for (uint32_t p0 = r0; p0 < r1 && first == 0xFFFFFFFFu; p0 += 32) {
+            uint32_t w[16];
+            codes32(p0, w);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                const uint32_t p = p0 + k;
+                if (p < r1 && first == 0xFFFFFFFFu) {
+                    if (carry <= p) first = p;
+                    else carry = max(carry, p + (code_of(w, k) & 0xFFFu));
+                }
             }
-            carry = max(carry, p + (CODE[p] & 0xFFFu));
         }
         fb[tid] = r0 < r1 ? first : 0xFFFFFFFFu;
     }
@@ -444,11 +482,13 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
     uint32_t cnt = 0;
     for (uint32_t w = w0; w < w1; ++w) {
         uint32_t x = bm[w];
+        if (!x) continue;
+        uint32_t cw[16];
+        codes32(w << 5, cw);
         while (x) {
-            const uint32_t p = (w << 5) + (uint32_t)__builtin_ctz(x);
+            const int k = __builtin_ctz(x);
             x &= x - 1;
-            const uint32_t cd = CODE[p];
-            cnt += 1u + (cd >> 12);
+            cnt += 1u + (code_of(cw, k) >> 12);
         }
     }
     const uint32_t incl = wave_incl_scan(cnt, lane);
