@@ -507,21 +507,46 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
     uint32_t ti = misc[wv] + incl - cnt;
     for (uint32_t w = w0; w < w1; ++w) {
         uint32_t x = bm[w];
+        if (!x) continue;
+        // the word's codes, and its 32 input bytes plus 8 more (a token's
+        // lazy literals may run past the word), in registers: a token then
+        // loads only the match word of a match
+        const uint32_t pw = w << 5;
+        uint32_t cw[16];
+        codes32(pw, cw);
+        uint64_t sb[5];
+        if (pw + 40 <= n) {
+#pragma unroll
+            for (int i = 0; i < 5; ++i) sb[i] = ld64u(s + pw + 8 * i);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                uint64_t v = 0;
+                for (uint32_t k = 0; k < 8; ++k)
+                    if (pw + 8 * i + k < n) v |= (uint64_t)s[pw + 8 * i + k] << (8 * k);
+                sb[i] = v;
+            }
+        }
+        auto byte_at = [&](uint32_t o) -> uint32_t {  // o < 40
+            const uint64_t v = o < 8 ? sb[0] : (o < 16 ? sb[1] : (o < 24 ? sb[2] : (o < 32 ? sb[3] : sb[4])));
+            return (uint32_t)(v >> (8 * (o & 7))) & 0xFFu;
+        };
         while (x) {
-            const uint32_t p = (w << 5) + (uint32_t)__builtin_ctz(x);
+            const int kk = __builtin_ctz(x);
             x &= x - 1;
-            const uint32_t cd = CODE[p];
+            const uint32_t o = (uint32_t)kk, p = pw + o;
+            const uint32_t cd = code_of(cw, kk);
             const uint32_t lits = cd >> 12, step = cd & 0xFFFu;
             for (uint32_t k = 0; k < lits; ++k) {
                 if ((ti % BLK_SYMS) == 0 && ti / BLK_SYMS < maxblk) blkpos[(uint64_t)c * maxblk + ti / BLK_SYMS] = p + k;
-                TK[ti++] = s[p + k];
+                TK[ti++] = byte_at(o + k);
             }
             if ((ti % BLK_SYMS) == 0 && ti / BLK_SYMS < maxblk) blkpos[(uint64_t)c * maxblk + ti / BLK_SYMS] = p + lits;
             if (step > lits + 1) {
                 const uint32_t m = ML[p + lits];
                 TK[ti++] = 0x80000000u | (ml_len(m) << 16) | ml_dist(m);
             } else {
-                TK[ti++] = s[p + lits];
+                TK[ti++] = byte_at(o + lits);
             }
         }
     }
