@@ -96,10 +96,14 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const u
         uint32_t* tab = smem + (uint32_t)wv * L6_TAB;
         for (uint32_t i = lane; i < L6_TAB; i += 64) tab[i] = 0;
         const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
-        for (uint32_t p0 = a; p0 < b; p0 += 512) {
-            uint32_t wd[8];
+        // Window loads of three more steps stay in flight, and every lane
+        // stores (lanes past the sub-range on position n - 1, which the
+        // resolution below never reads from T), so the waits are exact.
+        auto load8 = [&](uint32_t q0, uint32_t* w) {
 #pragma unroll
-            for (int j = 0; j < 8; ++j) wd[j] = ld32u(s + min(p0 + 64u * j + (uint32_t)lane, b - 1));
+            for (int j = 0; j < 8; ++j) w[j] = ld32u(s + min(q0 + 64u * j + (uint32_t)lane, b - 1));
+        };
+        auto step = [&](uint32_t p0, const uint32_t* wd) {
             uint32_t res[8];
             if (EXCH) {
 #pragma unroll
@@ -133,8 +137,27 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const u
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const uint32_t p = p0 + 64u * j + (uint32_t)lane;
-                if (p < b) T[p] = res[j];
+                T[p < b ? p : n - 1] = res[j];
             }
+        };
+        uint32_t w0[8], w1[8], w2[8], w3[8];
+        if (a < b) {
+            load8(a, w0);
+            load8(a + 512u, w1);
+            load8(a + 1024u, w2);
+        }
+        for (uint32_t p0 = a; p0 < b; p0 += 2048u) {
+            load8(p0 + 1536u, w3);
+            step(p0, w0);
+            if (p0 + 512u >= b) break;
+            load8(p0 + 2048u, w0);
+            step(p0 + 512u, w1);
+            if (p0 + 1024u >= b) break;
+            load8(p0 + 2560u, w1);
+            step(p0 + 1024u, w2);
+            if (p0 + 1536u >= b) break;
+            load8(p0 + 3072u, w2);
+            step(p0 + 1536u, w3);
         }
     }
     __syncthreads();
