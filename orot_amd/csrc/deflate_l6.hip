@@ -947,8 +947,12 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
     __shared__ uint8_t lls[288];
     __shared__ uint8_t dls[32];
     __shared__ uint32_t wsum[L6E_THREADS / 64 + 1];
+    __shared__ uint32_t hsum[L6E_THREADS / 64];
+    __shared__ uint16_t scnt[6][16];
+    __shared__ uint16_t sbase[6][16];
+    __shared__ uint8_t scl[19];
+    __shared__ uint8_t sclc[19];
     __shared__ uint64_t s_off;
-    __shared__ uint64_t s_sym;
     if (res[desc[c].buf & ~CHUNK_LAST].total_bytes == 0) return;
     const uint32_t ntok = ntok_in[c];
     const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
@@ -997,65 +1001,126 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
         }
         return;
     }
-    // fixed or dynamic: code tables
-    if (tid == 0) {
-        for (int i = 0; i < 288; ++i) lls[i] = R.type == 1 ? (uint8_t)fixed_ll_len(i) : R.ll[i];
-        for (int i = 0; i < 32; ++i) dls[i] = R.type == 1 ? 5 : R.dl[i];
-        // canonical codes (RFC 1951 3.2.2), bit-reversed
-        for (int tbl = 0; tbl < 2; ++tbl) {
-            const uint8_t* ln = tbl ? dls : lls;
-            uint16_t* cd = tbl ? dlc : llc;
-            const int ns = tbl ? 32 : 288;
-            uint32_t blc[16] = {0}, nxt[16] = {0};
-            for (int i = 0; i < ns; ++i)
-                if (ln[i]) blc[ln[i]]++;
-            uint32_t code = 0;
-            for (int bb = 1; bb <= 15; ++bb) {
-                code = (code + blc[bb - 1]) << 1;
-                nxt[bb] = code;
-            }
-            for (int i = 0; i < ns; ++i) cd[i] = ln[i] ? (uint16_t)rev_code(nxt[ln[i]]++, ln[i]) : 0;
-        }
-        uint64_t at = off;
-        put_bits_at(stg, at, (last ? 1u : 0u) | (R.type << 1), 3);
-        at += 3;
-        if (R.type == 2) {
-            put_bits_at(stg, at, R.hlit - 257, 5);
-            put_bits_at(stg, at + 5, R.hdist - 1, 5);
-            put_bits_at(stg, at + 10, R.hclen - 4, 4);
-            at += 14;
-            for (uint32_t i = 0; i < R.hclen; ++i) {
-                put_bits_at(stg, at, R.cl[c_cl_order[i]], 3);
-                at += 3;
-            }
-            uint32_t clb[16] = {0}, cln[16] = {0}, clc[19];
-            for (int i = 0; i < 19; ++i)
-                if (R.cl[i]) clb[R.cl[i]]++;
-            uint32_t code = 0;
-            for (int bb = 1; bb <= 15; ++bb) {
-                code = (code + clb[bb - 1]) << 1;
-                cln[bb] = code;
-            }
-            for (int i = 0; i < 19; ++i) clc[i] = R.cl[i] ? rev_code(cln[R.cl[i]]++, R.cl[i]) : 0u;
-            for (uint32_t i = 0; i < R.nrle; ++i) {
-                const uint32_t sy = R.rle_sym[i];
-                put_bits_at(stg, at, clc[sy], R.cl[sy]);
-                at += R.cl[sy];
-                const uint32_t xb = sy == 16 ? 2u : (sy == 17 ? 3u : (sy == 18 ? 7u : 0u));
-                if (xb) {
-                    put_bits_at(stg, at, R.rle_x[i], xb);
-                    at += xb;
-                }
+    // fixed or dynamic: code lengths, one symbol per lane.  Thread tid holds
+    // literal/length symbol tid (slot a); wave 0 also holds symbols 256..287
+    // in lanes 0..31 and distance symbols 0..31 in lanes 32..63 (slot b).
+    static_assert(L6E_THREADS == 256, "one literal/length symbol per thread");
+    const bool dyn = R.type == 2;
+    const uint32_t la = dyn ? (uint32_t)R.ll[tid] : fixed_ll_len((uint32_t)tid);
+    uint32_t lb = 0;
+    if (wv == 0)
+        lb = lane < 32 ? (dyn ? (uint32_t)R.ll[256 + lane] : fixed_ll_len(256u + (uint32_t)lane))
+                       : (dyn ? (uint32_t)R.dl[lane - 32] : 5u);
+    // canonical codes (RFC 1951 3.2.2): a symbol's code is the first code of
+    // its length plus the number of lower symbols of that length.  Segments
+    // 0..3 are waves 0..3 of slot a, 4 and 5 the two halves of slot b.
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t ra = 0, rb = 0;
+    for (uint32_t L = 1; L <= 15; ++L) {
+        const uint64_t ma = ballot(la == L);
+        if (la == L) ra = (uint32_t)__popcll(ma & lt);
+        if (lane == (int)L) scnt[wv][L] = (uint16_t)__popcll(ma);
+        if (wv == 0) {
+            const uint64_t mb = ballot(lb == L);
+            if (lb == L) rb = (uint32_t)__popcll(mb & lt & (lane < 32 ? 0xFFFFFFFFull : ~0xFFFFFFFFull));
+            if (lane == (int)L) {
+                scnt[4][L] = (uint16_t)__popcll(mb & 0xFFFFFFFFull);
+                scnt[5][L] = (uint16_t)__popcll(mb >> 32);
             }
         }
-        s_sym = at;
+    }
+    if (dyn && wv == 1) {
+        // code length code (lengths <= 7), one symbol per lane
+        const uint32_t ln = lane < 19 ? (uint32_t)R.cl[lane] : 0u;
+        uint32_t code = 0, prev = 0, mine = 0;
+        for (uint32_t L = 1; L <= 7; ++L) {
+            const uint64_t m = ballot(ln == L);
+            code = (code + prev) << 1;
+            if (ln == L) mine = code + (uint32_t)__popcll(m & lt);
+            prev = (uint32_t)__popcll(m);
+        }
+        if (lane < 19) {
+            scl[lane] = (uint8_t)ln;
+            sclc[lane] = (uint8_t)rev_code(mine, ln);
+        }
     }
     __syncthreads();
-    const uint64_t sym0 = s_sym;
-    // symbols: each thread a contiguous slice of the block's tokens
+    if (tid < 2) {
+        // first code of each length per segment: thread 0 the literal/length
+        // table (segments 0..4 in symbol order), thread 1 the distance table
+        const int s0 = tid ? 5 : 0, s1 = tid ? 6 : 5;
+        uint32_t code = 0, prev = 0;
+        for (int L = 1; L <= 15; ++L) {
+            code = (code + prev) << 1;
+            uint32_t run = code;
+            for (int sg = s0; sg < s1; ++sg) {
+                sbase[sg][L] = (uint16_t)run;
+                run += scnt[sg][L];
+            }
+            prev = run - code;
+        }
+    }
+    // code-length RLE items (at most 316), two per thread, placed by a scan
+    uint32_t hv0 = 0, hn0 = 0, hv1 = 0, hn1 = 0, hincl = 0;
+    if (dyn) {
+        const uint32_t nr = R.nrle;
+        auto item = [&](uint32_t i, uint32_t& v, uint32_t& n) {
+            if (i >= nr) return;
+            const uint32_t sy = R.rle_sym[i];
+            const uint32_t xb = sy == 16 ? 2u : (sy == 17 ? 3u : (sy == 18 ? 7u : 0u));
+            n = scl[sy] + xb;
+            v = sclc[sy] | (xb ? (uint32_t)R.rle_x[i] << scl[sy] : 0u);
+        };
+        item(2u * (uint32_t)tid, hv0, hn0);
+        item(2u * (uint32_t)tid + 1u, hv1, hn1);
+        hincl = wave_incl_scan(hn0 + hn1, lane);
+        if (lane == 63) hsum[wv] = hincl;
+    }
+    __syncthreads();
+    llc[tid] = la ? (uint16_t)rev_code(sbase[wv][la] + ra, la) : (uint16_t)0;
+    lls[tid] = (uint8_t)la;
+    if (wv == 0) {
+        const uint16_t cb = lb ? (uint16_t)rev_code(sbase[lane < 32 ? 4 : 5][lb] + rb, lb) : (uint16_t)0;
+        if (lane < 32) {
+            llc[256 + lane] = cb;
+            lls[256 + lane] = (uint8_t)lb;
+        } else {
+            dlc[lane - 32] = cb;
+            dls[lane - 32] = (uint8_t)lb;
+        }
+    }
+    if (dyn) {
+        // 3-bit block header, HLIT 5, HDIST 5, HCLEN 4, HCLEN x 3, RLE items
+        uint32_t base = 0;
+        for (int w = 0; w < wv; ++w) base += hsum[w];
+        const uint64_t at = off + 17 + 3ull * R.hclen + base + hincl - (hn0 + hn1);
+        put_bits_at(stg, at, hv0, hn0);
+        put_bits_at(stg, at + hn0, hv1, hn1);
+        if ((uint32_t)tid < R.hclen) put_bits_at(stg, off + 17 + 3ull * tid, scl[c_cl_order[tid]], 3);
+    }
+    if (tid == 0) {
+        const uint32_t h = (last ? 1u : 0u) | (R.type << 1);
+        if (dyn)
+            put_bits_at(stg, off, h | ((R.hlit - 257) << 3) | ((R.hdist - 1) << 8) | ((R.hclen - 4) << 13), 17);
+        else
+            put_bits_at(stg, off, h, 3);
+    }
+    __syncthreads();
+    const uint64_t sym0 = off + 3 + (dyn ? R.hdr_bits : 0u);
+    // symbols: each thread a contiguous slice of at most 64 tokens (a block
+    // holds at most BLK_SYMS), a multiple of 4 long so it starts 16-byte
+    // aligned, held in registers for the count and the emission
+    constexpr int QMAX = BLK_SYMS / L6E_THREADS / 4;
     const uint32_t nt = R.t1 - R.t0;
-    const uint32_t per = (nt + L6E_THREADS - 1) / L6E_THREADS;
+    const uint32_t per = ((nt + L6E_THREADS - 1) / L6E_THREADS + 3u) & ~3u;
     const uint32_t a = R.t0 + min(nt, (uint32_t)tid * per), e = R.t0 + min(nt, (uint32_t)tid * per + per);
+    uint4 q[QMAX];
+    const uint4* T4 = reinterpret_cast<const uint4*>(TK + a);
+#pragma unroll
+    for (int k = 0; k < QMAX; ++k) q[k] = a + 4u * k < e ? T4[k] : make_uint4(0u, 0u, 0u, 0u);
+    auto tok_at = [&](int k, int j) -> uint32_t {
+        return j == 0 ? q[k].x : (j == 1 ? q[k].y : (j == 2 ? q[k].z : q[k].w));
+    };
     // a match takes up to 15 + 5 + 15 + 13 = 48 bits with dynamic codes
     auto tok_code = [&](uint32_t v, uint64_t& code, uint32_t& len) {
         if (v & 0x80000000u) {
@@ -1071,13 +1136,19 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
             len = lls[v & 0xFFu];
         }
     };
+    auto tok_len = [&](uint32_t v) -> uint32_t {
+        if (v & 0x80000000u) {
+            const LenSym ls = len_sym((v >> 16) & 0x1FFu), ds = dist_sym(v & 0xFFFFu);
+            return lls[257 + ls.code] + ls.ebits + dls[ds.code] + ds.ebits;
+        }
+        return lls[v & 0xFFu];
+    };
     uint32_t mybits = 0;
-    for (uint32_t t = a; t < e; ++t) {
-        uint64_t cd;
-        uint32_t ln;
-        tok_code(TK[t], cd, ln);
-        mybits += ln;
-    }
+#pragma unroll
+    for (int k = 0; k < QMAX; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (a + 4u * k + j < e) mybits += tok_len(tok_at(k, j));
     const uint32_t incl = wave_incl_scan(mybits, lane);
     if (lane == 63) wsum[wv] = incl;
     __syncthreads();
@@ -1111,21 +1182,26 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
             acc >>= 32;
             nacc -= 32;
         };
-        for (uint32_t t = a; t < e; ++t) {
-            uint64_t cd;
-            uint32_t ln;
-            tok_code(TK[t], cd, ln);
-            // at most 48 bits: add in two halves so the register never overflows
-            const uint32_t l0 = min(ln, 32u);
-            acc |= (cd & ((1ull << l0) - 1ull)) << nacc;
-            nacc += l0;
-            if (nacc >= 32) flush_word();
-            if (ln > 32) {
-                acc |= (cd >> 32) << nacc;
-                nacc += ln - 32;
-                if (nacc >= 32) flush_word();
+#pragma unroll
+        for (int k = 0; k < QMAX; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (a + 4u * k + j < e) {
+                    uint64_t cd;
+                    uint32_t ln;
+                    tok_code(tok_at(k, j), cd, ln);
+                    // at most 48 bits: add in two halves so the register never overflows
+                    const uint32_t l0 = min(ln, 32u);
+                    acc |= (cd & ((1ull << l0) - 1ull)) << nacc;
+                    nacc += l0;
+                    if (nacc >= 32) flush_word();
+                    if (ln > 32) {
+                        acc |= (cd >> 32) << nacc;
+                        nacc += ln - 32;
+                        if (nacc >= 32) flush_word();
+                    }
+                }
             }
-        }
         if (nacc) atomicOr(&stg[wi], (uint32_t)acc);
     }
     if (tid == 0) put_bits_at(stg, sym0 + wsum[L6E_THREADS / 64], llc[256], lls[256]);  // EOB
