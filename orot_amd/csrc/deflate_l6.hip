@@ -622,15 +622,17 @@ __device__ __forceinline__ uint32_t fixed_ll_len(uint32_t sym) {
 }
 
 // Package-merge (Larmore & Hirschberg), same order as the oracle: symbols
-// sorted by (freq, index), a leaf precedes a package of equal weight.  One
-// thread; idx / wprev / wcur / leafbits are LDS scratch.
+// sorted by (freq, index), a leaf precedes a package of equal weight.  Called
+// by every thread of the workgroup; idx / wleaf / wa / wb are LDS scratch and
+// leafmask holds one bit per merged item of each level, set for leaves.
+constexpr int LM_WORDS = 576 / 32;
 __device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, uint8_t* lens, uint16_t* idx,
-                                uint32_t* wleaf, uint32_t* wa, uint32_t* wb, uint8_t* leafbits /* [16][576] */,
-                                uint32_t* s_cnt) {
-    // Called by every thread of the workgroup.
+                                uint32_t* wleaf, uint32_t* wa, uint32_t* wb, uint32_t* leafmask /* [16][LM_WORDS] */,
+                                uint32_t* s_cnt, uint32_t* s_leaves /* [16] */) {
     const int tid = threadIdx.x;
     if (tid == 0) *s_cnt = 0;
     for (int i = tid; i < nsym; i += L6B_THREADS) lens[i] = 0;
+    for (int i = tid; i < 16 * LM_WORDS; i += L6B_THREADS) leafmask[i] = 0;
     __syncthreads();
     // active symbols in (freq, symbol) order: each one's rank among the active
     for (int i = tid; i < nsym; i += L6B_THREADS) {
@@ -646,7 +648,6 @@ __device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, ui
     }
     __syncthreads();
     const int n = (int)*s_cnt;
-    __syncthreads();
     if (n == 0) return;
     if (n == 1) {
         if (tid == 0) lens[idx[0]] = 1;
@@ -656,7 +657,10 @@ __device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, ui
     for (int i = tid; i < n; i += L6B_THREADS) {
         wleaf[i] = freq[idx[i]];
         wa[i] = wleaf[i];
-        leafbits[max_bits * 576 + i] = 1;
+    }
+    for (int w = tid; w < LM_WORDS; w += L6B_THREADS) {
+        const int lo = 32 * w;
+        leafmask[max_bits * LM_WORDS + w] = n >= lo + 32 ? ~0u : (n > lo ? (1u << (n - lo)) - 1u : 0u);
     }
     __syncthreads();
     // each level: the sorted leaves merged with the sorted packages (pairs of the
@@ -676,7 +680,7 @@ __device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, ui
                 else hi = mid;
             }
             wcur[a + lo] = f;
-            leafbits[l * 576 + a + lo] = 1;
+            atomicOr(&leafmask[l * LM_WORDS + ((a + lo) >> 5)], 1u << ((a + lo) & 31));
         }
         for (int q = tid; q < np; q += L6B_THREADS) {
             const uint32_t pw = wprev[2 * q] + wprev[2 * q + 1];
@@ -687,7 +691,6 @@ __device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, ui
                 else hi = mid;
             }
             wcur[q + lo] = pw;
-            leafbits[l * 576 + q + lo] = 0;
         }
         __syncthreads();
         uint32_t* t = wprev;
@@ -695,20 +698,71 @@ __device__ void limited_lengths(const uint32_t* freq, int nsym, int max_bits, ui
         wcur = t;
         cntp = n + np;
     }
-    // a leaf's length = the number of levels whose selected prefix holds it
-    int take = 2 * n - 2;
-    for (int l = 1; l <= max_bits && take > 0; ++l) {
-        if (tid == 0) *s_cnt = 0;
-        __syncthreads();
-        uint32_t cnt = 0;
-        for (int i = tid; i < take; i += L6B_THREADS) cnt += leafbits[l * 576 + i];
-        if (cnt) atomicAdd(s_cnt, cnt);
-        __syncthreads();
-        const int leaves = (int)*s_cnt;
-        for (int i = tid; i < leaves; i += L6B_THREADS) lens[idx[i]]++;
-        __syncthreads();
-        take = 2 * (take - leaves);
+    // Level l selects its first `take` items; the leaves among them are the
+    // lightest leaves_l, and a leaf's length is the number of levels that
+    // select it.  One wave walks the levels with popcounts of the masks.
+    if (tid < 64) {
+        int take = 2 * n - 2;
+        for (int l = 1; l <= max_bits; ++l) {
+            uint32_t cnt = 0;
+            const int lo = 32 * tid;
+            if (take > 0 && tid < LM_WORDS && lo < take) {
+                uint32_t w = leafmask[l * LM_WORDS + tid];
+                if (take - lo < 32) w &= (1u << (take - lo)) - 1u;
+                cnt = (uint32_t)__popc(w);
+            }
+            for (int d = 32; d; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
+            if (tid == 0) s_leaves[l] = cnt;
+            take = 2 * (take - (int)cnt);
+        }
     }
+    __syncthreads();
+    for (int i = tid; i < n; i += L6B_THREADS) {
+        int len = 0;
+        for (int l = 1; l <= max_bits; ++l) len += i < (int)s_leaves[l] ? 1 : 0;
+        lens[idx[i]] = (uint8_t)len;
+    }
+    __syncthreads();
+}
+
+// Workgroup sum of one value per thread; red is LDS scratch of one word per wave.
+__device__ __forceinline__ uint32_t l6b_sum(uint32_t v, uint32_t* red) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int d = 32; d; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int w = 0; w < L6B_THREADS / 64; ++w) t += red[w];
+    __syncthreads();
+    return t;
+}
+
+// Code-length RLE items of one maximal run of `m` equal lengths `v`: the
+// greedy rule of deflate_block.cpp:347-370 restarted at each item, whose run
+// from any position inside the maximal run is min(rest, 138).  emit(sym, x)
+// takes x = 0xFF for a plain length.
+template <class F>
+__device__ __forceinline__ uint32_t rle_run_items(uint32_t v, uint32_t m, F&& emit) {
+    uint32_t items = 0;
+    while (m) {
+        const uint32_t run = m < 138u ? m : 138u;
+        if (v == 0 && run >= 3) {
+            emit(run <= 10 ? 17u : 18u, run <= 10 ? run - 3 : run - 11);
+            ++items;
+            m -= run;
+        } else if (v != 0 && run >= 4) {
+            const uint32_t rep = run - 1 < 6 ? run - 1 : 6u;
+            emit(v, 0xFFu);
+            emit(16u, rep - 3);
+            items += 2;
+            m -= 1 + rep;
+        } else {
+            emit(v, 0xFFu);
+            ++items;
+            --m;
+        }
+    }
+    return items;
 }
 
 __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint32_t* __restrict__ toks,
@@ -717,15 +771,15 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
                                               BlockRec* __restrict__ recs) {
     __shared__ uint32_t lit[288];
     __shared__ uint32_t dst[32];
-    __shared__ uint32_t rawsum;
     __shared__ uint16_t idx[288];
     __shared__ uint32_t wleaf[288];
     __shared__ uint32_t wa[576];
     __shared__ uint32_t wb[576];
-    __shared__ uint8_t leafbits[16 * 576];
+    __shared__ uint32_t leafmask[16 * LM_WORDS];
+    __shared__ uint32_t s_leaves[16];
     __shared__ uint32_t clf[19];
-    __shared__ uint32_t s_cnt;
-    __shared__ uint64_t s_extra;
+    __shared__ uint32_t red[3][L6B_THREADS / 64];
+    __shared__ uint32_t s_cnt, s_hl, s_hd;
     __shared__ BlockRec R;
     const uint32_t ntok = ntok_in[c];
     const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
@@ -733,26 +787,49 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
     const uint64_t cp = l6_stride(chunk);
     const uint32_t* TK = toks + (uint64_t)c * cp;
     const uint32_t t0 = b * BLK_SYMS, t1 = min(ntok, t0 + BLK_SYMS);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     for (int i = tid; i < 288; i += L6B_THREADS) lit[i] = 0;
     if (tid < 32) dst[tid] = 0;
-    if (tid == 0) rawsum = 0;
-    __syncthreads();
-    uint32_t raw = 0;
-    for (uint32_t t = t0 + tid; t < t1; t += L6B_THREADS) {
-        const uint32_t v = TK[t];
-        if (v & 0x80000000u) {
-            const uint32_t L = (v >> 16) & 0x1FFu, d = v & 0xFFFFu;
-            atomicAdd(&lit[257 + len_sym(L).code], 1u);
-            atomicAdd(&dst[dist_sym(d).code], 1u);
-            raw += L;
-        } else {
-            atomicAdd(&lit[v & 0xFFu], 1u);
-            raw += 1;
-        }
+    if (tid < 19) clf[tid] = 0;
+    if (tid == 0) {
+        s_hl = 0;
+        s_hd = 0;
     }
-    atomicAdd(&rawsum, raw);
     __syncthreads();
+    // histogram: t0 is a multiple of BLK_SYMS and the token stride a multiple
+    // of 8, so 16-byte loads are aligned; the words read past t1 lie inside
+    // the chunk's stride and are ignored
+    constexpr int HQ = BLK_SYMS / L6B_THREADS / 8;
+    const uint4* T4 = reinterpret_cast<const uint4*>(TK + t0);
+    uint32_t raw = 0;
+#pragma unroll 1
+    for (int half = 0; half < 2; ++half) {
+        uint4 q[HQ];
+#pragma unroll
+        for (int k = 0; k < HQ; ++k) {
+            const uint32_t g = (uint32_t)tid + L6B_THREADS * (uint32_t)(half * HQ + k);
+            q[k] = t0 + 4u * g < t1 ? T4[g] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < HQ; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t g = (uint32_t)tid + L6B_THREADS * (uint32_t)(half * HQ + k);
+                if (t0 + 4u * g + j < t1) {
+                    const uint32_t v = j == 0 ? q[k].x : (j == 1 ? q[k].y : (j == 2 ? q[k].z : q[k].w));
+                    if (v & 0x80000000u) {
+                        const uint32_t L = (v >> 16) & 0x1FFu, d = v & 0xFFFFu;
+                        atomicAdd(&lit[257 + len_sym(L).code], 1u);
+                        atomicAdd(&dst[dist_sym(d).code], 1u);
+                        raw += L;
+                    } else {
+                        atomicAdd(&lit[v & 0xFFu], 1u);
+                        raw += 1;
+                    }
+                }
+            }
+    }
+    const uint32_t rawsum = l6b_sum(raw, red[0]);  // its barriers also publish the histogram
     if (tid == 0) {
         lit[256] += 1;  // EOB
         R.t0 = t0;
@@ -762,91 +839,105 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
     }
     __syncthreads();
     // lengths
-    limited_lengths(lit, 288, 15, R.ll, idx, wleaf, wa, wb, leafbits, &s_cnt);
-    limited_lengths(dst, 32, 15, R.dl, idx, wleaf, wa, wb, leafbits, &s_cnt);
-    if (tid == 0) {
-        // fixed and dynamic sizes (deflate_block.cpp:50-92 for the fixed estimate)
-        uint64_t fixedb = 3;
-        for (int i = 0; i < 288; ++i) fixedb += (uint64_t)fixed_ll_len(i) * lit[i];
-        for (int i = 0; i < 32; ++i) fixedb += 5ull * dst[i];
-        uint64_t extra = 0;
-        for (int i = 0; i < 29; ++i) extra += (uint64_t)c_len_extra[i] * lit[257 + i];
-        for (int i = 0; i < 30; ++i) extra += (uint64_t)c_dist_extra[i] * dst[i];
-        fixedb += extra;
-        // dynamic header: EOB / distance fix-ups, trimmed counts, code-length RLE
-        // (deflate_block.cpp:196-290, 347-370)
-        if (!R.ll[256]) R.ll[256] = 1;
-        bool any = false;
-        for (int i = 0; i < 32; ++i) any |= R.dl[i] != 0;
-        if (!any) R.dl[0] = 1;
-        int hlit = 286;
-        while (hlit > 257 && !R.ll[hlit - 1]) --hlit;
-        int hdist = 30;
-        while (hdist > 1 && !R.dl[hdist - 1]) --hdist;
-        const int total = hlit + hdist;
-        int nr = 0;
-        for (int i = 0; i < total;) {
-            const uint32_t sym = i < hlit ? R.ll[i] : R.dl[i - hlit];
-            int run = 1;
-            while (i + run < total && run < 138) {
-                const uint32_t nx = (i + run) < hlit ? R.ll[i + run] : R.dl[i + run - hlit];
-                if (nx != sym) break;
-                ++run;
-            }
-            if (sym == 0 && run >= 3) {
-                R.rle_sym[nr] = run <= 10 ? 17 : 18;
-                R.rle_x[nr++] = (uint8_t)(run <= 10 ? run - 3 : run - 11);
-                i += run;
-            } else if (sym != 0 && run >= 4) {
-                R.rle_sym[nr] = (uint8_t)sym;
-                R.rle_x[nr++] = 0xFF;
-                const int rep = run - 1 < 6 ? run - 1 : 6;
-                R.rle_sym[nr] = 16;
-                R.rle_x[nr++] = (uint8_t)(rep - 3);
-                i += 1 + rep;
-            } else {
-                R.rle_sym[nr] = (uint8_t)sym;
-                R.rle_x[nr++] = 0xFF;
-                ++i;
-            }
+    limited_lengths(lit, 288, 15, R.ll, idx, wleaf, wa, wb, leafmask, &s_cnt, s_leaves);
+    limited_lengths(dst, 32, 15, R.dl, idx, wleaf, wa, wb, leafmask, &s_cnt, s_leaves);
+    // dynamic header fix-ups: EOB and at least one distance code
+    // (deflate_block.cpp:196-290)
+    if (tid < 64) {
+        const uint64_t any = ballot(tid < 32 && R.dl[tid] != 0);
+        if (tid == 0) {
+            if (!R.ll[256]) R.ll[256] = 1;
+            if (!any) R.dl[0] = 1;
         }
-        for (int i = 0; i < 19; ++i) clf[i] = 0;
-        for (int i = 0; i < nr; ++i) clf[R.rle_sym[i]]++;
-        R.hlit = (uint32_t)hlit;
-        R.hdist = (uint32_t)hdist;
-        R.nrle = (uint32_t)nr;
-        R.body_bits = fixedb;  // parked here until the type is chosen
-        s_extra = extra;
     }
     __syncthreads();
-    limited_lengths(clf, 19, 7, R.cl, idx, wleaf, wa, wb, leafbits, &s_cnt);
-    if (tid == 0) {
-        const int nr = (int)R.nrle;
-        const uint64_t extra = s_extra;
-        const uint64_t fixedb = R.body_bits;
-        int act = 0;
-        for (int i = 0; i < 19; ++i) act += R.cl[i] != 0;
-        if (act == 1) {
+    // fixed and dynamic sizes (deflate_block.cpp:50-92 for the fixed estimate)
+    // and the trimmed counts; a block's sizes stay far below 2^32 bits
+    uint32_t fx = 0, ex = 0, dd = 0;
+    for (int i = tid; i < 288; i += L6B_THREADS) {
+        const uint32_t f = lit[i];
+        fx += fixed_ll_len((uint32_t)i) * f;
+        dd += (uint32_t)R.ll[i] * f;
+        if (i >= 257 && i < 286) ex += (uint32_t)c_len_extra[i - 257] * f;
+        if (i < 286 && R.ll[i]) atomicMax(&s_hl, (uint32_t)i + 1u);
+    }
+    if (tid < 32) {
+        const uint32_t f = dst[tid];
+        fx += 5u * f;
+        dd += (uint32_t)R.dl[tid] * f;
+        if (tid < 30) ex += (uint32_t)c_dist_extra[tid] * f;
+        if (tid < 30 && R.dl[tid]) atomicMax(&s_hd, (uint32_t)tid + 1u);
+    }
+    const uint64_t extra = l6b_sum(ex, red[0]);
+    const uint64_t fixedb = 3ull + l6b_sum(fx, red[1]) + extra;
+    const uint64_t dyn_data = l6b_sum(dd, red[2]) + extra;
+    const uint32_t hlit = max(257u, s_hl), hdist = max(1u, s_hd);
+    // code-length RLE over ll[0..hlit) dl[0..hdist) (deflate_block.cpp:347-370):
+    // thread t takes positions 2t and 2t + 1; a maximal run of equal lengths
+    // is encoded by the thread holding its first position
+    const uint32_t total = hlit + hdist;
+    auto seq = [&](uint32_t k) -> uint32_t { return k < hlit ? R.ll[k] : R.dl[k - hlit]; };
+    uint32_t mrun[2] = {0, 0}, vrun[2] = {0, 0}, nitems = 0;
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t k = 2u * (uint32_t)tid + (uint32_t)h;
+        if (k >= total) break;
+        const uint32_t v = seq(k);
+        if (k > 0 && seq(k - 1) == v) continue;
+        uint32_t m = 1;
+        while (k + m < total && seq(k + m) == v) ++m;
+        mrun[h] = m;
+        vrun[h] = v;
+        nitems += rle_run_items(v, m, [](uint32_t, uint32_t) {});
+    }
+    const uint32_t incl = wave_incl_scan(nitems, lane);
+    if (lane == 63) red[0][tid >> 6] = incl;
+    __syncthreads();
+    uint32_t o = incl - nitems;
+    for (int w = 0; w < (tid >> 6); ++w) o += red[0][w];
+    uint32_t nr = 0;
+    for (int w = 0; w < L6B_THREADS / 64; ++w) nr += red[0][w];
+    for (int h = 0; h < 2; ++h) {
+        if (!mrun[h]) continue;
+        rle_run_items(vrun[h], mrun[h], [&](uint32_t sy, uint32_t x) {
+            R.rle_sym[o] = (uint8_t)sy;
+            R.rle_x[o] = (uint8_t)x;
+            atomicAdd(&clf[sy], 1u);
+            ++o;
+        });
+    }
+    __syncthreads();
+    limited_lengths(clf, 19, 7, R.cl, idx, wleaf, wa, wb, leafmask, &s_cnt, s_leaves);
+    // a single code length code gets a partner; HCLEN trimmed in transmission order
+    if (tid < 64) {
+        const uint64_t act = ballot(tid < 19 && R.cl[tid] != 0);
+        if (tid == 0 && __popcll(act) == 1) {
             for (int i = 0; i < 19; ++i)
                 if (!R.cl[i]) {
                     R.cl[i] = 1;
                     break;
                 }
         }
-        int hclen = 19;
-        while (hclen > 4 && !R.cl[c_cl_order[hclen - 1]]) --hclen;
-        uint32_t hb = 5 + 5 + 4 + 3u * (uint32_t)hclen;
-        for (int i = 0; i < nr; ++i) {
-            const int sy = R.rle_sym[i];
-            hb += R.cl[sy] + (sy == 16 ? 2u : (sy == 17 ? 3u : (sy == 18 ? 7u : 0u)));
-        }
+    }
+    __syncthreads();
+    uint32_t hclen = 4;
+    {
+        const uint64_t nz = ballot(lane < 19 && R.cl[c_cl_order[lane < 19 ? lane : 0]] != 0);
+        if (nz) hclen = max(4u, 64u - (uint32_t)__clzll(nz));
+    }
+    uint32_t ib = 0;
+    for (uint32_t i = (uint32_t)tid; i < nr; i += L6B_THREADS) {
+        const uint32_t sy = R.rle_sym[i];
+        ib += R.cl[sy] + (sy == 16 ? 2u : (sy == 17 ? 3u : (sy == 18 ? 7u : 0u)));
+    }
+    const uint32_t hb = 5 + 5 + 4 + 3u * hclen + l6b_sum(ib, red[0]);
+    if (tid == 0) {
         // data bits with the fixed-up lengths (EOB / dist[0] may have changed)
-        uint64_t dyn_data = extra;
-        for (int i = 0; i < 288; ++i) dyn_data += (uint64_t)R.ll[i] * lit[i];
-        for (int i = 0; i < 32; ++i) dyn_data += (uint64_t)R.dl[i] * dst[i];
         const uint64_t dynb = 3ull + hb + dyn_data;
         const uint64_t storedb = ((uint64_t)R.raw_len + 5) * 8;
-        R.hclen = (uint32_t)hclen;
+        R.hlit = hlit;
+        R.hdist = hdist;
+        R.nrle = nr;
+        R.hclen = hclen;
         R.hdr_bits = hb;
         if (storedb <= fixedb && storedb <= dynb) {
             R.type = 0;
