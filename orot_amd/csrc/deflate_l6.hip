@@ -43,7 +43,8 @@ constexpr int L6M_THREADS = 1024;
 constexpr uint32_t L6M_TILE = 4096;
 constexpr uint32_t L6M_DRING = 65536;  // bytes, power of two
 constexpr uint32_t L6M_PRING = 36864;  // chain steps: 32 KiB + one tile
-constexpr uint32_t L6M_LDS = L6M_DRING + L6M_PRING * 2;
+constexpr uint32_t L6M_MIRROR = 4;     // words of the byte ring repeated past its end
+constexpr uint32_t L6M_LDS = L6M_DRING + 4 * L6M_MIRROR + L6M_PRING * 2;
 constexpr int L6M_WALKS = 4;  // chain walks in step per thread
 
 constexpr int L6R_THREADS = 1024;
@@ -200,20 +201,20 @@ __global__ __launch_bounds__(L6P_THREADS) void l6_pred_kernel(const uint8_t* __r
 }
 
 // ---------------------------------------------------------------------------
-// M(x) for every position: LDS rings of bytes (64 KiB) and chain steps.
+// M(x) for every position: LDS rings of bytes (64 KiB) and chain steps.  The
+// byte ring's first L6M_MIRROR words are repeated past its end, so the words
+// of an unaligned read are consecutive in LDS.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t ring_u8(const uint32_t* R, uint32_t a) {
     return (R[(a >> 2) & (L6M_DRING / 4 - 1)] >> (8 * (a & 3u))) & 0xFFu;
 }
 __device__ __forceinline__ uint32_t ring_u32(const uint32_t* R, uint32_t a) {
-    const uint32_t i = a >> 2, k = a & 3u;
-    constexpr uint32_t M = L6M_DRING / 4 - 1;
-    return __builtin_amdgcn_alignbyte(R[(i + 1) & M], R[i & M], k);
+    const uint32_t i = (a >> 2) & (L6M_DRING / 4 - 1), k = a & 3u;
+    return __builtin_amdgcn_alignbyte(R[i + 1], R[i], k);
 }
 __device__ __forceinline__ uint64_t ring_u64(const uint32_t* R, uint32_t a) {
-    const uint32_t i = a >> 2, k = a & 3u;
-    constexpr uint32_t M = L6M_DRING / 4 - 1;
-    const uint32_t w0 = R[i & M], w1 = R[(i + 1) & M], w2 = R[(i + 2) & M];
+    const uint32_t i = (a >> 2) & (L6M_DRING / 4 - 1), k = a & 3u;
+    const uint32_t w0 = R[i], w1 = R[i + 1], w2 = R[i + 2];
     return (uint64_t)__builtin_amdgcn_alignbyte(w1, w0, k) | ((uint64_t)__builtin_amdgcn_alignbyte(w2, w1, k) << 32);
 }
 
@@ -222,7 +223,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                                               uint32_t chunk, const uint16_t* __restrict__ pd,
                                               uint32_t* __restrict__ mlw, const L6Params prm) {
     uint32_t* DR = smem;                                            // byte ring
-    uint16_t* PR = reinterpret_cast<uint16_t*>(smem + L6M_DRING / 4);  // chain-step ring
+    uint16_t* PR = reinterpret_cast<uint16_t*>(smem + L6M_DRING / 4 + L6M_MIRROR);  // chain-step ring
     // positions 0 .. n-1 are the halo (H bytes) followed by the chunk; M is
     // computed for the chunk's positions, ML[x - H]
     const uint32_t H = l6_halo(c, desc, bufs);
@@ -247,7 +248,9 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                 for (uint32_t k = 0; k < 4; ++k)
                     if (a + k < n) v |= (uint32_t)s[a + k] << (8 * k);
             }
-            DR[(a >> 2) & (L6M_DRING / 4 - 1)] = v;
+            const uint32_t wi = (a >> 2) & (L6M_DRING / 4 - 1);
+            DR[wi] = v;
+            if (wi < L6M_MIRROR) DR[L6M_DRING / 4 + wi] = v;
         }
         dfill = max(dfill, dneed);
         for (uint32_t q = x0 + tid; q < xe; q += L6M_THREADS) PR[q % L6M_PRING] = PD[q];
@@ -281,50 +284,63 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                 m[i].xb = ring_u64(DR, x + 8);
                 m[i].w = (uint32_t)m[i].xa;
             }
+            // Each round: the chain steps of the walks, then their candidates'
+            // first words and words at best - 3, then the comparisons, so the
+            // walks' LDS reads overlap.  Dead walks read one fixed address (an
+            // LDS broadcast, no bank conflict) and ignore the value.
             for (;;) {
-                bool any = false;
+                uint32_t st[L6M_WALKS];
 #pragma unroll
-                for (int i = 0; i < L6M_WALKS; ++i) any |= m[i].live;
+                for (int i = 0; i < L6M_WALKS; ++i) st[i] = PR[m[i].live ? m[i].ri : 0u];
+                bool any = false;
+                uint32_t q[L6M_WALKS], c0[L6M_WALKS], c1[L6M_WALKS];
+#pragma unroll
+                for (int i = 0; i < L6M_WALKS; ++i) {
+                    Walk& a = m[i];
+                    q[i] = a.cur - st[i];
+                    const bool ok = a.live && a.k < max_chain && st[i] != 0 && a.x - q[i] <= 32768u;
+                    a.live = ok;
+                    any |= ok;
+                    if (ok) {
+                        ++a.k;
+                        a.cur = q[i];
+                        a.ri = a.ri >= st[i] ? a.ri - st[i] : a.ri + L6M_PRING - st[i];  // st <= 32768 < L6M_PRING
+                    }
+                    // addresses masked to 0 for dead walks, kept opaque so the
+                    // select is not turned back into branches around the reads
+                    uint32_t a0 = q[i] & (0u - (uint32_t)ok);
+                    uint32_t a1 = (q[i] + a.best - 3) & (0u - (uint32_t)(ok && a.best >= 4));
+                    asm volatile("" : "+v"(a0), "+v"(a1));
+                    c0[i] = ring_u32(DR, a0);
+                    c1[i] = ring_u32(DR, a1);
+                }
                 if (!any) break;
 #pragma unroll
                 for (int i = 0; i < L6M_WALKS; ++i) {
                     Walk& a = m[i];
-                    if (!a.live) continue;
-                    if (a.k >= max_chain) {
-                        a.live = false;
-                        continue;
-                    }
-                    ++a.k;
-                    const uint32_t st = PR[a.ri];
-                    const uint32_t q = a.cur - st;
-                    if (st == 0 || a.x - q > 32768u) {
-                        a.live = false;
-                        continue;
-                    }
-                    a.cur = q;
-                    a.ri = a.ri >= st ? a.ri - st : a.ri + L6M_PRING - st;  // st <= 32768 < L6M_PRING
-                    if (ring_u32(DR, q) != a.w) continue;
+                    if (!a.live || c0[i] != a.w) continue;
                     // only a match that also covers byte `best` can be longer
                     // than best (best < cap here, so these bytes are in the chunk)
-                    if (a.best >= 4 && ring_u32(DR, q + a.best - 3) != a.xbw) continue;
+                    if (a.best >= 4 && c1[i] != a.xbw) continue;
+                    const uint32_t qq = q[i];
                     uint32_t l;
-                    uint64_t df = ring_u64(DR, q) ^ a.xa;
+                    uint64_t df = ring_u64(DR, qq) ^ a.xa;
                     if (df) {
                         l = (uint32_t)(__builtin_ctzll(df) >> 3);
-                    } else if ((df = ring_u64(DR, q + 8) ^ a.xb) != 0) {
+                    } else if ((df = ring_u64(DR, qq + 8) ^ a.xb) != 0) {
                         l = 8 + (uint32_t)(__builtin_ctzll(df) >> 3);
                     } else {
                         l = 16;
                         while (l < a.cap) {
                             if (l + 8 <= a.cap) {
-                                const uint64_t d2 = ring_u64(DR, a.x + l) ^ ring_u64(DR, q + l);
+                                const uint64_t d2 = ring_u64(DR, a.x + l) ^ ring_u64(DR, qq + l);
                                 if (d2) {
                                     l += (uint32_t)(__builtin_ctzll(d2) >> 3);
                                     break;
                                 }
                                 l += 8;
                             } else {
-                                if (ring_u8(DR, a.x + l) != ring_u8(DR, q + l)) break;
+                                if (ring_u8(DR, a.x + l) != ring_u8(DR, qq + l)) break;
                                 ++l;
                             }
                         }
@@ -332,7 +348,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
                     l = min(l, a.cap);
                     if (l > a.best) {
                         a.best = l;
-                        a.bd = a.x - q;
+                        a.bd = a.x - qq;
                         if (l >= 4) a.xbw = ring_u32(DR, a.x + l - 3);
                         // nothing later can be longer than nice (stop rule) or cap
                         if ((int)l >= nice || l >= a.cap) a.live = false;
