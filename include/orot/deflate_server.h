@@ -40,7 +40,9 @@ typedef struct deflate_server_completion {
 } deflate_server_completion;
 
 deflate_server* deflate_server_new(const deflate_server_config* cfg);
-/* Waits for every request in flight, then frees the server and its pool. */
+/* Waits for every request in flight and runs the queued ones that fit, then
+ * frees the server and its pool.  Queued requests that cannot be admitted
+ * (the pool is held by outputs not yet released) are dropped. */
 void deflate_server_free(deflate_server* srv);
 
 /* Queue one request.  d_in: input already in device memory (it must stay

@@ -132,6 +132,7 @@ struct deflate_server {
     uint64_t next_id = 1;
     uint64_t n_submitted = 0, n_completed = 0, n_queued = 0, n_inflight = 0, n_batches = 0, max_queue = 0;
     bool stop = false;
+    uint64_t wake = 0;  // bumped by submit / release / stop: the worker's reason to look again
     std::thread worker;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
@@ -272,15 +273,21 @@ struct deflate_server {
     void run() {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
+            const uint64_t seen = wake;
             const bool retired = retire();
             if (retired) cv_done.notify_all();
-            launch();
-            if (stop && n_queued == 0 && !any_busy()) break;
-            if (any_busy()) {
+            const bool launched = launch();
+            const bool busy = any_busy();
+            // at stop, requests that can never be admitted (the pool is full of
+            // outputs nobody will release) are dropped with the server
+            if (stop && !busy && (n_queued == 0 || !launched)) break;
+            if (busy) {
                 // a batch is on the GPU: check again shortly
                 cv_work.wait_for(lk, std::chrono::microseconds(50));
-            } else {
-                cv_work.wait(lk, [&] { return stop || n_queued > 0; });
+            } else if (!launched && !retired) {
+                // nothing runs and the queue heads wait for memory (or the
+                // queues are empty): sleep until a submit, release or stop
+                cv_work.wait(lk, [&] { return stop || wake != seen; });
             }
         }
     }
@@ -309,6 +316,7 @@ struct deflate_server {
         queues[level * 4 + format].push_back(p);
         ++n_submitted;
         ++n_queued;
+        ++wake;
         max_queue = std::max(max_queue, n_queued);
         cv_work.notify_one();
         return p->id;
@@ -348,6 +356,7 @@ void deflate_server_free(deflate_server* s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
+        ++s->wake;
         s->cv_work.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
@@ -425,6 +434,7 @@ int deflate_server_release(deflate_server* s, uint64_t id) {
     if (it == s->live.end() || !it->second->done) return -1;
     s->pool.release(it->second->out_off, it->second->cap);
     s->live.erase(it);
+    ++s->wake;
     s->cv_work.notify_one();  // freed memory may admit the queue head
     return 0;
 }
