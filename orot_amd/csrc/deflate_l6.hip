@@ -46,6 +46,8 @@ constexpr uint32_t L6M_PRING = 36864;  // chain steps: 32 KiB + one tile
 constexpr uint32_t L6M_MIRROR = 4;     // words of the byte ring repeated past its end
 constexpr uint32_t L6M_LDS = L6M_DRING + 4 * L6M_MIRROR + L6M_PRING * 2;
 constexpr int L6M_WALKS = 4;  // chain walks in step per thread
+constexpr uint32_t L6M_HIST = (32768 + L6M_TILE - 1) / L6M_TILE;  // tiles a walk reaches back
+constexpr uint32_t L6M_MAX_SEGS = 16;  // workgroups sharing one chunk's matching
 
 constexpr int L6R_THREADS = 1024;
 constexpr int L6R_WAVES = L6R_THREADS / 64;
@@ -218,10 +220,14 @@ __device__ __forceinline__ uint64_t ring_u64(const uint32_t* R, uint32_t a) {
     return (uint64_t)__builtin_amdgcn_alignbyte(w1, w0, k) | ((uint64_t)__builtin_amdgcn_alignbyte(w2, w1, k) << 32);
 }
 
-__device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
-                                              const ChunkDesc* __restrict__ desc, const BufDesc* __restrict__ bufs,
-                                              uint32_t chunk, const uint16_t* __restrict__ pd,
-                                              uint32_t* __restrict__ mlw, const L6Params prm) {
+// Segment `seg` of `segs` of chunk c: a contiguous run of its tiles, whose
+// rings are first filled from the L6M_HIST tiles before it (a walk reaches
+// back 32 KiB at most), so segments of one chunk run in separate workgroups.
+__device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t seg, uint32_t segs, uint32_t* smem,
+                                              const uint8_t* __restrict__ src, const ChunkDesc* __restrict__ desc,
+                                              const BufDesc* __restrict__ bufs, uint32_t chunk,
+                                              const uint16_t* __restrict__ pd, uint32_t* __restrict__ mlw,
+                                              const L6Params prm) {
     uint32_t* DR = smem;                                            // byte ring
     uint16_t* PR = reinterpret_cast<uint16_t*>(smem + L6M_DRING / 4 + L6M_MIRROR);  // chain-step ring
     // positions 0 .. n-1 are the halo (H bytes) followed by the chunk; M is
@@ -234,8 +240,15 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
     uint32_t* ML = mlw + (uint64_t)c * cp;
     const int tid = threadIdx.x;
     const int max_chain = prm.max_chain, nice = prm.nice_len;
-    uint32_t dfill = 0;  // bytes [0, dfill) have been loaded into the ring (word granular)
-    for (uint32_t x0 = 0; x0 < n; x0 += L6M_TILE) {
+    // tiles [t_lo, t_hi) hold this segment's positions; walks start at tile H / TILE
+    const uint32_t tiles = (n + L6M_TILE - 1) / L6M_TILE, t_walk = H / L6M_TILE;
+    const uint32_t per = (tiles - t_walk + segs - 1) / segs;
+    const uint32_t t_lo = t_walk + seg * per, t_hi = min(tiles, t_lo + per);
+    if (t_lo >= t_hi) return;
+    const uint32_t x_lo = t_lo * L6M_TILE;
+    const uint32_t x_first = (t_lo > L6M_HIST ? t_lo - L6M_HIST : 0u) * L6M_TILE;
+    uint32_t dfill = x_first;  // bytes [x_first, dfill) have been loaded into the ring (word granular)
+    for (uint32_t x0 = x_first; x0 < t_hi * L6M_TILE; x0 += L6M_TILE) {
         const uint32_t xe = min(n, x0 + L6M_TILE);
         // bytes up to xe + 258 + 8 (rounded to words), chain steps of [x0, xe)
         const uint32_t dneed = min((n + 3u) & ~3u, (xe + 258u + 8u + 3u) & ~3u);
@@ -255,7 +268,7 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t* smem, const 
         dfill = max(dfill, dneed);
         for (uint32_t q = x0 + tid; q < xe; q += L6M_THREADS) PR[q % L6M_PRING] = PD[q];
         __syncthreads();
-        if (xe <= H) continue;  // a tile wholly in the halo only fills the rings
+        if (xe <= H || x0 < x_lo) continue;  // halo and history tiles only fill the rings
         // L6M_WALKS positions per thread, x + i * L6M_THREADS, walk their
         // chains in step, so each step has that many independent LDS loads
         // in flight.
@@ -369,12 +382,13 @@ __global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __
                                                                const uint32_t* __restrict__ order,
                                                                BatchHdr* __restrict__ hdr, uint32_t chunk,
                                                                const uint16_t* __restrict__ pd,
-                                                               uint32_t* __restrict__ mlw, const L6Params prm) {
+                                                               uint32_t* __restrict__ mlw, const L6Params prm,
+                                                               uint32_t segs) {
     extern __shared__ uint32_t smem[];
     for (;;) {
-        const uint32_t k = next_item(&hdr->queue[Q_MATCH], hdr->nchunks, smem);
+        const uint32_t k = next_item(&hdr->queue[Q_MATCH], hdr->nchunks * segs, smem);
         if (k == NO_ITEM) break;
-        l6_match_body(order[k], smem, src, desc, bufs, chunk, pd, mlw, prm);
+        l6_match_body(order[k / segs], k % segs, segs, smem, src, desc, bufs, chunk, pd, mlw, prm);
         __syncthreads();
     }
 }
@@ -1384,9 +1398,17 @@ hipError_t launch_l6_front(const Plan& p) {
     const uint32_t maxblk = l6_max_blocks(p.chunk);
     hipError_t e = p.exch_ok ? launch_pred<true>(p) : launch_pred<false>(p);
     if (e != hipSuccess) return e;
-    const unsigned gm = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6M_LDS, L6M_THREADS));
+    // a batch of few chunks splits each chunk's matching over several
+    // workgroups; a captured graph (counts only in device memory) keeps one
+    const uint32_t mcap = p.cus * l6_per_cu(p, L6M_LDS, L6M_THREADS);
+    uint32_t segs = 1;
+    if (!p.persistent_grid && p.nchunks) {
+        segs = (mcap + p.nchunks - 1) / p.nchunks;
+        if (segs > L6M_MAX_SEGS) segs = L6M_MAX_SEGS;
+    }
+    const unsigned gm = grid_for(p.persistent_grid, p.nchunks * segs, mcap);
     hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.d_desc, p.d_bufs,
-                       p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6);
+                       p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6, segs);
     const unsigned gr = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, l6r_lds(p.chunk), L6R_THREADS));
     hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), l6r_lds(p.chunk), p.stream, p.d_in, p.d_desc, p.d_order,
                        p.d_hdr, p.chunk, p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.l6);
