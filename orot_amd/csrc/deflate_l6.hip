@@ -47,7 +47,7 @@ constexpr uint32_t L6M_MIRROR = 4;     // words of the byte ring repeated past i
 constexpr uint32_t L6M_LDS = L6M_DRING + 4 * L6M_MIRROR + L6M_PRING * 2;
 constexpr int L6M_WALKS = 4;  // chain walks in step per thread
 constexpr uint32_t L6M_HIST = (32768 + L6M_TILE - 1) / L6M_TILE;  // tiles a walk reaches back
-constexpr uint32_t L6M_MAX_SEGS = 16;  // workgroups sharing one chunk's matching
+constexpr uint32_t L6M_MAX_SEGS = 16;  // workgroups sharing one chunk's pred / matching
 
 constexpr int L6R_THREADS = 1024;
 constexpr int L6R_WAVES = L6R_THREADS / 64;
@@ -80,11 +80,15 @@ __device__ __forceinline__ uint32_t ml_dist(uint32_t m) { return m >> 16; }
 // ---------------------------------------------------------------------------
 // C(p) and the in-window chain step PD[p].
 // ---------------------------------------------------------------------------
+// Segment `seg` of `segs` of chunk c writes PD for positions [pa, pb).  Its
+// tables cover [lo, pb) with lo = pa - 32 KiB (or 0): a predecessor before lo
+// is more than 32 KiB away and gives step 0 all the same.  The positions of
+// [lo, pa) only fill the tables, so segments run in separate workgroups.
 template <bool EXCH>
-__device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
-                                             const ChunkDesc* __restrict__ desc, const BufDesc* __restrict__ bufs,
-                                             uint32_t chunk, uint32_t* __restrict__ tokw,
-                                             uint16_t* __restrict__ pd) {
+__device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t seg, uint32_t segs, uint32_t* smem,
+                                             const uint8_t* __restrict__ src, const ChunkDesc* __restrict__ desc,
+                                             const BufDesc* __restrict__ bufs, uint32_t chunk,
+                                             uint32_t* __restrict__ tokw, uint16_t* __restrict__ pd) {
     // positions 0 .. n-1 are the halo (H bytes) followed by the chunk
     const uint32_t H = l6_halo(c, desc, bufs);
     const uint32_t n = desc[c].len + H;
@@ -94,11 +98,16 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const u
     uint16_t* PD = pd + (uint64_t)c * cp;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t nh = n >= 4 ? n - 3 : 0;
-    const uint32_t SR = (((nh + L6P_W - 1) / L6P_W) + 63u) & ~63u;
+    const uint32_t per = (((n + segs - 1) / segs) + 4095u) & ~4095u;
+    const uint32_t pa = seg * per, pb = min(n, pa + per);
+    if (pa >= pb) return;
+    const uint32_t lo = pa > 32768u ? pa - 32768u : 0u;
+    const uint32_t he = min(pb, nh);  // hashed positions of the window: [lo, he)
+    const uint32_t SR = ((((he > lo ? he - lo : 0u) + L6P_W - 1) / L6P_W) + 63u) & ~63u;
     {
         uint32_t* tab = smem + (uint32_t)wv * L6_TAB;
         for (uint32_t i = lane; i < L6_TAB; i += 64) tab[i] = 0;
-        const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
+        const uint32_t a = lo + (uint32_t)wv * SR, b = min(a + SR, he);
         // Window loads of three more steps stay in flight, and every lane
         // stores (lanes past the sub-range on position n - 1, which the
         // resolution below never reads from T), so the waits are exact.
@@ -137,10 +146,11 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const u
                     }
                 }
             }
+            // history positions (below pa) belong to another segment's T
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const uint32_t p = p0 + 64u * j + (uint32_t)lane;
-                T[p < b ? p : n - 1] = res[j];
+                T[p < b && p >= pa ? p : n - 1] = res[j];
             }
         };
         uint32_t w0[8], w1[8], w2[8], w3[8];
@@ -165,11 +175,11 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t* smem, const u
     }
     __syncthreads();
     // first occurrences of a sub-range take the latest earlier sub-range's entry
-    for (uint32_t p = tid; p < n; p += L6P_THREADS) {
+    for (uint32_t p = pa + tid; p < pb; p += L6P_THREADS) {
         uint32_t step = 0;
         if (p < nh) {
             uint32_t C1 = T[p];
-            const uint32_t w = p / SR;
+            const uint32_t w = (p - lo) / SR;
             if (C1 == 0 && w > 0) {
                 const uint32_t h = hash4<L6_HB>(ld32u(s + p));
                 for (int w2 = (int)w - 1; w2 >= 0; --w2) {
@@ -192,12 +202,13 @@ __global__ __launch_bounds__(L6P_THREADS) void l6_pred_kernel(const uint8_t* __r
                                                               const BufDesc* __restrict__ bufs,
                                                               const uint32_t* __restrict__ order,
                                                               BatchHdr* __restrict__ hdr, uint32_t chunk,
-                                                              uint32_t* __restrict__ tokw, uint16_t* __restrict__ pd) {
+                                                              uint32_t* __restrict__ tokw, uint16_t* __restrict__ pd,
+                                                              uint32_t segs) {
     extern __shared__ uint32_t smem[];
     for (;;) {
-        const uint32_t k = next_item(&hdr->queue[Q_PRED], hdr->nchunks, smem);
+        const uint32_t k = next_item(&hdr->queue[Q_PRED], hdr->nchunks * segs, smem);
         if (k == NO_ITEM) break;
-        l6_pred_body<EXCH>(order[k], smem, src, desc, bufs, chunk, tokw, pd);
+        l6_pred_body<EXCH>(order[k / segs], k % segs, segs, smem, src, desc, bufs, chunk, tokw, pd);
         __syncthreads();
     }
 }
@@ -1354,6 +1365,15 @@ inline uint32_t l6_per_cu(const Plan& p, size_t smem, int threads) {
     return k ? k : 1u;
 }
 
+// Workgroups per chunk for the pred and match kernels: a batch of few chunks
+// splits each chunk so that the grid fills `cap` workgroups; a captured
+// graph (counts only in device memory) keeps one.
+inline uint32_t l6_segments(const Plan& p, uint32_t cap) {
+    if (p.persistent_grid || p.nchunks == 0) return 1;
+    const uint32_t segs = (cap + p.nchunks - 1) / p.nchunks;
+    return segs < L6M_MAX_SEGS ? segs : L6M_MAX_SEGS;
+}
+
 template <class K>
 hipError_t l6_allow_lds(K* kernel, size_t smem) {
     return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1362,9 +1382,11 @@ hipError_t l6_allow_lds(K* kernel, size_t smem) {
 
 template <bool EXCH>
 hipError_t launch_pred(const Plan& p) {
-    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, L6P_LDS, L6P_THREADS));
+    const uint32_t cap = p.cus * l6_per_cu(p, L6P_LDS, L6P_THREADS);
+    const uint32_t segs = l6_segments(p, cap);
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks * segs, cap);
     hipLaunchKernelGGL((l6_pred_kernel<EXCH>), dim3(g), dim3(L6P_THREADS), L6P_LDS, p.stream, p.d_in, p.d_desc,
-                       p.d_bufs, p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_pd);
+                       p.d_bufs, p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_pd, segs);
     return hipSuccess;
 }
 
@@ -1398,14 +1420,8 @@ hipError_t launch_l6_front(const Plan& p) {
     const uint32_t maxblk = l6_max_blocks(p.chunk);
     hipError_t e = p.exch_ok ? launch_pred<true>(p) : launch_pred<false>(p);
     if (e != hipSuccess) return e;
-    // a batch of few chunks splits each chunk's matching over several
-    // workgroups; a captured graph (counts only in device memory) keeps one
     const uint32_t mcap = p.cus * l6_per_cu(p, L6M_LDS, L6M_THREADS);
-    uint32_t segs = 1;
-    if (!p.persistent_grid && p.nchunks) {
-        segs = (mcap + p.nchunks - 1) / p.nchunks;
-        if (segs > L6M_MAX_SEGS) segs = L6M_MAX_SEGS;
-    }
+    const uint32_t segs = l6_segments(p, mcap);
     const unsigned gm = grid_for(p.persistent_grid, p.nchunks * segs, mcap);
     hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.d_desc, p.d_bufs,
                        p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6, segs);
