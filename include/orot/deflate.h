@@ -4,10 +4,10 @@
  *
  * The types, values and entry points below keep the meaning of orot's
  * include/orot/deflate.h; each declaration cites the orot line it replaces.
- * Entry points of orot that are not part of the GPU compress path
- * (deflate_decompress, the streaming API, deflate_set_allocator) are not
- * exported by this library.  deflate_compress is served by the chunked GPU
- * pipeline (its stream differs from orot's single-block one).
+ * Entry points of orot that are not part of the GPU path (the streaming
+ * API, deflate_set_allocator) are not exported by this library.
+ * deflate_compress is served by the chunked GPU pipeline (its stream differs
+ * from orot's single-block one); deflate_decompress by a GPU inflate.
  *
  * Deviations from orot, all deliberate:
  *   - gzip trailers carry the IEEE CRC-32 on every host (orot's x86 build
@@ -15,7 +15,10 @@
  *   - deflate_parallel_compress returns 0 instead of writing past
  *     out_capacity when the compressed stream does not fit;
  *   - num_threads is accepted and ignored (the GPU decides its parallelism);
- *   - levels 1-3 accept block sizes up to 256 KiB (L1) / 128 KiB (L2, L3).
+ *   - levels 1-3 accept block sizes up to 256 KiB (L1) / 128 KiB (L2, L3);
+ *   - deflate_decompress reports an output buffer that is too small as
+ *     DEFLATE_NEED_OUTPUT, as orot does with its libdeflate backend (its
+ *     built-in inflater reports DEFLATE_DATA_ERROR).
  */
 #ifndef OROT_DEFLATE_H
 #define OROT_DEFLATE_H
@@ -68,6 +71,16 @@ size_t deflate_compress_bound(size_t in_size, deflate_format format);
  * chunk size), not as orot's single block. */
 size_t deflate_compress(const void* in, size_t in_size, void* out, size_t out_capacity, int level,
                         deflate_format format);
+
+/* Whole-buffer inflate (orot include/orot/deflate.h:79-89,
+ * src/api/deflate_api.cpp:138-215 with the framing checks of
+ * src/formats/zlib_wrapper.cpp and gzip_wrapper.cpp:76-138).  DEFLATE_OK with
+ * *actual_out_size set; DEFLATE_DATA_ERROR for a corrupt or truncated stream,
+ * a bad header or a checksum / ISIZE mismatch; DEFLATE_NEED_OUTPUT when the
+ * output does not fit out_capacity; DEFLATE_PARAM_ERROR for bad arguments;
+ * DEFLATE_MEM_ERROR when the GPU is unusable. */
+deflate_result deflate_decompress(const void* in, size_t in_size, void* out, size_t out_capacity,
+                                  size_t* actual_out_size, deflate_format format);
 
 /* Parallel compressor (orot include/orot/deflate.h:120-145,
  * src/api/stream_api.cpp:384-406).  block_size 0 selects 128 KiB for levels
@@ -149,6 +162,18 @@ inline uint32_t adler32(std::span<const uint8_t> data, uint32_t init = 1) {
 }
 inline uint32_t crc32(std::span<const uint8_t> data, uint32_t init = 0) {
     return deflate_crc32(init, data.data(), data.size());
+}
+
+/* orot include/orot/deflate.h:238-256: one call into a max_output buffer. */
+inline std::vector<uint8_t> decompress(std::span<const uint8_t> input, Format format = Format::Zlib,
+                                       size_t max_output = 256ULL * 1024 * 1024) {
+    std::vector<uint8_t> out(max_output);
+    size_t actual = 0;
+    const deflate_result r = deflate_decompress(input.data(), input.size(), out.data(), out.size(), &actual,
+                                                static_cast<deflate_format>(format));
+    if (r != DEFLATE_OK) throw Error(r, "deflate::decompress failed");
+    out.resize(actual);
+    return out;
 }
 
 }  // namespace deflate

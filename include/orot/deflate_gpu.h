@@ -27,6 +27,25 @@ size_t deflate_parallel_compress_device(deflate_parallel_ctx* ctx, const void* d
 void deflate_parallel_set_timing(deflate_parallel_ctx* ctx, int on);
 int deflate_parallel_stage_times(const deflate_parallel_ctx* ctx, float* ms, int max_values);
 
+/* deflate_decompress with d_in / d_out in device memory; blocks until done. */
+deflate_result deflate_decompress_device(const void* d_in, size_t in_size, void* d_out, size_t out_capacity,
+                                         size_t* actual_out_size, deflate_format format);
+
+/* Batched inflate of `count` independent streams of one format, all in
+ * device memory: stream i is d_in[in_off, in_off + in_size) and inflates into
+ * d_out[out_off, out_off + out_capacity).  results[i] receives the
+ * deflate_decompress result of stream i and out_sizes[i] its size (0 unless
+ * DEFLATE_OK); both are host arrays.  Returns 0, or -1 when the batch could
+ * not run (message in orot_gpu_last_error()). */
+typedef struct deflate_inflate_item {
+    uint64_t in_off;
+    uint64_t in_size;
+    uint64_t out_off;
+    uint64_t out_capacity;
+} deflate_inflate_item;
+int deflate_batch_decompress_device(const void* d_in, void* d_out, const deflate_inflate_item* items, size_t count,
+                                    deflate_format format, uint64_t* out_sizes, int32_t* results);
+
 /* Device memory helpers (hipMalloc / hipFree / hipMemcpy / sync).
  * kind: 0 host->device, 1 device->host, 2 device->device.  Return 0 on
  * success. */

@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(_HERE, "liborot_amd.so")
 EXPORTED_SYMBOLS = (
     "deflate_compress_bound",
     "deflate_compress",
+    "deflate_decompress",
     "deflate_parallel_new",
     "deflate_parallel_free",
     "deflate_parallel_compress",
@@ -31,6 +32,8 @@ EXPORTED_SYMBOLS = (
     "deflate_parallel_compress_device",
     "deflate_parallel_set_timing",
     "deflate_parallel_stage_times",
+    "deflate_decompress_device",
+    "deflate_batch_decompress_device",
     "orot_gpu_malloc",
     "orot_gpu_free",
     "orot_gpu_memcpy",
@@ -64,6 +67,13 @@ BATCH_AUTOTUNE = 2
 
 class BatchItem(ctypes.Structure):
     """deflate_batch_item (include/orot/deflate_batch.h)."""
+
+    _fields_ = [("in_offset", ctypes.c_uint64), ("in_size", ctypes.c_uint64),
+                ("out_offset", ctypes.c_uint64), ("out_capacity", ctypes.c_uint64)]
+
+
+class InflateItem(ctypes.Structure):
+    """deflate_inflate_item (include/orot/deflate_gpu.h)."""
 
     _fields_ = [("in_offset", ctypes.c_uint64), ("in_size", ctypes.c_uint64),
                 ("out_offset", ctypes.c_uint64), ("out_capacity", ctypes.c_uint64)]
@@ -159,6 +169,13 @@ def lib() -> ctypes.CDLL:
     L.deflate_parallel_set_timing.argtypes = [vp, ctypes.c_int]
     L.deflate_parallel_stage_times.restype = ctypes.c_int
     L.deflate_parallel_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
+    L.deflate_decompress.restype = ctypes.c_int
+    L.deflate_decompress.argtypes = [vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.c_int]
+    L.deflate_decompress_device.restype = ctypes.c_int
+    L.deflate_decompress_device.argtypes = [vp, sz, vp, sz, ctypes.POINTER(sz), ctypes.c_int]
+    L.deflate_batch_decompress_device.restype = ctypes.c_int
+    L.deflate_batch_decompress_device.argtypes = [vp, vp, ctypes.POINTER(InflateItem), sz, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32)]
     L.deflate_crc32.restype = u32
     L.deflate_crc32.argtypes = [u32, vp, sz]
     L.deflate_adler32.restype = u32
@@ -244,6 +261,45 @@ def compress(data, level: int = Level.DEFAULT, fmt: int = Format.GZIP) -> bytes:
     if n == 0:
         raise Error(Result.BUF_ERROR, f"deflate_compress: {last_error()}")
     return out[:n].tobytes()
+
+
+def decompress_into(data, out: np.ndarray, fmt: int = Format.GZIP) -> tuple[int, int]:
+    """deflate_decompress (orot include/orot/deflate.h:79-89) into a
+    preallocated uint8 array: returns (result code, decompressed size)."""
+    a = _as_u8(data)
+    got = ctypes.c_size_t(0)
+    r = lib().deflate_decompress(a.ctypes.data if a.size else None, a.size, out.ctypes.data if out.size else None,
+                                 out.size, ctypes.byref(got), int(fmt))
+    return int(r), int(got.value)
+
+
+def decompress(data, fmt: int = Format.GZIP, max_output: int = 256 << 20) -> bytes:
+    """orot::deflate::decompress (orot include/orot/deflate.h:238-256), with
+    the same max_output guard; unlike orot's single max_output-sized call, the
+    buffer starts small and doubles while the result is NEED_OUTPUT."""
+    a = _as_u8(data)
+    cap = min(max_output, max(4 * a.size, 64 << 10))
+    while True:
+        out = np.empty(cap, dtype=np.uint8)
+        r, n = decompress_into(a, out, fmt)
+        if r == Result.OK:
+            return out[:n].tobytes()
+        if r == Result.NEED_OUTPUT and cap < max_output:
+            cap = min(max_output, cap * 2)
+            continue
+        raise Error(r, f"deflate::decompress failed ({last_error()})")
+
+
+def batch_decompress_device(d_in: int, d_out: int, items, fmt: int = Format.GZIP):
+    """deflate_batch_decompress_device: items are (in_offset, in_size,
+    out_offset, out_capacity); returns (sizes, result codes) as numpy arrays."""
+    n = len(items)
+    arr = (InflateItem * max(n, 1))(*[InflateItem(*map(int, it)) for it in items])
+    sizes = (ctypes.c_uint64 * max(n, 1))()
+    res = (ctypes.c_int32 * max(n, 1))()
+    if lib().deflate_batch_decompress_device(d_in, d_out, arr, n, int(fmt), sizes, res) != 0:
+        raise Error(Result.MEM_ERROR, f"deflate_batch_decompress_device: {last_error()}")
+    return np.ctypeslib.as_array(sizes)[:n].copy(), np.ctypeslib.as_array(res)[:n].copy()
 
 
 class ParallelCompressor:

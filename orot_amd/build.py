@@ -26,7 +26,8 @@ COMMON = [
     "-I" + os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["deflate_kernels.hip", "deflate_l6.hip", "engine.cpp", "deflate_api.cpp", "server.cpp"]
+SOURCES = ["deflate_kernels.hip", "deflate_l6.hip", "inflate.hip", "engine.cpp", "deflate_api.cpp",
+           "inflate_api.cpp", "server.cpp"]
 
 
 def _run(cmd, verbose):

@@ -131,6 +131,20 @@ size_t l6_block_record_bytes();
 hipError_t launch_l6_front(const Plan& plan);
 hipError_t launch_l6_back(const Plan& plan);
 
+// Batched inflate (inflate.hip): one stream per item, order[] the items
+// longest first; *d_queue is the device work-queue head (zeroed here).
+struct InflateItem {
+    uint64_t in_off, in_size, out_off, out_cap;
+};
+struct InflateResult {
+    uint64_t out_size;
+    int32_t status;  // deflate_result value
+    uint32_t pad_;
+};
+hipError_t launch_inflate(const uint8_t* d_in, uint8_t* d_out, const InflateItem* d_items, const uint32_t* d_order,
+                          uint32_t count, int format, InflateResult* d_res, uint32_t* d_queue, int cus,
+                          hipStream_t stream);
+
 // Checksums of a device buffer (whole-buffer entry points).
 hipError_t launch_checksum(const uint8_t* d_in, uint64_t n, int kind /*0 crc,1 adler*/, uint32_t init,
                            uint32_t* d_out_value, ChunkInfo* d_tmp, uint32_t tmp_chunks, hipStream_t stream);

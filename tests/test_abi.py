@@ -65,3 +65,27 @@ def test_no_gpu_means_no_context():
     )
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env={**os.environ, "PYTHONPATH": ROOT})
     assert r.returncode == 0
+
+
+def test_decompress_arguments_and_no_gpu():
+    """deflate_decompress rejects bad arguments with DEFLATE_PARAM_ERROR before
+    touching the GPU, and otherwise reports DEFLATE_MEM_ERROR: there is no CPU
+    inflate behind the ABI, and the GPU inflate is switched off."""
+    import subprocess
+    import sys
+
+    code = (
+        "import ctypes,orot_amd,sys,zlib\n"
+        "L=orot_amd.lib()\n"
+        "got=ctypes.c_size_t(0)\n"
+        "buf=ctypes.create_string_buffer(64)\n"
+        "bad=[L.deflate_decompress(None,5,buf,64,ctypes.byref(got),0),\n"
+        "     L.deflate_decompress(buf,5,None,64,ctypes.byref(got),0),\n"
+        "     L.deflate_decompress(buf,5,buf,64,ctypes.byref(got),7)]\n"
+        "s=zlib.compress(b'abc')\n"
+        "r=L.deflate_decompress(s,len(s),buf,64,ctypes.byref(got),1)\n"
+        "ok=all(b==-5 for b in bad) and r==-2\n"
+        "sys.exit(0 if ok else 1)\n"
+    )
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env={**os.environ, "PYTHONPATH": ROOT})
+    assert r.returncode == 0
