@@ -20,8 +20,10 @@
 //   l6_match_kernel   M(x) for every x: the chunk is swept in tiles of 4096
 //                     positions with the last 32 KiB of bytes and chain steps
 //                     in LDS rings.
-//   l6_parse_kernel   lazy decisions per position, parse barriers, exact
-//                     parse, tokens (orot's Token layout) in order.
+//   l6_decide_kernel  lazy decision per position (chunk segments in
+//                     separate workgroups).
+//   l6_parse_kernel   parse barriers, exact parse, tokens (orot's Token
+//                     layout) in order.
 //   l6_block_kernel   per block: histograms, package-merge lengths, code-length
 //                     run-length header, exact sizes, block type.
 //   l6_size_kernel    chunk size for each incoming bit offset mod 8.
@@ -432,6 +434,44 @@ __device__ __forceinline__ uint32_t l6_decide(const uint32_t* ML, uint32_t n, ui
     return (q + b - p) | ((q - p) << 12);
 }
 
+// Decisions CODE[p] for every position of segment `seg` of `segs` of each
+// chunk (coalesced over positions, eight per thread per pass with their match
+// words loaded first, addresses clamped into the chunk).  Items are chunk x
+// segment, grid-stride, counts from the batch header.
+constexpr int L6D_THREADS = 1024;
+__global__ __launch_bounds__(L6D_THREADS) void l6_decide_kernel(const ChunkDesc* __restrict__ desc,
+                                                                const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                                const uint32_t* __restrict__ mlw,
+                                                                uint16_t* __restrict__ codew, const L6Params prm,
+                                                                uint32_t segs) {
+    const uint32_t items = hdr->nchunks * segs;
+    const int lazy = prm.lazy_depth;
+    for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t c = it / segs, seg = it % segs;
+        const uint32_t n = desc[c].len;
+        const uint32_t per = (((n + segs - 1) / segs) + 4095u) & ~4095u;
+        const uint32_t pa = seg * per, pb = min(n, pa + per);
+        if (pa >= pb) continue;
+        const uint64_t cp = l6_stride(chunk);
+        const uint32_t* ML = mlw + (uint64_t)c * cp;
+        uint16_t* CODE = codew + (uint64_t)c * cp;
+        for (uint32_t p0 = pa + threadIdx.x; p0 < pb; p0 += 8 * L6D_THREADS) {
+            uint32_t m0[8], m1[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + (uint32_t)k * L6D_THREADS;
+                m0[k] = ML[min(p, n - 1)];
+                m1[k] = ML[min(p + 1, n - 1)];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t p = p0 + (uint32_t)k * L6D_THREADS;
+                if (p < pb) CODE[p] = (uint16_t)l6_decide(ML, n, p, lazy, m0[k], m1[k]);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
                                               const ChunkDesc* __restrict__ desc, uint32_t chunk,
                                               const uint32_t* __restrict__ mlw, uint16_t* __restrict__ codew,
@@ -448,29 +488,13 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
     uint16_t* CODE = codew + (uint64_t)c * cp;
     uint32_t* TK = toks + (uint64_t)c * cp;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int lazy = prm.lazy_depth;
+    (void)prm;
     const uint32_t S = max(32u, (((n + L6R_THREADS - 1) / L6R_THREADS) + 31u) & ~31u);
     const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
     // bitmap words owned by this range (S is a multiple of 32; an empty range
     // owns none, even when r0 falls inside the last range's final word)
     const uint32_t w0 = r0 >> 5, w1 = r0 < r1 ? (r1 + 31) >> 5 : w0;
-    // decisions (coalesced over positions), eight per thread per pass with
-    // their match words loaded first (addresses clamped into the chunk), and
-    // bitmap clear
-    for (uint32_t p0 = tid; p0 < n; p0 += 8 * L6R_THREADS) {
-        uint32_t m0[8], m1[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t p = p0 + (uint32_t)k * L6R_THREADS;
-            m0[k] = ML[min(p, n - 1)];
-            m1[k] = ML[min(p + 1, n - 1)];
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t p = p0 + (uint32_t)k * L6R_THREADS;
-            if (p < n) CODE[p] = (uint16_t)l6_decide(ML, n, p, lazy, m0[k], m1[k]);
-        }
-    }
+    // the decisions CODE[] come from l6_decide_kernel; clear the bitmap
     for (uint32_t w = w0; w < w1; ++w) bm[w] = 0;
     __syncthreads();
     // the range's codes 32 at a time (r0 is a multiple of 32, so the 16-byte
@@ -1425,6 +1449,11 @@ hipError_t launch_l6_front(const Plan& p) {
     const unsigned gm = grid_for(p.persistent_grid, p.nchunks * segs, mcap);
     hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.d_desc, p.d_bufs,
                        p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6, segs);
+    const uint32_t dcap = p.cus * 2u;  // two 1024-thread workgroups per CU
+    const uint32_t dsegs = l6_segments(p, dcap);
+    const unsigned gd = grid_for(p.persistent_grid, p.nchunks * dsegs, dcap);
+    hipLaunchKernelGGL(l6_decide_kernel, dim3(gd), dim3(L6D_THREADS), 0, p.stream, p.d_desc, p.d_hdr, p.chunk, p.d_tok,
+                       p.d_pd, p.l6, dsegs);
     const unsigned gr = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, l6r_lds(p.chunk), L6R_THREADS));
     hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), l6r_lds(p.chunk), p.stream, p.d_in, p.d_desc, p.d_order,
                        p.d_hdr, p.chunk, p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.l6);
