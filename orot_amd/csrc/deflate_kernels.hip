@@ -992,6 +992,8 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     for (uint32_t p0 = r0; p0 < r1; p0 += 8)
         *reinterpret_cast<uint2*>(code + p0) = *reinterpret_cast<const uint2*>(Q + L.group((uint32_t)tid, (p0 - r0) >> 3));
     for (uint32_t w = tid; w < ((n + 31) >> 5); w += LC_THREADS) bm[w] = 0;
+    uint32_t* run258 = misc + 32;  // bit t: every position of range t steps 258
+    if (tid < (int)(LC_THREADS / 32)) run258[tid] = 0;
     __syncthreads();
 
     // ---- P1: first parse barrier of each range.  reach(p) = p + step(p);
@@ -999,12 +1001,18 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     // every greedy parse visits it.
     {
         uint32_t runmax = 0;
+        bool all258 = r0 < r1;
         for (uint32_t p = r0; p < r1; p += 4) {
             const uint32_t cw = D[p >> 2];
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                if (p + k < r1) runmax = max(runmax, p + k + code_step((cw >> (8 * k)) & 0xFFu, T, L, p + k));
+                if (p + k < r1) {
+                    const uint32_t b = (cw >> (8 * k)) & 0xFFu;
+                    all258 &= b == 254u;
+                    runmax = max(runmax, p + k + code_step(b, T, L, p + k));
+                }
         }
+        if (all258) atomicOr(&run258[tid >> 5], 1u << (tid & 31));
         uint32_t v = runmax;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -1042,6 +1050,9 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
 
     // ---- P2: exact greedy parse (lz77.cpp:378-406) from this range's
     // barrier to the first barrier of a later range; token starts -> bm.
+    // Through ranges whose every step is 258 the token starts are pos + 258 k:
+    // they are visited four at a time without reading their codes, with the
+    // same barrier test and bit per token.
     {
         const uint32_t pstart = fb[tid];
         if (pstart != 0xFFFFFFFFu) {
@@ -1049,7 +1060,28 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             while (pos < n) {
                 if (pos >= r1 && fb[L.range_of(pos)] == pos) break;
                 atomicOr(&bm[pos >> 5], 1u << (pos & 31));
-                pos += code_step(code[pos], T, L, pos);
+                const uint32_t b = code[pos];
+                pos += code_step(b, T, L, pos);
+                if (b != 254u) continue;
+                for (;;) {
+                    uint32_t q[4], t[4], f[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        q[j] = pos + 258u * (uint32_t)j;
+                        t[j] = L.range_of(min(q[j], n - 1));
+                        f[j] = fb[t[j]];
+                    }
+                    int stop = 4;
+#pragma unroll
+                    for (int j = 3; j >= 0; --j) {
+                        const bool run = q[j] < n && ((run258[t[j] >> 5] >> (t[j] & 31)) & 1u) &&
+                                         !(q[j] >= r1 && f[j] == q[j]);
+                        if (!run) stop = j;
+                    }
+                    for (int j = 0; j < stop; ++j) atomicOr(&bm[q[j] >> 5], 1u << (q[j] & 31));
+                    pos += 258u * (uint32_t)stop;
+                    if (stop < 4) break;
+                }
             }
         }
     }
