@@ -1051,7 +1051,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     // ---- P2: exact greedy parse (lz77.cpp:378-406) from this range's
     // barrier to the first barrier of a later range; token starts -> bm.
     // Through ranges whose every step is 258 the token starts are pos + 258 k:
-    // they are visited four at a time without reading their codes, with the
+    // they are visited eight at a time without reading their codes, with the
     // same barrier test and bit per token.
     {
         const uint32_t pstart = fb[tid];
@@ -1064,23 +1064,27 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 pos += code_step(b, T, L, pos);
                 if (b != 254u) continue;
                 for (;;) {
-                    uint32_t q[4], t[4], f[4];
+                    // all loads first (addresses clamped into the chunk), then
+                    // one mask: the run ends at the first start past n, outside
+                    // a 258-step range, or on a barrier of a later range
+                    uint32_t q[8], f[8], fl[8];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
+                    for (int j = 0; j < 8; ++j) {
                         q[j] = pos + 258u * (uint32_t)j;
-                        t[j] = L.range_of(min(q[j], n - 1));
-                        f[j] = fb[t[j]];
+                        const uint32_t t = L.range_of(min(q[j], n - 1));
+                        f[j] = fb[t];
+                        fl[j] = run258[t >> 5] >> (t & 31);
                     }
-                    int stop = 4;
+                    uint32_t ok = 0;
 #pragma unroll
-                    for (int j = 3; j >= 0; --j) {
-                        const bool run = q[j] < n && ((run258[t[j] >> 5] >> (t[j] & 31)) & 1u) &&
-                                         !(q[j] >= r1 && f[j] == q[j]);
-                        if (!run) stop = j;
-                    }
-                    for (int j = 0; j < stop; ++j) atomicOr(&bm[q[j] >> 5], 1u << (q[j] & 31));
-                    pos += 258u * (uint32_t)stop;
-                    if (stop < 4) break;
+                    for (int j = 0; j < 8; ++j)
+                        ok |= ((uint32_t)(q[j] < n) & fl[j] & (uint32_t)(q[j] < r1 || f[j] != q[j]) & 1u) << j;
+                    const uint32_t stop = (uint32_t)__builtin_ctz(~ok);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j)
+                        if ((uint32_t)j < stop) atomicOr(&bm[q[j] >> 5], 1u << (q[j] & 31));
+                    pos += 258u * stop;
+                    if (stop < 8) break;
                 }
             }
         }
