@@ -1034,7 +1034,11 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         uint32_t carry = max(misc[wv], lane ? prev_lane : 0u);
         if (tid == 0) carry = 0;
         uint32_t first = 0xFFFFFFFFu;
-        for (uint32_t p = r0; p < r1 && first == 0xFFFFFFFFu; p += 4) {
+        // a range of 258-byte steps (shorter than 258 positions): a token at
+        // its first position reaches past its end, so only r0 can be a barrier
+        const bool skip = all258 && L.S < 258u;
+        if (skip) first = carry <= r0 ? r0 : 0xFFFFFFFFu;
+        for (uint32_t p = r0; !skip && p < r1 && first == 0xFFFFFFFFu; p += 4) {
             const uint32_t cw = D[p >> 2];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
