@@ -78,7 +78,9 @@ size_t deflate_compress(const void* in, size_t in_size, void* out, size_t out_ca
  * *actual_out_size set; DEFLATE_DATA_ERROR for a corrupt or truncated stream,
  * a bad header or a checksum / ISIZE mismatch; DEFLATE_NEED_OUTPUT when the
  * output does not fit out_capacity; DEFLATE_PARAM_ERROR for bad arguments;
- * DEFLATE_MEM_ERROR when the GPU is unusable. */
+ * DEFLATE_MEM_ERROR when the GPU is unusable.  In this build the GPU inflate
+ * is switched off: after the argument checks the call returns
+ * DEFLATE_MEM_ERROR, with the reason in orot_gpu_last_error(). */
 deflate_result deflate_decompress(const void* in, size_t in_size, void* out, size_t out_capacity,
                                   size_t* actual_out_size, deflate_format format);
 

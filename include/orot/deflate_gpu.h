@@ -36,7 +36,8 @@ deflate_result deflate_decompress_device(const void* d_in, size_t in_size, void*
  * d_out[out_off, out_off + out_capacity).  results[i] receives the
  * deflate_decompress result of stream i and out_sizes[i] its size (0 unless
  * DEFLATE_OK); both are host arrays.  Returns 0, or -1 when the batch could
- * not run (message in orot_gpu_last_error()). */
+ * not run (message in orot_gpu_last_error()).  In this build the GPU inflate
+ * is switched off: both entry points fail after their argument checks. */
 typedef struct deflate_inflate_item {
     uint64_t in_off;
     uint64_t in_size;
