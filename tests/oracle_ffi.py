@@ -43,6 +43,8 @@ def oracle() -> ctypes.CDLL:
         L.oracle_gpu_parallel_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz, _vp, _sz, _vp, _sz]
         L.oracle_l6_lz77.restype = _sz
         L.oracle_l6_lz77.argtypes = [_vp, _sz, ctypes.c_int, _vp]
+        L.oracle_l6_match.restype = None
+        L.oracle_l6_match.argtypes = [_vp, _sz, ctypes.c_int, _vp, _vp]
         L.oracle_encode_block_fixed.restype = ctypes.c_int
         L.oracle_encode_block_fixed.argtypes = [_vp, _sz, _vp, _sz, ctypes.c_int, _vp, _vp]
         L.oracle_limited_lengths.restype = ctypes.c_int
@@ -129,6 +131,15 @@ def l6_lz77(data, level: int) -> np.ndarray:
     t = np.zeros(max(1, a.size), dtype=np.uint32)
     n = oracle().oracle_l6_lz77(a.ctypes.data, a.size, level, t.ctypes.data)
     return t[:n]
+
+
+def l6_match(data, level: int):
+    """M(x) of the L6 class for every position: (lengths, distances)."""
+    a = _u8(data)
+    ml = np.zeros(max(1, a.size), dtype=np.uint32)
+    md = np.zeros(max(1, a.size), dtype=np.uint32)
+    oracle().oracle_l6_match(a.ctypes.data, a.size, level, ml.ctypes.data, md.ctypes.data)
+    return ml[:a.size], md[:a.size]
 
 
 def limited_lengths(freq, max_bits: int) -> np.ndarray:

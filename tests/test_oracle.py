@@ -282,3 +282,92 @@ def test_oracle_equals_orot_build(kind):
     for level in (0, 1, 2, 3):
         for fmt in (0, 1, 2):
             assert orc.parallel_compress(data, level, fmt) == orc.ref_parallel_compress(data, level, fmt)
+
+
+# ---- L6-class matcher against a brute-force restatement of its definition:
+# the chain of x is every earlier position with the same 12-bit hash (whole
+# 4-byte window in the buffer), nearest first; at most max_chain of them are
+# examined, the walk ends at the first one more than 32 KiB back, a candidate
+# counts only when its first 4 bytes equal x's, a longer match replaces the
+# best (ties keep the nearer), and the walk stops at nice_len.
+_L6_LEVELS = {4: (16, 128), 6: (32, 258), 9: (256, 258)}
+
+
+def _h12(b: bytes, p: int) -> int:
+    return ((int.from_bytes(b[p:p + 4], "little") * 0x9E3779B1) & 0xFFFFFFFF) >> 20
+
+
+def _brute_l6_match(b: bytes, level: int):
+    max_chain, nice = _L6_LEVELS[level]
+    n = len(b)
+    by_hash = {}
+    ml, md = [0] * n, [0] * n
+    for x in range(n):
+        if x + 4 > n:
+            continue
+        h = _h12(b, x)
+        chain = by_hash.get(h, [])
+        best = bd = 0
+        cap = min(258, n - x)
+        for k, q in enumerate(reversed(chain)):
+            if k >= max_chain or x - q > 32768:
+                break
+            if b[q:q + 4] != b[x:x + 4]:
+                continue
+            l = 0
+            while l < cap and b[q + l] == b[x + l]:
+                l += 1
+            if l > best:
+                best, bd = l, x - q
+                if l >= nice:
+                    break
+        if best >= 3:
+            ml[x], md[x] = best, bd
+        by_hash.setdefault(h, []).append(x)
+    return ml, md
+
+
+@pytest.mark.parametrize("level", sorted(_L6_LEVELS))
+@pytest.mark.parametrize("kind", ["text", "records", "zeros", "random"])
+def test_l6_match_bruteforce(level, kind):
+    d = corpus.generate(kind, 6000, 41 + level).tobytes()
+    # a repeated piece far back exercises the 32 KiB window
+    d = d + corpus.generate("random", 30000, 5).tobytes() + d[:3000]
+    ml, md = orc.l6_match(d, level)
+    bml, bmd = _brute_l6_match(d, level)
+    assert list(ml) == bml and list(md) == bmd
+
+
+@pytest.mark.parametrize("level", [4, 6, 9])
+def test_l6_tokens_rebuild_input(level):
+    """Invariant of the token stream: literals and (length, distance) copies
+    rebuild the input, every distance is within 32 KiB and every length is
+    in [3, 258]."""
+    d = np.concatenate([corpus.generate(k, 20000, 8) for k in corpus.GENERATORS]).tobytes()
+    out = bytearray()
+    for t in orc.l6_lz77(d, level):
+        t = int(t)
+        if t & 0x80000000:
+            L, dist = (t >> 16) & 0x1FF, t & 0xFFFF
+            assert 3 <= L <= 258 and 1 <= dist <= 32768 and dist <= len(out)
+            for _ in range(L):
+                out.append(out[-dist])
+        else:
+            out.append(t & 0xFF)
+    assert bytes(out) == d
+
+
+def test_l6_match_window_edge():
+    """A copy exactly 32768 bytes back is found; one 32769 back is not (the
+    window of RFC 1951 distances), and the brute force agrees everywhere."""
+    rng = np.random.default_rng(77)
+    a = rng.integers(0, 256, 200, dtype=np.uint8).tobytes()
+    fill = rng.integers(0, 256, 32768 - 200, dtype=np.uint8).tobytes()
+    near = a + fill + a                     # second copy at distance 32768
+    far = a + fill + b"\x00" + a            # second copy at distance 32769
+    for level in (4, 6):
+        ml, md = orc.l6_match(near, level)
+        assert ml[32768] >= 128 and md[32768] == 32768
+        ml2, md2 = orc.l6_match(far, level)
+        assert md2[32769] != 32769
+        assert (list(ml), list(md)) == _brute_l6_match(near, level)
