@@ -5,7 +5,7 @@
  * The types, values and entry points below keep the meaning of orot's
  * include/orot/deflate.h; each declaration cites the orot line it replaces.
  * Entry points of orot that are not part of the GPU path (the streaming
- * API, deflate_set_allocator) are not exported by this library.
+ * decompressor, deflate_set_allocator) are not exported by this library.
  * deflate_compress is served by the chunked GPU pipeline (its stream differs
  * from orot's single-block one); deflate_decompress by a GPU inflate.
  *
@@ -18,7 +18,10 @@
  *   - levels 1-3 accept block sizes up to 256 KiB (L1) / 128 KiB (L2, L3);
  *   - deflate_decompress reports an output buffer that is too small as
  *     DEFLATE_NEED_OUTPUT, as orot does with its libdeflate backend (its
- *     built-in inflater reports DEFLATE_DATA_ERROR).
+ *     built-in inflater reports DEFLATE_DATA_ERROR);
+ *   - the streaming compressor cuts its input at whole chunks of the
+ *     level's block size and ends a sync or full flush with an empty stored
+ *     block, so its streams stay valid past orot's single 128 KiB block.
  */
 #ifndef OROT_DEFLATE_H
 #define OROT_DEFLATE_H
@@ -49,6 +52,14 @@ typedef enum deflate_format {
     DEFLATE_FORMAT_ZLIB = 1, /* RFC 1950, Adler-32 trailer */
     DEFLATE_FORMAT_GZIP = 2  /* RFC 1952, CRC-32 + ISIZE trailer */
 } deflate_format;
+
+/* Flush modes (orot include/orot/deflate.h:42-47). */
+typedef enum deflate_flush {
+    DEFLATE_NO_FLUSH = 0,   /* collect input */
+    DEFLATE_SYNC_FLUSH = 1, /* all input so far decodable, output byte-aligned */
+    DEFLATE_FULL_FLUSH = 2, /* as SYNC_FLUSH, and later matches do not reach back past it */
+    DEFLATE_FINISH = 3      /* final block and trailer */
+} deflate_flush;
 
 /* Levels (orot include/orot/deflate.h:35-39); values are clamped to 0..12. */
 #define DEFLATE_LEVEL_STORE 0
@@ -97,6 +108,28 @@ void deflate_parallel_free(deflate_parallel_ctx* ctx);
  * until done.  Returns the compressed size or 0 on failure. */
 size_t deflate_parallel_compress(deflate_parallel_ctx* ctx, const void* in, size_t in_size, void* out,
                                  size_t out_capacity);
+
+/* Streaming compressor (orot include/orot/deflate.h:92-108,
+ * src/api/stream_api.cpp:134-247, src/compress/compressor.cpp:21-97).
+ * Input is collected in pinned host memory and compressed on the GPU in
+ * pieces of whole chunks (32 MiB), each piece's bits continuing the last.
+ * Fed with DEFLATE_NO_FLUSH up to DEFLATE_FINISH, the stream is byte for byte
+ * what deflate_parallel_compress makes of the concatenated input with the
+ * level's default block size; for 1000 bytes to 128 KiB of input that is
+ * also orot's own stream.  SYNC and FULL flushes compress what is pending and
+ * append an empty stored block (00 00 FF FF); a FULL flush also drops the
+ * dictionary of levels >= 4.  Results: DEFLATE_OK (input consumed, output
+ * drained), DEFLATE_NEED_OUTPUT (call again with more output space),
+ * DEFLATE_STREAM_END (after DEFLATE_FINISH, once everything is out),
+ * DEFLATE_PARAM_ERROR (NULL pointers, unknown flush), DEFLATE_MEM_ERROR (a
+ * GPU failure; the stream is unusable afterwards).  deflate_stream_new
+ * returns NULL when no GPU is usable. */
+typedef struct deflate_stream deflate_stream;
+
+deflate_stream* deflate_stream_new(int level, deflate_format format);
+void deflate_stream_free(deflate_stream* s);
+deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_in, size_t* avail_in,
+                                       uint8_t** next_out, size_t* avail_out, deflate_flush flush);
 
 /* Checksums (orot include/orot/deflate.h:149-150). */
 uint32_t deflate_adler32(uint32_t initial, const void* data, size_t len);
@@ -157,6 +190,44 @@ public:
 
 private:
     deflate_parallel_ctx* ctx_;
+};
+
+/* orot include/orot/deflate.h:258-304: feed() with DEFLATE_NO_FLUSH, finish()
+ * with DEFLATE_FINISH; each returns the bytes written to out. */
+class Compressor {
+public:
+    explicit Compressor(int level = Level::Default, Format fmt = Format::Zlib)
+        : s_(deflate_stream_new(level, static_cast<deflate_format>(fmt))) {
+        if (!s_) throw Error(DEFLATE_MEM_ERROR, "Compressor: allocation failed");
+    }
+    ~Compressor() { deflate_stream_free(s_); }
+    Compressor(const Compressor&) = delete;
+    Compressor& operator=(const Compressor&) = delete;
+
+    size_t feed(std::span<const uint8_t> in, std::span<uint8_t> out) {
+        const uint8_t* next_in = in.data();
+        size_t avail_in = in.size();
+        uint8_t* next_out = out.data();
+        size_t avail_out = out.size();
+        const deflate_result r =
+            deflate_stream_compress(s_, &next_in, &avail_in, &next_out, &avail_out, DEFLATE_NO_FLUSH);
+        if (r < 0) throw Error(r, "Compressor::feed failed");
+        return out.size() - avail_out;
+    }
+
+    size_t finish(std::span<uint8_t> out) {
+        const uint8_t* next_in = nullptr;
+        size_t avail_in = 0;
+        uint8_t* next_out = out.data();
+        size_t avail_out = out.size();
+        const deflate_result r =
+            deflate_stream_compress(s_, &next_in, &avail_in, &next_out, &avail_out, DEFLATE_FINISH);
+        if (r != DEFLATE_OK && r != DEFLATE_STREAM_END) throw Error(r, "Compressor::finish failed");
+        return out.size() - avail_out;
+    }
+
+private:
+    deflate_stream* s_;
 };
 
 inline uint32_t adler32(std::span<const uint8_t> data, uint32_t init = 1) {

@@ -50,6 +50,9 @@ EXPORTED_SYMBOLS = (
     "deflate_batch_autotune",
     "deflate_batch_set_timing",
     "deflate_batch_stage_times",
+    "deflate_stream_new",
+    "deflate_stream_free",
+    "deflate_stream_compress",
     "deflate_server_new",
     "deflate_server_free",
     "deflate_server_submit_device",
@@ -177,6 +180,13 @@ def lib() -> ctypes.CDLL:
     L.deflate_batch_decompress_device.argtypes = [vp, vp, ctypes.POINTER(InflateItem), sz, ctypes.c_int,
                                                   ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32)]
     L.deflate_crc32.restype = u32
+    L.deflate_stream_new.restype = vp
+    L.deflate_stream_new.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.deflate_stream_free.restype = None
+    L.deflate_stream_free.argtypes = [vp]
+    L.deflate_stream_compress.restype = ctypes.c_int
+    L.deflate_stream_compress.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(sz), ctypes.POINTER(vp),
+                                          ctypes.POINTER(sz), ctypes.c_int]
     L.deflate_crc32.argtypes = [u32, vp, sz]
     L.deflate_adler32.restype = u32
     L.deflate_adler32.argtypes = [u32, vp, sz]
@@ -341,6 +351,78 @@ class ParallelCompressor:
         if self._ctx:
             lib().deflate_parallel_free(self._ctx)
             self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Flush(enum.IntEnum):
+    """deflate_flush (orot include/orot/deflate.h:42-47)."""
+    NO_FLUSH = 0
+    SYNC_FLUSH = 1
+    FULL_FLUSH = 2
+    FINISH = 3
+
+
+class Compressor:
+    """orot::deflate::Compressor (orot include/orot/deflate.h:258-304) over
+    deflate_stream_new / deflate_stream_compress: input collects on the host
+    and compresses on the GPU in pieces of whole chunks."""
+
+    def __init__(self, level: int = Level.DEFAULT, fmt: int = Format.ZLIB):
+        self._s = lib().deflate_stream_new(int(level), int(fmt))
+        if not self._s:
+            raise Error(Result.MEM_ERROR, f"Compressor: allocation failed ({last_error()})")
+
+    def compress_call(self, data, out: np.ndarray, flush: int = Flush.NO_FLUSH):
+        """One deflate_stream_compress call: returns (result, input bytes
+        consumed, output bytes written)."""
+        src = _as_u8(data)
+        ni = ctypes.c_void_p(src.ctypes.data if src.size else None)
+        ai = ctypes.c_size_t(src.size)
+        no = ctypes.c_void_p(out.ctypes.data if out.size else None)
+        ao = ctypes.c_size_t(out.size)
+        r = lib().deflate_stream_compress(self._s, ctypes.byref(ni), ctypes.byref(ai), ctypes.byref(no),
+                                          ctypes.byref(ao), int(flush))
+        return int(r), src.size - ai.value, out.size - ao.value
+
+    def write(self, data, flush: int = Flush.NO_FLUSH, out_chunk: int = 1 << 20) -> bytes:
+        """Feeds all of `data` with `flush`, calling again while the result is
+        NEED_OUTPUT; returns every output byte produced."""
+        src = _as_u8(data)
+        parts, pos = [], 0
+        out = np.empty(out_chunk, dtype=np.uint8)
+        while True:
+            r, used, n = self.compress_call(src[pos:], out, flush)
+            pos += used
+            parts.append(out[:n].tobytes())
+            if r < 0:
+                raise Error(r, f"deflate_stream_compress failed ({last_error()})")
+            if r == Result.STREAM_END or (r == Result.OK and pos == src.size):
+                return b"".join(parts)
+
+    def feed(self, data) -> bytes:
+        return self.write(data, Flush.NO_FLUSH)
+
+    def flush(self, full: bool = False) -> bytes:
+        return self.write(b"", Flush.FULL_FLUSH if full else Flush.SYNC_FLUSH)
+
+    def finish(self) -> bytes:
+        return self.write(b"", Flush.FINISH)
+
+    def close(self) -> None:
+        if self._s:
+            lib().deflate_stream_free(self._s)
+            self._s = None
 
     def __del__(self):
         try:

@@ -15,13 +15,28 @@
 namespace orot_gpu {
 
 constexpr uint32_t CHUNK_LAST = 0x80000000u;  // ChunkDesc::buf flag: last chunk of its buffer
+constexpr uint32_t CHUNK_OPEN = 0x40000000u;  // ... of a stream piece that does not end the stream
+constexpr uint32_t CHUNK_BUF = 0x3FFFFFFFu;   // the buffer index bits
+
+// The chunk's last block carries BFINAL: it ends its buffer and the buffer is
+// not an open stream piece.
+__host__ __device__ constexpr bool chunk_final(uint32_t buf) {
+    return (buf & (CHUNK_LAST | CHUNK_OPEN)) == CHUNK_LAST;
+}
 
 // One chunk of one buffer.
 struct ChunkDesc {
     uint64_t in_off;  // first byte of the chunk in the input arena
     uint32_t len;     // bytes
-    uint32_t buf;     // buffer index | CHUNK_LAST
+    uint32_t buf;     // buffer index | CHUNK_LAST | CHUNK_OPEN
 };
+
+// BufDesc::piece: the buffer is one piece of a stream.  Its output is the raw
+// bitstream continuing a previous piece: no header or trailer, bits starting
+// at offset (piece & 7) of byte 0, whose lower bits are the held bits
+// (piece >> 8) & 0xFF; the result's checksum is the piece's own.
+constexpr uint32_t PIECE = 0x80000000u;
+constexpr uint32_t PIECE_OPEN = 0x40000000u;  // the piece does not end the stream
 
 // One buffer of the batch.
 struct BufDesc {
@@ -31,6 +46,8 @@ struct BufDesc {
     uint64_t cap;      // output capacity
     uint32_t chunk0;   // its chunks are [chunk0, chunk0 + nchunks)
     uint32_t nchunks;
+    uint32_t piece;    // 0, or PIECE | PIECE_OPEN? | held bits << 8 | bit offset
+    uint32_t hist;     // bytes before in_off that precede the buffer in its stream (<= 32 KiB)
 };
 
 // Work-queue heads of the persistent kernels.

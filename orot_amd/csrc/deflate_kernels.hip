@@ -512,7 +512,7 @@ __device__ __forceinline__ void l1_lz77_body(uint32_t c, uint32_t* smem, const u
     __syncthreads();
     const uint32_t excl = scan[wv] + incl - mybits;
     const uint64_t fixed = 3ull + scan[L1_WAVES] + 7ull;
-    const bool is_last = (desc[c].buf & CHUNK_LAST) != 0;
+    const bool is_last = chunk_final(desc[c].buf);
     const bool stored = (uint64_t)(n + 5) * 8ull <= fixed;
     if (tid == 0) {
         info[c].fixed_bits = fixed;
@@ -1177,7 +1177,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     __syncthreads();
     const uint32_t excl = misc[wv] + incl - mybits;
     const uint64_t fixed = 3ull + misc[LC_WAVES] + 7ull;
-    const bool is_last = (desc[c].buf & CHUNK_LAST) != 0;
+    const bool is_last = chunk_final(desc[c].buf);
     const bool stored = (uint64_t)(n + 5) * 8ull <= fixed;
     if (tid == 0) {
         info[c].fixed_bits = fixed;
@@ -1306,7 +1306,9 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(ChunkInfo* __rest
     const uint32_t nbufs = hdr->nbufs;
     for (uint32_t bi = blockIdx.x; bi < nbufs; bi += gridDim.x) {
         const BufDesc B = bufs[bi];
-        uint64_t carry = 0;
+        const bool piece = (B.piece & PIECE) != 0;
+        // a stream piece starts at the bit offset its stream has reached
+        uint64_t carry = piece ? (B.piece & 7u) : 0u;
         for (uint32_t t0 = 0; t0 < B.nchunks; t0 += FIN_THREADS) {
             const uint32_t i = t0 + (uint32_t)tid;
             const uint32_t cnt = min((uint32_t)FIN_THREADS, B.nchunks - t0);
@@ -1379,15 +1381,15 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(ChunkInfo* __rest
                 const uint32_t s2 = (uint32_t)((n % ADLER_MOD + red1[0]) % ADLER_MOD);
                 ck = (s2 << 16) | s1;
             }
-            const uint32_t hb = format == 1 ? 2u : (format == 2 ? 10u : 0u);
-            const uint32_t tb = format == 1 ? 4u : (format == 2 ? 8u : 0u);
+            const uint32_t hb = piece ? 0u : (format == 1 ? 2u : (format == 2 ? 10u : 0u));
+            const uint32_t tb = piece ? 0u : (format == 1 ? 4u : (format == 2 ? 8u : 0u));
             const uint64_t payload = (payload_bits + 7) >> 3;
             const uint64_t total = hb + payload + tb;
             Result& R = res[bi];
             R.payload_bits = payload_bits;
             R.checksum = ck;
             R.total_bytes = total <= B.cap ? total : 0;
-            if (total <= B.cap) {
+            if (total <= B.cap && !piece) {
                 uint8_t* out = out_arena + B.out_off;
                 if (format == 1) {
                     out[0] = 0x78;
@@ -1419,7 +1421,8 @@ struct ChunkView {
     uint64_t off;
     uint32_t len;
     uint32_t stored;
-    uint32_t last;
+    uint32_t last;  // last chunk of its buffer
+    uint32_t fin;   // ... and it ends the stream (BFINAL)
 };
 
 __device__ __forceinline__ uint32_t stored_byte(const ChunkView& cv, uint64_t j) {
@@ -1440,7 +1443,7 @@ __device__ __forceinline__ uint32_t stored_byte(const ChunkView& cv, uint64_t j)
         Lk = min(L - k * 65535u, 65535u);
         dbase = (uint64_t)k * 65535u;
     }
-    if (q == 0) return (cv.last && k == pieces - 1) ? 1u : 0u;  // header byte of pieces >= 1
+    if (q == 0) return (cv.fin && k == pieces - 1) ? 1u : 0u;  // header byte of pieces >= 1
     if (q == 1) return Lk & 0xFFu;
     if (q == 2) return (Lk >> 8) & 0xFFu;
     if (q == 3) return (~Lk) & 0xFFu;
@@ -1483,7 +1486,7 @@ __device__ __forceinline__ uint32_t chunk_bits(const ChunkView& cv, uint64_t r, 
     const uint64_t pad0 = (8 - ((cv.off + 3) & 7)) & 7;
     if (r < 3 + pad0) {
         const uint32_t pieces = cv.len == 0 ? 1u : (cv.len + 65534u) / 65535u;
-        const uint32_t hv = (cv.last && pieces == 1) ? 1u : 0u;
+        const uint32_t hv = (cv.fin && pieces == 1) ? 1u : 0u;
         return (r == 0 ? hv : 0u) & mask;
     }
     return stored_byte(cv, (r - 3 - pad0) >> 3) & mask;
@@ -1498,6 +1501,7 @@ __device__ __forceinline__ ChunkView make_view(const uint8_t* src, const ChunkDe
     v.len = info[c].raw_len;
     v.stored = info[c].is_stored == 1 ? 1u : 0u;
     v.last = (desc[c].buf & CHUNK_LAST) ? 1u : 0u;
+    v.fin = chunk_final(desc[c].buf) ? 1u : 0u;
     return v;
 }
 
@@ -1513,7 +1517,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
                                                        uint32_t hdr_bytes) {
     const uint32_t nchunks = hdr->nchunks;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const uint32_t bi = desc[c].buf & ~CHUNK_LAST;
+        const uint32_t bi = desc[c].buf & CHUNK_BUF;
         if (res[bi].total_bytes == 0) continue;
         const ChunkView cv = make_view(src, desc, stage, stage_words, info, c);
         const bool has_next = !cv.last;
@@ -1522,7 +1526,14 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
         const uint64_t o = cv.off;
         const uint64_t e = has_next ? nv.off : res[bi].payload_bits;
         const uint64_t b0 = (o + 7) >> 3, b1 = (e + 7) >> 3;
-        uint8_t* dst = out_arena + bufs[bi].out_off + hdr_bytes;
+        const uint32_t piece = bufs[bi].piece;
+        uint8_t* dst = out_arena + bufs[bi].out_off + ((piece & PIECE) ? 0u : hdr_bytes);
+        // a stream piece's first chunk starts inside byte 0, below the held
+        // bits of the stream (every chunk is at least 10 bits, so byte 0
+        // holds no bits of a second chunk)
+        if ((piece & PIECE) && c == bufs[bi].chunk0 && (o & 7) && threadIdx.x == 0)
+            dst[o >> 3] = (uint8_t)(((piece >> 8) & ((1u << (o & 7)) - 1u)) |
+                                    (chunk_bits(cv, 0, 8 - (uint32_t)(o & 7)) << (o & 7)));
         auto byte_at = [&](uint64_t b) -> uint32_t {
             const uint64_t r = 8 * b - o;
             const uint32_t k = (uint32_t)min<uint64_t>(8, e - 8 * b);

@@ -65,12 +65,15 @@ constexpr uint32_t BLK_SYMS = 16384;
 
 // Cross-chunk dictionary: a chunk that is not the first of its buffer sees up
 // to L6_HALO bytes before it as positions -H..-1 (in its chains and as match
-// sources); its tokens cover its own bytes only.  Per-position arrays of the
+// sources); its tokens cover its own bytes only.  The first chunk of a stream
+// piece sees the stream's bytes before the piece (BufDesc::hist of them, laid
+// out just before it in the input arena).  Per-position arrays of the
 // L6 class are laid out with room for the halo.
 constexpr uint32_t L6_HALO = 32768;
 __host__ __device__ constexpr uint64_t l6_stride(uint32_t chunk) { return ((uint64_t)chunk + L6_HALO + 7u) & ~7ull; }
 __device__ __forceinline__ uint32_t l6_halo(uint32_t c, const ChunkDesc* desc, const BufDesc* bufs) {
-    const uint64_t rel = desc[c].in_off - bufs[desc[c].buf & ~CHUNK_LAST].in_off;
+    const BufDesc& B = bufs[desc[c].buf & CHUNK_BUF];
+    const uint64_t rel = desc[c].in_off - B.in_off + B.hist;
     return (uint32_t)min<uint64_t>(rel, L6_HALO);
 }
 constexpr int L6B_THREADS = 256;
@@ -1109,7 +1112,7 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
     __shared__ uint8_t scl[19];
     __shared__ uint8_t sclc[19];
     __shared__ uint64_t s_off;
-    if (res[desc[c].buf & ~CHUNK_LAST].total_bytes == 0) return;
+    if (res[desc[c].buf & CHUNK_BUF].total_bytes == 0) return;
     const uint32_t ntok = ntok_in[c];
     const uint32_t nblk = ntok ? (ntok + BLK_SYMS - 1) / BLK_SYMS : 1;
     if (b >= nblk) return;
@@ -1120,7 +1123,7 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
     const uint32_t* TK = toks + (uint64_t)c * cp;
     const uint8_t* s = src + desc[c].in_off;
     uint32_t* stg = stage + (uint64_t)c * stage_words;
-    const bool last = (desc[c].buf & CHUNK_LAST) != 0 && (b == nblk - 1);
+    const bool last = chunk_final(desc[c].buf) && (b == nblk - 1);
     if (tid == 0) {
         const uint64_t r = info[c].bit_off & 7;
         uint64_t off = r;

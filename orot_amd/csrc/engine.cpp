@@ -252,10 +252,13 @@ bool Engine::build_tables(const BufferSpec* bufs, size_t count, uint32_t& nchunk
         bd[i].cap = b.cap;
         bd[i].chunk0 = c;
         bd[i].nchunks = (uint32_t)nch;
+        bd[i].piece = b.piece;
+        bd[i].hist = b.hist;
+        const uint32_t last = CHUNK_LAST | ((b.piece & PIECE_OPEN) ? CHUNK_OPEN : 0u);
         for (uint64_t j = 0; j < nch; ++j, ++c) {
             cd[c].in_off = b.in_off + j * chunk;
             cd[c].len = (uint32_t)std::min<uint64_t>(chunk, b.in_len - j * chunk);
-            cd[c].buf = (uint32_t)i | (j + 1 == nch ? CHUNK_LAST : 0u);
+            cd[c].buf = (uint32_t)i | (j + 1 == nch ? last : 0u);
         }
     }
     nchunks = c;

@@ -20,6 +20,8 @@ struct BufferSpec {
     uint64_t in_len;
     uint64_t out_off;
     uint64_t cap;
+    uint32_t piece = 0;  // BufDesc::piece: 0 for a whole stream
+    uint32_t hist = 0;   // stream bytes before in_off in the arena (L6 class dictionary)
 };
 
 struct EngineConfig {
@@ -57,6 +59,8 @@ public:
     bool wait(uint64_t* sizes);
     // True when the last submit has finished (non-blocking).
     bool done() const;
+    // Results of the last submit, valid after wait().
+    const Result* results() const { return h_res_; }
 
     void set_timing(bool on) { timing_ = on; }
     // Device time of the last submit: 4 stages without a graph, the whole

@@ -1,0 +1,271 @@
+// stream_api.cpp -- orot's streaming compressor (include/orot/deflate.h:92-108,
+// src/api/stream_api.cpp:134-247 over src/compress/compressor.cpp:21-97) on
+// the GPU pipeline.
+//
+// Input collects in pinned host memory.  When a full piece of PIECE_TARGET bytes
+// (a whole number of chunks) is held and more input arrives, or on a flush,
+// the pending bytes go to the device and compress as one buffer in stream
+// piece mode: raw bitstream, no framing, bits continuing at the stream's
+// current bit offset, and the last block final only at DEFLATE_FINISH.
+// Pieces are cut at chunk boundaries of the whole stream, so a stream fed
+// without flushes is, byte for byte, what deflate_parallel_compress makes of
+// the concatenated input (the L6 class gets the 32 KiB before each piece as
+// its dictionary, as a chunk inside one buffer does).
+//
+// orot's Compressor accumulates 128 KiB blocks with a BitWriter flushed to a
+// byte after each one and no marker, so its streams are valid only while the
+// whole input fits one block and no flush is requested; a sync or full flush
+// here appends the empty stored block of RFC 1951 3.2.4 (00 00 FF FF), as
+// zlib's Z_SYNC_FLUSH does, and the stream stays valid across any number of
+// pieces and flushes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "../../include/orot/deflate.h"
+#include "deflate_gpu.h"
+#include "engine.h"
+#include "host_common.h"
+
+using orot_gpu::BufferSpec;
+using orot_gpu::Engine;
+using orot_gpu::EngineConfig;
+using orot_gpu::set_error;
+using orot_gpu::set_error_msg;
+
+namespace {
+
+constexpr size_t HIST = 32768;              // dictionary bytes kept before a piece
+constexpr size_t PIECE_TARGET = 32u << 20;  // bytes per GPU piece (rounded to whole chunks)
+constexpr uint32_t MOD_ADLER = 65521u;
+
+// CRC-32 of A||B from crc(A), crc(B) and |B|: crc(A) * x^(8|B|) + crc(B) over
+// GF(2) modulo the reflected polynomial 0xEDB88320 (bit 31 = x^0).
+uint32_t gf2_mul_host(uint32_t a, uint32_t b) {
+    uint32_t p = 0;
+    for (int i = 0; i < 32; ++i) {
+        if (a & 0x80000000u) p ^= b;
+        a <<= 1;
+        b = (b >> 1) ^ (0xEDB88320u & (0u - (b & 1u)));
+    }
+    return p;
+}
+uint32_t crc32_concat(uint32_t ca, uint32_t cb, uint64_t nb) {
+    uint32_t r = 0x80000000u, x = 0x00800000u;  // x^0, x^8
+    for (; nb; nb >>= 1, x = gf2_mul_host(x, x))
+        if (nb & 1) r = gf2_mul_host(x, r);
+    return gf2_mul_host(r, ca) ^ cb;
+}
+// Adler-32 of A||B: s1 = s1a + s1b - 1, s2 = s2a + s2b + |B| (s1a - 1), mod 65521.
+uint32_t adler32_concat(uint32_t aa, uint32_t ab, uint64_t nb) {
+    const uint64_t m = MOD_ADLER, n = nb % m;
+    const uint64_t s1a = aa & 0xFFFFu, s2a = aa >> 16, s1b = ab & 0xFFFFu, s2b = ab >> 16;
+    const uint64_t s1 = (s1a + s1b + m - 1) % m;
+    const uint64_t s2 = (s2a + s2b + n * ((s1a + m - 1) % m)) % m;
+    return (uint32_t)((s2 << 16) | s1);
+}
+
+}  // namespace
+
+struct deflate_stream {
+    Engine eng;
+    int format = 0;
+    size_t piece_cap = 0;       // bytes per piece, a multiple of the chunk
+    uint8_t* h_in = nullptr;    // pinned: HIST dictionary bytes, then the pending piece
+    size_t pend = 0;            // pending input bytes at h_in + HIST
+    size_t hist = 0;            // stream bytes kept just before them (<= HIST)
+    uint8_t* d_in = nullptr;    // device copy of the same layout
+    uint8_t* d_out = nullptr;
+    size_t out_cap = 0;         // device output capacity of one piece
+    uint8_t* h_out = nullptr;   // pinned staging of finished output bytes
+    size_t out_len = 0, out_pos = 0;
+    uint32_t held = 0, nheld = 0;  // bits past the last whole output byte
+    uint64_t total_in = 0;
+    uint64_t dict_in = 0;       // bytes since the stream start or the last full flush
+    uint32_t check = 0;         // running Adler-32 (zlib) or CRC-32 (gzip)
+    bool header = false;        // header queued
+    bool unsynced = false;      // blocks written since the last flush marker
+    bool ended = false;         // final block and trailer queued
+    bool failed = false;
+
+    ~deflate_stream() {
+        if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
+        if (h_in) (void)hipHostFree(h_in);
+        if (h_out) (void)hipHostFree(h_out);
+    }
+
+    void put_byte(uint8_t b) { h_out[out_len++] = b; }
+    void put_bits(uint32_t v, uint32_t n) {  // LSB first, n <= 24
+        held |= v << nheld;
+        nheld += n;
+        while (nheld >= 8) {
+            put_byte((uint8_t)held);
+            held >>= 8;
+            nheld -= 8;
+        }
+    }
+    void align() {
+        if (nheld) put_bits(0, 8 - nheld);
+    }
+
+    // Compresses the pending bytes as one piece (final unless `open`) and
+    // queues its whole bytes; the staging buffer must be empty.
+    bool piece(bool open) {
+        const size_t n = pend;
+        const hipStream_t st = eng.stream();
+        hipError_t e = hipMemcpyAsync(d_in + HIST - hist, h_in + HIST - hist, hist + n, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) {
+            set_error("deflate_stream_compress: hipMemcpy(in)", e);
+            return false;
+        }
+        BufferSpec b{HIST, n, 0, out_cap};
+        b.piece = orot_gpu::PIECE | (open ? orot_gpu::PIECE_OPEN : 0u) | (held << 8) | nheld;
+        b.hist = (uint32_t)hist;
+        uint64_t got = 0;
+        if (!eng.submit(d_in, d_out, &b, 1, false) || !eng.wait(&got)) return false;
+        if (got == 0) {
+            set_error_msg("deflate_stream_compress: piece output exceeds its bound");
+            return false;
+        }
+        const orot_gpu::Result R = eng.results()[0];
+        const uint64_t whole = R.payload_bits >> 3;
+        e = hipMemcpyAsync(h_out + out_len, d_out, got, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            set_error("deflate_stream_compress: hipMemcpy(out)", e);
+            return false;
+        }
+        nheld = (uint32_t)(R.payload_bits & 7);
+        held = nheld ? h_out[out_len + whole] & ((1u << nheld) - 1u) : 0u;
+        out_len += whole;
+        if (format == DEFLATE_FORMAT_ZLIB) check = adler32_concat(check, R.checksum, n);
+        if (format == DEFLATE_FORMAT_GZIP) check = crc32_concat(check, R.checksum, n);
+        total_in += n;
+        dict_in += n;
+        // the last HIST bytes since the start or the last full flush become
+        // the next piece's dictionary
+        const size_t keep = std::min<uint64_t>(HIST, dict_in);
+        std::memmove(h_in + HIST - keep, h_in + HIST + n - keep, keep);
+        hist = keep;
+        pend = 0;
+        unsynced = true;
+        return true;
+    }
+
+    void sync_marker() {  // empty stored block, BFINAL = 0
+        put_bits(0, 3);
+        align();
+        put_byte(0x00);
+        put_byte(0x00);
+        put_byte(0xFF);
+        put_byte(0xFF);
+        unsynced = false;
+    }
+
+    bool finish() {
+        if (!piece(false)) return false;  // also with no pending bytes: the final (empty) block
+        align();
+        if (format == DEFLATE_FORMAT_ZLIB) {
+            for (int k = 3; k >= 0; --k) put_byte((uint8_t)(check >> (8 * k)));
+        } else if (format == DEFLATE_FORMAT_GZIP) {
+            for (int k = 0; k < 4; ++k) put_byte((uint8_t)(check >> (8 * k)));
+            for (int k = 0; k < 4; ++k) put_byte((uint8_t)(total_in >> (8 * k)));
+        }
+        ended = true;
+        return true;
+    }
+
+    void drain(uint8_t** next_out, size_t* avail_out) {
+        const size_t c = std::min(*avail_out, out_len - out_pos);
+        if (c) std::memcpy(*next_out, h_out + out_pos, c);
+        *next_out += c;
+        *avail_out -= c;
+        out_pos += c;
+        if (out_pos == out_len) out_pos = out_len = 0;
+    }
+};
+
+extern "C" {
+
+deflate_stream* deflate_stream_new(int level, deflate_format format) {
+    auto* s = new (std::nothrow) deflate_stream;
+    if (!s) return nullptr;
+    EngineConfig cfg;
+    cfg.level = level;
+    cfg.format = (int)format;
+    if (!s->eng.init(cfg)) {
+        delete s;
+        return nullptr;
+    }
+    s->format = s->eng.config().format;
+    const size_t chunk = s->eng.config().chunk;
+    s->piece_cap = std::max<size_t>(1, PIECE_TARGET / chunk) * chunk;
+    const size_t n = s->piece_cap, nch = n / chunk;
+    s->out_cap = std::max(deflate_compress_bound(n, DEFLATE_FORMAT_RAW) + 64, n + 6 * (nch + n / 65535 + 2) + 64);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->d_in), HIST + n + 64);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_out), s->out_cap);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_in), HIST + n, 0);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out), s->out_cap + 64, 0);
+    if (e != hipSuccess) {
+        set_error("deflate_stream_new: allocation", e);
+        delete s;
+        return nullptr;
+    }
+    s->check = s->format == DEFLATE_FORMAT_ZLIB ? 1u : 0u;
+    return s;
+}
+
+void deflate_stream_free(deflate_stream* s) { delete s; }
+
+deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_in, size_t* avail_in,
+                                       uint8_t** next_out, size_t* avail_out, deflate_flush flush) {
+    if (!s || !next_in || !avail_in || !next_out || !avail_out || (*avail_in && !*next_in) ||
+        (*avail_out && !*next_out) || flush < DEFLATE_NO_FLUSH || flush > DEFLATE_FINISH)
+        return DEFLATE_PARAM_ERROR;
+    if (s->failed) return DEFLATE_MEM_ERROR;
+    if (!s->header) {
+        if (s->format == DEFLATE_FORMAT_ZLIB) {
+            s->put_byte(0x78);
+            s->put_byte(0x9C);
+        } else if (s->format == DEFLATE_FORMAT_GZIP) {
+            const uint8_t g[10] = {0x1F, 0x8B, 8, 0, 0, 0, 0, 0, 0, 255};
+            for (uint8_t b : g) s->put_byte(b);
+        }
+        s->header = true;
+    }
+    for (;;) {
+        s->drain(next_out, avail_out);
+        if (s->out_len) return DEFLATE_NEED_OUTPUT;
+        if (s->ended) return DEFLATE_STREAM_END;
+        bool ok = true;
+        if (*avail_in) {
+            if (s->pend == s->piece_cap) {
+                ok = s->piece(true);  // full, and the stream goes on
+            } else {
+                const size_t c = std::min(*avail_in, s->piece_cap - s->pend);
+                std::memcpy(s->h_in + HIST + s->pend, *next_in, c);
+                s->pend += c;
+                *next_in += c;
+                *avail_in -= c;
+                continue;
+            }
+        } else if (flush == DEFLATE_FINISH) {
+            ok = s->finish();
+        } else if ((flush == DEFLATE_SYNC_FLUSH || flush == DEFLATE_FULL_FLUSH) && (s->pend || s->unsynced)) {
+            ok = !s->pend || s->piece(true);
+            if (ok) s->sync_marker();
+            if (ok && flush == DEFLATE_FULL_FLUSH) s->hist = s->dict_in = 0;  // no matches reach back past it
+        } else {
+            return DEFLATE_OK;
+        }
+        if (!ok) {
+            s->failed = true;
+            return DEFLATE_MEM_ERROR;
+        }
+    }
+}
+
+}  // extern "C"

@@ -167,6 +167,34 @@ def ref_lz77(data, level: int) -> np.ndarray:
     return t[:n]
 
 
+def ref_stream(data, level: int, fmt: int, one_call: bool = False) -> bytes:
+    """orot's own streaming compressor (src/api/stream_api.cpp:134-247): all
+    input in one DEFLATE_NO_FLUSH call, then DEFLATE_FINISH (or the input with
+    DEFLATE_FINISH in one call)."""
+    a = _u8(data)
+    L = ref()
+    L.deflate_stream_new.restype = _vp
+    L.deflate_stream_new.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.deflate_stream_free.restype = None
+    L.deflate_stream_free.argtypes = [_vp]
+    P = ctypes.POINTER
+    L.deflate_stream_compress.restype = ctypes.c_int
+    L.deflate_stream_compress.argtypes = [_vp, P(_vp), P(_sz), P(_vp), P(_sz), ctypes.c_int]
+    s = L.deflate_stream_new(level, fmt)
+    out = np.zeros(a.size + a.size // 8 + 4096, dtype=np.uint8)
+    ni, ai = _vp(a.ctypes.data), _sz(a.size)
+    no, ao = _vp(out.ctypes.data), _sz(out.size)
+    try:
+        for flush in ((3,) if one_call else (0, 3)):
+            r = L.deflate_stream_compress(s, ctypes.byref(ni), ctypes.byref(ai), ctypes.byref(no), ctypes.byref(ao),
+                                          flush)
+            if r < 0 or r == 3:
+                raise RuntimeError(f"orot deflate_stream_compress: {r}")
+    finally:
+        L.deflate_stream_free(s)
+    return out[:out.size - ao.value].tobytes()
+
+
 def ref_compress(data, level: int, fmt: int) -> bytes:
     """orot's own whole-buffer deflate_compress (src/api/deflate_api.cpp:100-136)."""
     a = _u8(data)

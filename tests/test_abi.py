@@ -67,6 +67,26 @@ def test_no_gpu_means_no_context():
     assert r.returncode == 0
 
 
+def test_stream_arguments_and_no_gpu():
+    """deflate_stream_compress rejects a NULL stream with DEFLATE_PARAM_ERROR,
+    and deflate_stream_new returns NULL where no GPU is usable: the stream
+    compresses on the GPU only.  Runs where torch sees no GPU."""
+    import subprocess
+    import sys
+
+    code = (
+        "import ctypes,orot_amd,sys,torch\n"
+        "L=orot_amd.lib()\n"
+        "if torch.cuda.is_available(): sys.exit(0)\n"
+        "p=ctypes.c_void_p(0); n=ctypes.c_size_t(0)\n"
+        "r=L.deflate_stream_compress(None,ctypes.byref(p),ctypes.byref(n),ctypes.byref(p),ctypes.byref(n),0)\n"
+        "s=L.deflate_stream_new(1,1)\n"
+        "sys.exit(0 if r==-5 and not s else 1)\n"
+    )
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env={**os.environ, "PYTHONPATH": ROOT})
+    assert r.returncode == 0
+
+
 def test_decompress_arguments_and_no_gpu():
     """deflate_decompress rejects bad arguments with DEFLATE_PARAM_ERROR before
     touching the GPU, and otherwise reports DEFLATE_MEM_ERROR: there is no CPU

@@ -284,6 +284,23 @@ def test_oracle_equals_orot_build(kind):
             assert orc.parallel_compress(data, level, fmt) == orc.ref_parallel_compress(data, level, fmt)
 
 
+# ---- orot's streaming compressor writes its parallel path's bytes wherever its
+# stream is valid: the whole input in one 128 KiB block that is flushed as the
+# final one, with no flush before (src/compress/compressor.cpp:21-97: a full
+# block is written as soon as it fills, without BFINAL, so 128 KiB fed before
+# DEFLATE_FINISH never gets a final block).  This is the definition the GPU
+# stream keeps.
+@pytest.mark.skipif(not orc.ref_available(), reason="orot not built in this checkout (oracle/_ref)")
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+def test_orot_stream_is_parallel_bytes(kind):
+    for n, one_call in ((1000, False), (4096, False), (65536, False), (131071, False), (131072, True)):
+        data = corpus.generate(kind, n, 33)
+        for level in (0, 1, 2, 3):
+            for fmt in (0, 1, 2):
+                got = orc.ref_stream(data, level, fmt, one_call)
+                assert got == orc.parallel_compress(data, level, fmt), (n, level, fmt)
+
+
 # ---- L6-class matcher against a brute-force restatement of its definition:
 # the chain of x is every earlier position with the same 12-bit hash (whole
 # 4-byte window in the buffer), nearest first; at most max_chain of them are
