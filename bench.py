@@ -338,6 +338,140 @@ def bench_text16(args) -> dict:
     }, golden, "orot parallel L1 gzip, 128 KiB chunks, the same 16 MiB text buffer", None)
 
 
+# Streaming workload: the default 256 MiB buffer through deflate_stream_compress
+# in 4 MiB writes, host memory to host memory, level 1, raw.
+STREAM_WRITE = 4 << 20
+OROT_STREAM_BLOCK = 131071  # largest input orot's stream writes as one valid final block
+
+
+def cpu_stream_baseline(data: np.ndarray, budget_s: float = 15.0) -> dict:
+    """orot's own streaming compressor on one core, over consecutive
+    131071-byte streams of the buffer (its streams are valid only up to one
+    128 KiB block), until budget_s has passed; the oracle's parallel path on
+    the same pieces (the same bytes) when orot is not built here."""
+    ref_x86 = os.path.join(ROOT, "oracle", "_ref", "liborot_ref_x86.so")
+    out = np.empty(OROT_STREAM_BLOCK * 2 + 4096, dtype=np.uint8)
+    P = ctypes.POINTER
+    if os.path.exists(ref_x86):
+        L = ctypes.CDLL(ref_x86)
+        L.deflate_stream_new.restype = ctypes.c_void_p
+        L.deflate_stream_new.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.deflate_stream_free.argtypes = [ctypes.c_void_p]
+        L.deflate_stream_compress.restype = ctypes.c_int
+        L.deflate_stream_compress.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p), P(ctypes.c_size_t),
+                                              P(ctypes.c_void_p), P(ctypes.c_size_t), ctypes.c_int]
+
+        def one(piece):
+            st = L.deflate_stream_new(1, 0)
+            ni, ai = ctypes.c_void_p(piece.ctypes.data), ctypes.c_size_t(piece.size)
+            no, ao = ctypes.c_void_p(out.ctypes.data), ctypes.c_size_t(out.size)
+            L.deflate_stream_compress(st, ctypes.byref(ni), ctypes.byref(ai), ctypes.byref(no), ctypes.byref(ao), 3)
+            L.deflate_stream_free(st)
+            return out.size - ao.value
+        kind = "reference"
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_ffi  # noqa: E402
+
+        O = oracle_ffi.oracle()
+
+        def one(piece):
+            return O.oracle_parallel_compress(1, 0, 1, 131072, piece.ctypes.data, piece.size, out.ctypes.data,
+                                              out.size)
+        kind = "port"
+    one(data[:OROT_STREAM_BLOCK])  # warm-up
+    nbytes = clen = 0
+    t0 = time.perf_counter()
+    for p0 in range(0, data.size - OROT_STREAM_BLOCK + 1, OROT_STREAM_BLOCK):
+        clen += one(data[p0:p0 + OROT_STREAM_BLOCK])
+        nbytes += OROT_STREAM_BLOCK
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(nbytes / dt / 1e9, 4),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"orot deflate_stream_compress L1 raw, one stream per {OROT_STREAM_BLOCK} bytes of the 4-segment "
+                  f"buffer, the first {nbytes >> 20} MiB ({clen} B out)",
+    }
+
+
+def bench_stream(args) -> dict:
+    """A step is one whole stream: a fresh deflate_stream over the 256 MiB
+    four-segment buffer, fed in 4 MiB writes from host memory, output into a
+    host buffer, DEFLATE_FINISH at the end."""
+    golden = None
+    gpath = os.path.join(ROOT, "tests", "golden", "bench_digest.json")
+    if os.path.exists(gpath):
+        with open(gpath) as f:
+            golden = json.load(f)
+    data = workload()
+    n = data.size
+    out = np.empty(orot_amd.compress_bound(n, 0) + (1 << 20), dtype=np.uint8)
+
+    def one_stream() -> int:
+        pos = 0
+        with orot_amd.Compressor(1, orot_amd.Format.RAW) as c:
+            for p0 in range(0, n, STREAM_WRITE):
+                src = data[p0:p0 + STREAM_WRITE]
+                used = 0
+                while used < src.size:
+                    r, u, w = c.compress_call(src[used:], out[pos:], orot_amd.Flush.NO_FLUSH)
+                    used += u
+                    pos += w
+                    if r < 0:
+                        raise SystemExit(f"stream failed: {orot_amd.last_error()}")
+            while True:
+                r, _, w = c.compress_call(b"", out[pos:], orot_amd.Flush.FINISH)
+                pos += w
+                if r == orot_amd.Result.STREAM_END:
+                    return pos
+                if r < 0:
+                    raise SystemExit(f"stream failed: {orot_amd.last_error()}")
+
+    clen = one_stream()
+    comp = out[:clen].tobytes()
+    inflated_ok = zlib.decompress(comp, -15) == data.tobytes()
+    digest = hashlib.sha256(comp).hexdigest()
+    matches_orot = golden is not None and golden.get("sha256") == digest and golden.get("size") == clen
+    del comp
+    for _ in range(args.warmup):
+        one_stream()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        one_stream()
+        lat.append(time.perf_counter() - s0)
+    total = time.perf_counter() - t0
+    lat_ms = [x * 1e3 for x in lat]
+    rec = base_record(args, n * args.steps / total / 1e9, total / args.steps * 1e3, {
+        "workload": "streaming compressor: the 256 MiB 4-segment buffer (zeros, Zipf text, random, 64-byte "
+                    "records) through deflate_stream_compress in 4 MiB writes, host memory to host memory, "
+                    "L1 greedy, raw DEFLATE, 32 MiB GPU pieces of 128 KiB chunks",
+        "level": 1,
+        "format": "raw",
+        "chunk": 131072,
+        "input_bytes": n,
+    })
+    rec.update({
+        "p50_ms": round(pct(lat_ms, 0.5), 3),
+        "p99_ms": round(pct(lat_ms, 0.99), 3),
+        "compressed_bytes": clen,
+        "ratio": round(clen / n, 5),
+        "inflate_ok": inflated_ok,
+        "matches_orot_bytes": matches_orot,
+        # the kernels are the default workload's; a stream is bound by the
+        # host copies and PCIe around them
+        "roofline": None,
+    })
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_stream_baseline(data)
+    return rec
+
+
 # Batch workload: 4096 buffers, sizes log-uniform in [4 KiB, 4 MiB], content
 # type uniform over the four generators, level 1, zlib.
 BATCH_COUNT = 4096
@@ -652,12 +786,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
-    ap.add_argument("--workload", choices=("l1", "text16", "l6", "batch", "serve"), default="l1",
+    ap.add_argument("--workload", choices=("l1", "text16", "l6", "batch", "serve", "stream"), default="l1",
                     help="l1: 256 MiB four-segment buffer at level 1 (default); "
                          "text16: 16 MiB Zipf text at level 1, gzip (orot's own CPU configuration); "
                          "l6: 256 x 8 MiB text/records buffers at level 6, gzip, batched; "
                          "batch: 4096 mixed buffers of 4 KiB - 4 MiB at level 1, zlib, batched; "
-                         "serve: Poisson request stream through the continuous-batching server")
+                         "serve: Poisson request stream through the continuous-batching server; "
+                         "stream: the l1 buffer through the streaming compressor, host to host, 4 MiB writes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale HBM counters")
@@ -669,7 +804,7 @@ def main() -> int:
               "batches shard across GPUs with no exchange (see the design notes).", file=sys.stderr)
         return 2
     rec = {"l1": bench_l1, "text16": bench_text16, "l6": bench_l6, "batch": bench_batch,
-           "serve": bench_serve}[args.workload](args)
+           "serve": bench_serve, "stream": bench_stream}[args.workload](args)
     try:
         rec["device"] = orot_amd.device_info()[0]
     except Exception:
