@@ -112,7 +112,11 @@ size_t deflate_parallel_compress(deflate_parallel_ctx* ctx, const void* in, size
 /* Streaming compressor (orot include/orot/deflate.h:92-108,
  * src/api/stream_api.cpp:134-247, src/compress/compressor.cpp:21-97).
  * Input is collected in pinned host memory and compressed on the GPU in
- * pieces of whole chunks (32 MiB), each piece's bits continuing the last.
+ * pieces of whole chunks (32 MiB), each piece's bits continuing the last;
+ * the next piece fills while the GPU compresses the previous one.
+ * deflate_stream_free keeps up to two freed streams (about 200 MiB of pinned
+ * and device memory each) for reuse by deflate_stream_new with the same
+ * level and format, until the process ends.
  * Fed with DEFLATE_NO_FLUSH up to DEFLATE_FINISH, the stream is byte for byte
  * what deflate_parallel_compress makes of the concatenated input with the
  * level's default block size; for 1000 bytes to 128 KiB of input that is

@@ -7,6 +7,9 @@
 // the pending bytes go to the device and compress as one buffer in stream
 // piece mode: raw bitstream, no framing, bits continuing at the stream's
 // current bit offset, and the last block final only at DEFLATE_FINISH.
+// Two input buffers alternate: the next piece fills while the GPU compresses
+// the last one, which is collected (its bit offset and checksum) before the
+// next launch.
 // Pieces are cut at chunk boundaries of the whole stream, so a stream fed
 // without flushes is, byte for byte, what deflate_parallel_compress makes of
 // the concatenated input (the L6 class gets the 32 KiB before each piece as
@@ -22,7 +25,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <vector>
 
 #include "../../include/orot/deflate.h"
 #include "deflate_gpu.h"
@@ -71,11 +76,15 @@ uint32_t adler32_concat(uint32_t aa, uint32_t ab, uint64_t nb) {
 
 struct deflate_stream {
     Engine eng;
+    int level_arg = 0, format_arg = 0;  // as passed to deflate_stream_new (the reuse key)
     int format = 0;
     size_t piece_cap = 0;       // bytes per piece, a multiple of the chunk
-    uint8_t* h_in = nullptr;    // pinned: HIST dictionary bytes, then the pending piece
+    uint8_t* h_buf[2] = {};     // pinned: HIST dictionary bytes, then a piece
+    uint8_t* h_in = nullptr;    // the buffer being filled
     size_t pend = 0;            // pending input bytes at h_in + HIST
     size_t hist = 0;            // stream bytes kept just before them (<= HIST)
+    bool inflight = false;      // a piece is on the GPU (from the other buffer)
+    size_t fly_n = 0;           // its input bytes
     uint8_t* d_in = nullptr;    // device copy of the same layout
     uint8_t* d_out = nullptr;
     size_t out_cap = 0;         // device output capacity of one piece
@@ -91,10 +100,28 @@ struct deflate_stream {
     bool failed = false;
 
     ~deflate_stream() {
+        if (inflight) {
+            uint64_t got = 0;
+            (void)eng.wait(&got);
+        }
         if (d_in) (void)hipFree(d_in);
         if (d_out) (void)hipFree(d_out);
-        if (h_in) (void)hipHostFree(h_in);
+        for (uint8_t* b : h_buf)
+            if (b) (void)hipHostFree(b);
         if (h_out) (void)hipHostFree(h_out);
+    }
+
+    // State of a new stream; the engine, buffers and arenas stay.
+    void reset() {
+        h_in = h_buf[0];
+        pend = hist = 0;
+        inflight = false;
+        fly_n = 0;
+        out_len = out_pos = 0;
+        held = nheld = 0;
+        total_in = dict_in = 0;
+        check = format == DEFLATE_FORMAT_ZLIB ? 1u : 0u;
+        header = unsynced = ended = failed = false;
     }
 
     void put_byte(uint8_t b) { h_out[out_len++] = b; }
@@ -111,9 +138,11 @@ struct deflate_stream {
         if (nheld) put_bits(0, 8 - nheld);
     }
 
-    // Compresses the pending bytes as one piece (final unless `open`) and
-    // queues its whole bytes; the staging buffer must be empty.
-    bool piece(bool open) {
+    // Starts the pending bytes as one piece (final unless `open`) on the GPU
+    // and switches filling to the other buffer, whose dictionary is the last
+    // HIST bytes since the start or the last full flush.  The previous piece
+    // must have been collected: this one continues at its bit offset.
+    bool launch(bool open) {
         const size_t n = pend;
         const hipStream_t st = eng.stream();
         hipError_t e = hipMemcpyAsync(d_in + HIST - hist, h_in + HIST - hist, hist + n, hipMemcpyHostToDevice, st);
@@ -124,15 +153,33 @@ struct deflate_stream {
         BufferSpec b{HIST, n, 0, out_cap};
         b.piece = orot_gpu::PIECE | (open ? orot_gpu::PIECE_OPEN : 0u) | (held << 8) | nheld;
         b.hist = (uint32_t)hist;
+        if (!eng.submit(d_in, d_out, &b, 1, false)) return false;
+        inflight = true;
+        fly_n = n;
+        dict_in += n;
+        const size_t keep = std::min<uint64_t>(HIST, dict_in);
+        uint8_t* next = h_in == h_buf[0] ? h_buf[1] : h_buf[0];
+        std::memcpy(next + HIST - keep, h_in + HIST + n - keep, keep);
+        h_in = next;
+        hist = keep;
+        pend = 0;
+        return true;
+    }
+
+    // Waits for the piece on the GPU and queues its whole bytes; the staging
+    // buffer must be empty.
+    bool collect() {
         uint64_t got = 0;
-        if (!eng.submit(d_in, d_out, &b, 1, false) || !eng.wait(&got)) return false;
+        inflight = false;
+        if (!eng.wait(&got)) return false;
         if (got == 0) {
             set_error_msg("deflate_stream_compress: piece output exceeds its bound");
             return false;
         }
         const orot_gpu::Result R = eng.results()[0];
         const uint64_t whole = R.payload_bits >> 3;
-        e = hipMemcpyAsync(h_out + out_len, d_out, got, hipMemcpyDeviceToHost, st);
+        const hipStream_t st = eng.stream();
+        hipError_t e = hipMemcpyAsync(h_out + out_len, d_out, got, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
             set_error("deflate_stream_compress: hipMemcpy(out)", e);
@@ -141,16 +188,9 @@ struct deflate_stream {
         nheld = (uint32_t)(R.payload_bits & 7);
         held = nheld ? h_out[out_len + whole] & ((1u << nheld) - 1u) : 0u;
         out_len += whole;
-        if (format == DEFLATE_FORMAT_ZLIB) check = adler32_concat(check, R.checksum, n);
-        if (format == DEFLATE_FORMAT_GZIP) check = crc32_concat(check, R.checksum, n);
-        total_in += n;
-        dict_in += n;
-        // the last HIST bytes since the start or the last full flush become
-        // the next piece's dictionary
-        const size_t keep = std::min<uint64_t>(HIST, dict_in);
-        std::memmove(h_in + HIST - keep, h_in + HIST + n - keep, keep);
-        hist = keep;
-        pend = 0;
+        if (format == DEFLATE_FORMAT_ZLIB) check = adler32_concat(check, R.checksum, fly_n);
+        if (format == DEFLATE_FORMAT_GZIP) check = crc32_concat(check, R.checksum, fly_n);
+        total_in += fly_n;
         unsynced = true;
         return true;
     }
@@ -165,8 +205,8 @@ struct deflate_stream {
         unsynced = false;
     }
 
-    bool finish() {
-        if (!piece(false)) return false;  // also with no pending bytes: the final (empty) block
+    bool finish() {  // no piece in flight
+        if (!launch(false) || !collect()) return false;  // also with no pending bytes: the final (empty) block
         align();
         if (format == DEFLATE_FORMAT_ZLIB) {
             for (int k = 3; k >= 0; --k) put_byte((uint8_t)(check >> (8 * k)));
@@ -188,11 +228,41 @@ struct deflate_stream {
     }
 };
 
+namespace {
+
+// Freed streams kept for reuse by the next deflate_stream_new with the same
+// level and format: a stream's pinned buffers are first touched on its first
+// pieces, which costs as much as compressing a few of them.  At most
+// STREAM_POOL are kept (about 100 MiB of pinned and 100 MiB of device memory
+// each); the pool lives until the process ends.
+constexpr size_t STREAM_POOL = 2;
+std::mutex g_pool_mu;
+std::vector<deflate_stream*>& pool() {
+    static auto* p = new std::vector<deflate_stream*>();
+    return *p;
+}
+
+}  // namespace
+
 extern "C" {
 
 deflate_stream* deflate_stream_new(int level, deflate_format format) {
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto& P = pool();
+        for (size_t i = 0; i < P.size(); ++i) {
+            if (P[i]->level_arg == level && P[i]->format_arg == (int)format) {
+                deflate_stream* s = P[i];
+                P.erase(P.begin() + (long)i);
+                s->reset();
+                return s;
+            }
+        }
+    }
     auto* s = new (std::nothrow) deflate_stream;
     if (!s) return nullptr;
+    s->level_arg = level;
+    s->format_arg = (int)format;
     EngineConfig cfg;
     cfg.level = level;
     cfg.format = (int)format;
@@ -207,18 +277,35 @@ deflate_stream* deflate_stream_new(int level, deflate_format format) {
     s->out_cap = std::max(deflate_compress_bound(n, DEFLATE_FORMAT_RAW) + 64, n + 6 * (nch + n / 65535 + 2) + 64);
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->d_in), HIST + n + 64);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->d_out), s->out_cap);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_in), HIST + n, 0);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_buf[0]), HIST + n, 0);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_buf[1]), HIST + n, 0);
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&s->h_out), s->out_cap + 64, 0);
     if (e != hipSuccess) {
         set_error("deflate_stream_new: allocation", e);
         delete s;
         return nullptr;
     }
+    s->h_in = s->h_buf[0];
     s->check = s->format == DEFLATE_FORMAT_ZLIB ? 1u : 0u;
     return s;
 }
 
-void deflate_stream_free(deflate_stream* s) { delete s; }
+void deflate_stream_free(deflate_stream* s) {
+    if (!s) return;
+    if (s->inflight) {
+        uint64_t got = 0;
+        s->inflight = false;
+        if (!s->eng.wait(&got)) s->failed = true;
+    }
+    if (!s->failed) {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        if (pool().size() < STREAM_POOL) {
+            pool().push_back(s);
+            return;
+        }
+    }
+    delete s;
+}
 
 deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_in, size_t* avail_in,
                                        uint8_t** next_out, size_t* avail_out, deflate_flush flush) {
@@ -243,7 +330,8 @@ deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_i
         bool ok = true;
         if (*avail_in) {
             if (s->pend == s->piece_cap) {
-                ok = s->piece(true);  // full, and the stream goes on
+                // full, and the stream goes on: the piece before it first
+                ok = s->inflight ? s->collect() : s->launch(true);
             } else {
                 const size_t c = std::min(*avail_in, s->piece_cap - s->pend);
                 std::memcpy(s->h_in + HIST + s->pend, *next_in, c);
@@ -252,10 +340,12 @@ deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_i
                 *avail_in -= c;
                 continue;
             }
+        } else if (flush != DEFLATE_NO_FLUSH && s->inflight) {
+            ok = s->collect();  // a flush or the end writes everything before it
         } else if (flush == DEFLATE_FINISH) {
             ok = s->finish();
         } else if ((flush == DEFLATE_SYNC_FLUSH || flush == DEFLATE_FULL_FLUSH) && (s->pend || s->unsynced)) {
-            ok = !s->pend || s->piece(true);
+            ok = !s->pend || (s->launch(true) && s->collect());
             if (ok) s->sync_marker();
             if (ok && flush == DEFLATE_FULL_FLUSH) s->hist = s->dict_in = 0;  // no matches reach back past it
         } else {
