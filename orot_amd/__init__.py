@@ -1,8 +1,9 @@
 """MI355X (gfx950) DEFLATE compressor with orot's parallel compress API.
 
 Python mirror of orot's C / C++ interface (orot include/orot/deflate.h):
-``compress_bound``, ``ParallelCompressor``, ``crc32``, ``adler32``, the
-``Format`` / ``Level`` constants and the ``Error`` exception, all served by
+``compress_bound``, ``ParallelCompressor``, the streaming ``Compressor``,
+``crc32``, ``adler32``, the ``Format`` / ``Level`` / ``Flush`` constants and
+the ``Error`` exception, all served by
 liborot_amd.so through its C ABI (include/orot/deflate.h,
 include/orot/deflate_gpu.h), plus ``BatchCompressor`` over the batch
 entry points of include/orot/deflate_batch.h.  The library must be built

@@ -21,7 +21,9 @@
 //                        stored/fixed choice (deflate_block.cpp:414-421) and
 //                        emission into a per-chunk staging bitstream.
 //   finalize_kernel   bit offset of every chunk (stored blocks depend on the
-//                     incoming offset mod 8), checksum combine, framing.
+//                     incoming offset mod 8), checksum combine, framing; a
+//                     stream piece starts at its stream's bit offset and has
+//                     no framing.
 //   assemble_kernel   one workgroup per chunk: shifts the chunk's bits to its
 //                     offset in the output (stored chunks copy the input).
 #include "deflate_gpu.h"
