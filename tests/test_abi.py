@@ -67,6 +67,51 @@ def test_no_gpu_means_no_context():
     assert r.returncode == 0
 
 
+def test_headers_compile_and_link(tmp_path):
+    """Every header compiles as C99, and with its C++ wrappers as C++20; a
+    program using orot::deflate::Compressor and ParallelCompressor links
+    against the library.  Without a GPU the constructors throw
+    orot::deflate::Error with DEFLATE_MEM_ERROR; with one, a stream round
+    trip runs."""
+    import shutil
+    import subprocess
+
+    if not shutil.which("gcc") or not shutil.which("g++"):
+        pytest.skip("no host compiler")
+    inc = os.path.join(ROOT, "include")
+    hdrs = sorted(h for h in os.listdir(os.path.join(inc, "orot")) if h.endswith(".h"))
+    c_src = tmp_path / "all.c"
+    c_src.write_text("".join(f"#include <orot/{h}>\n" for h in hdrs) + "int main(void) { return 0; }\n")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", f"-I{inc}", str(c_src)], check=True)
+    cpp = tmp_path / "use.cpp"
+    cpp.write_text("".join(f"#include <orot/{h}>\n" for h in hdrs) + r"""
+#include <cstdio>
+#include <vector>
+int main() {
+    try {
+        orot::deflate::Compressor c(1, orot::deflate::Format::Gzip);
+        std::vector<uint8_t> in(100000, 'a'), out(200000);
+        size_t n = c.feed(in, out);
+        n += c.finish(std::span<uint8_t>(out).subspan(n));
+        return n > 18 ? 0 : 1;
+    } catch (const orot::deflate::Error& e) {
+        if (e.code() != DEFLATE_MEM_ERROR) return 2;
+    }
+    try {
+        orot::deflate::ParallelCompressor pc(1);
+        return 0;
+    } catch (const orot::deflate::Error& e) {
+        return e.code() == DEFLATE_MEM_ERROR ? 0 : 3;
+    }
+}
+""")
+    exe = tmp_path / "use"
+    libdir = os.path.join(ROOT, "orot_amd")
+    subprocess.run(["g++", "-std=c++20", "-Wall", "-Werror", f"-I{inc}", str(cpp), "-o", str(exe), f"-L{libdir}",
+                    "-lorot_amd", "-L/opt/rocm/lib", f"-Wl,-rpath,{libdir}", "-Wl,-rpath,/opt/rocm/lib"], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
+
+
 def test_stream_arguments_and_no_gpu():
     """deflate_stream_compress rejects a NULL stream with DEFLATE_PARAM_ERROR,
     and deflate_stream_new returns NULL where no GPU is usable: the stream

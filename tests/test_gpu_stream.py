@@ -112,6 +112,21 @@ def test_stream_flushes(amd, level, fmt):
     assert inflate(out, fmt) == data.tobytes()
 
 
+@pytest.mark.parametrize("level", [1, 6, 9])
+def test_full_flush_is_a_restart_point(amd, level):
+    """After DEFLATE_FULL_FLUSH no match reaches back past the flush: the
+    bytes after its marker inflate on their own (RFC 1951 raw, no
+    dictionary); after a sync flush the L6 class may reach back."""
+    data = mixed(3_000_000, 77)
+    cut = 1_234_567
+    with amd.Compressor(level, 0) as c:
+        head = c.write(data[:cut], amd.Flush.FULL_FLUSH)
+        tail = c.write(data[cut:], amd.Flush.NO_FLUSH) + c.finish()
+    assert head.endswith(b"\x00\x00\xff\xff")
+    assert inflate(tail, 0) == data[cut:].tobytes()
+    assert inflate(head + tail, 0) == data.tobytes()
+
+
 def test_stream_flush_only(amd):
     """A sync flush before any input, flushes with nothing pending, and an
     empty stream finished after a flush."""
