@@ -113,7 +113,9 @@ size_t deflate_parallel_compress(deflate_parallel_ctx* ctx, const void* in, size
  * src/api/stream_api.cpp:134-247, src/compress/compressor.cpp:21-97).
  * Input is collected in pinned host memory and compressed on the GPU in
  * pieces of whole chunks (32 MiB), each piece's bits continuing the last;
- * the next piece fills while the GPU compresses the previous one.
+ * the next piece fills while the GPU compresses the previous one, and host
+ * copies of 1 MiB or more are split over three helper threads that the
+ * library starts on first use.
  * deflate_stream_free keeps up to two freed streams (about 200 MiB of pinned
  * and device memory each) for reuse by deflate_stream_new with the same
  * level and format, until the process ends.

@@ -24,9 +24,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/orot/deflate.h"
@@ -71,6 +73,73 @@ uint32_t adler32_concat(uint32_t aa, uint32_t ab, uint64_t nb) {
     const uint64_t s2 = (s2a + s2b + n * ((s1a + m - 1) % m)) % m;
     return (uint32_t)((s2 << 16) | s1);
 }
+
+// Host copies of at least COPY_SPLIT bytes (input into a pinned piece, output
+// out of the pinned staging) are split over the caller and COPY_THREADS
+// helper threads: one thread's memcpy does not reach the host's memory
+// bandwidth.  The helpers start on first use and live until the process
+// ends; one split copy runs at a time.
+constexpr size_t COPY_SPLIT = 1u << 20;
+constexpr int COPY_THREADS = 3;
+
+class CopyPool {
+public:
+    static CopyPool& get() {
+        static auto* p = new CopyPool();
+        return *p;
+    }
+
+    void copy(uint8_t* d, const uint8_t* s, size_t n) {
+        if (n < COPY_SPLIT) {
+            std::memcpy(d, s, n);
+            return;
+        }
+        std::lock_guard<std::mutex> one(call_mu_);
+        const size_t per = ((n + COPY_THREADS) / (COPY_THREADS + 1) + 63) & ~(size_t)63;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = d;
+            src_ = s;
+            n_ = n;
+            per_ = per;
+            pending_ = COPY_THREADS;
+            ++gen_;
+        }
+        cv_.notify_all();
+        std::memcpy(d, s, std::min(n, per));  // part 0 on the caller
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    CopyPool() {
+        for (int k = 1; k <= COPY_THREADS; ++k) std::thread([this, k] { run(k); }).detach();
+    }
+
+    void run(int k) {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return gen_ != seen; });
+            seen = gen_;
+            uint8_t* d = dst_;
+            const uint8_t* s = src_;
+            const size_t a = std::min(n_, (size_t)k * per_), b = std::min(n_, a + per_);
+            lk.unlock();
+            if (b > a) std::memcpy(d + a, s + a, b - a);
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    uint64_t gen_ = 0;
+    int pending_ = 0;
+    uint8_t* dst_ = nullptr;
+    const uint8_t* src_ = nullptr;
+    size_t n_ = 0, per_ = 0;
+};
 
 }  // namespace
 
@@ -220,7 +289,7 @@ struct deflate_stream {
 
     void drain(uint8_t** next_out, size_t* avail_out) {
         const size_t c = std::min(*avail_out, out_len - out_pos);
-        if (c) std::memcpy(*next_out, h_out + out_pos, c);
+        if (c) CopyPool::get().copy(*next_out, h_out + out_pos, c);
         *next_out += c;
         *avail_out -= c;
         out_pos += c;
@@ -334,7 +403,7 @@ deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_i
                 ok = s->inflight ? s->collect() : s->launch(true);
             } else {
                 const size_t c = std::min(*avail_in, s->piece_cap - s->pend);
-                std::memcpy(s->h_in + HIST + s->pend, *next_in, c);
+                CopyPool::get().copy(s->h_in + HIST + s->pend, *next_in, c);
                 s->pend += c;
                 *next_in += c;
                 *avail_in -= c;
