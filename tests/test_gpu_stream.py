@@ -57,7 +57,7 @@ def inflate(s: bytes, fmt: int) -> bytes:
 
 
 @pytest.mark.parametrize("fmt", [0, 1, 2])
-@pytest.mark.parametrize("level", [0, 1, 2, 3, 6, 9])
+@pytest.mark.parametrize("level", [0, 1, 2, 3, 4, 6, 9, 12])
 @pytest.mark.parametrize("n", [0, 1, 1000, 131072, 131073, 3_000_001])
 def test_stream_equals_parallel(amd, level, fmt, n):
     data = mixed(n, 40 + n % 97)
@@ -70,7 +70,7 @@ def test_stream_equals_parallel(amd, level, fmt, n):
         assert s == oracle_ffi.parallel_compress(data, level, fmt)
 
 
-@pytest.mark.parametrize("level,fmt", [(1, 2), (6, 1), (2, 0)])
+@pytest.mark.parametrize("level,fmt", [(1, 2), (6, 1), (2, 0), (12, 2)])
 def test_stream_over_several_pieces(amd, level, fmt):
     """Input of more than two 32 MiB pieces: the pieces' bit offsets, the
     dictionary carried across them (level 6) and the checksum combine."""
