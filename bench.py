@@ -800,8 +800,8 @@ def main() -> int:
     if args.calibrate_copy:
         return calibrate_copy()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box here holds a single MI355X; "
-              "batches shard across GPUs with no exchange (see the design notes).", file=sys.stderr)
+        print("bench.py measures one GPU only (--gpus 1). Across GPUs a batch would split into "
+              "independent sub-batches, one per GPU, with no exchange between them.", file=sys.stderr)
         return 2
     rec = {"l1": bench_l1, "text16": bench_text16, "l6": bench_l6, "batch": bench_batch,
            "serve": bench_serve, "stream": bench_stream}[args.workload](args)
