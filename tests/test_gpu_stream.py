@@ -70,6 +70,20 @@ def test_stream_equals_parallel(amd, level, fmt, n):
         assert s == oracle_ffi.parallel_compress(data, level, fmt)
 
 
+@pytest.mark.skipif(not oracle_ffi.ref_available(), reason="orot not built in this checkout (oracle/_ref)")
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+def test_stream_equals_orot_stream(amd, kind):
+    """Directly against orot's own streaming compressor, built from its
+    sources, wherever its stream is valid: up to one 128 KiB block, fed
+    before DEFLATE_FINISH (under 128 KiB) or with it (exactly 128 KiB)."""
+    for n, one_call in ((1000, False), (65536, False), (131071, False), (131072, True)):
+        data = corpus.generate(kind, n, 61)
+        for level in (0, 1, 2, 3):
+            for fmt in (0, 1, 2):
+                assert feed(amd, level, fmt, data, 3) == oracle_ffi.ref_stream(data, level, fmt, one_call), \
+                    (n, level, fmt)
+
+
 @pytest.mark.parametrize("level,fmt", [(1, 2), (6, 1), (2, 0), (12, 2)])
 def test_stream_over_several_pieces(amd, level, fmt):
     """Input of more than two 32 MiB pieces: the pieces' bit offsets, the
