@@ -19,6 +19,21 @@ extern "C" {
 size_t deflate_parallel_compress_device(deflate_parallel_ctx* ctx, const void* d_in, size_t in_size, void* d_out,
                                         size_t out_capacity);
 
+/* No counterpart in orot: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same output bytes as deflate_parallel_compress_device on a
+ * context of this level, format and block size (0: the level's default),
+ * computed without any context and without allocating: the working memory
+ * is a fixed 64 MiB device array inside the library, one per GPU, held
+ * until the process ends.  Inputs whose working set exceeds it (above
+ * 4 MiB at some levels; 8 MiB fits at levels 0-3) return 0 with the limit in
+ * orot_gpu_last_error().  Runs on the current device's null stream and
+ * blocks until the size is known.  Calls on one GPU share the array: this
+ * library serializes them within a process; calls that overlap on one GPU
+ * from several processes race.  Returns the compressed size, or 0. */
+size_t deflate_parallel_compress_device_noalloc(int level, deflate_format format, size_t block_size, const void* d_in,
+                                                size_t in_size, void* d_out, size_t out_capacity);
+
 /* Per-stage device time of the last call on ctx, in milliseconds, measured
  * with HIP events on the context's stream: [0] checksum, [1] LZ77 + block
  * emission, [2] offset scan + framing, [3] assembly.  Timing is recorded

@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "deflate_adler32",
     "deflate_crc32",
     "deflate_parallel_compress_device",
+    "deflate_parallel_compress_device_noalloc",
     "deflate_parallel_set_timing",
     "deflate_parallel_stage_times",
     "deflate_decompress_device",
@@ -169,6 +170,8 @@ def lib() -> ctypes.CDLL:
     L.deflate_parallel_compress.argtypes = [vp, vp, sz, vp, sz]
     L.deflate_parallel_compress_device.restype = sz
     L.deflate_parallel_compress_device.argtypes = [vp, vp, sz, vp, sz]
+    L.deflate_parallel_compress_device_noalloc.restype = sz
+    L.deflate_parallel_compress_device_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, sz, vp, sz, vp, sz]
     L.deflate_parallel_set_timing.restype = None
     L.deflate_parallel_set_timing.argtypes = [vp, ctypes.c_int]
     L.deflate_parallel_stage_times.restype = ctypes.c_int

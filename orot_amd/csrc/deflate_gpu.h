@@ -123,8 +123,10 @@ struct Plan {
     int cus;                // compute units
 };
 
-// One-time start-up check behind Plan::exch_ok.
+// One-time start-up check behind Plan::exch_ok (allocates one device word).
 hipError_t check_exchange_order(hipStream_t stream, bool* ok);
+// The same check on a caller-provided device word (no allocation).
+hipError_t check_exchange_order_at(hipStream_t stream, uint32_t* d_word, bool* ok);
 
 // Largest chunk the L1-class kernels accept for the given hash width.
 uint32_t max_chunk_for_hash_bits(int hash_bits);

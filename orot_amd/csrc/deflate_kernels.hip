@@ -1681,20 +1681,25 @@ hipError_t launch_l1_class(const Plan& p) {
 
 }  // namespace
 
+hipError_t check_exchange_order_at(hipStream_t stream, uint32_t* d_word, bool* ok) {
+    uint32_t h = 0;
+    hipError_t e = hipMemsetAsync(d_word, 0, sizeof(uint32_t), stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(exchange_order_kernel, dim3(1024), dim3(64), 0, stream, d_word);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, d_word, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    *ok = e == hipSuccess && h == 0;
+    return e;
+}
+
 hipError_t check_exchange_order(hipStream_t stream, bool* ok) {
     uint32_t* d = nullptr;
     hipError_t e = hipMalloc(&d, sizeof(uint32_t));
     if (e != hipSuccess) return e;
-    uint32_t h = 0;
-    e = hipMemsetAsync(d, 0, sizeof(uint32_t), stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(exchange_order_kernel, dim3(1024), dim3(64), 0, stream, d);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    e = check_exchange_order_at(stream, d, ok);
     (void)hipFree(d);
-    *ok = e == hipSuccess && h == 0;
     return e;
 }
 
