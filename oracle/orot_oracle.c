@@ -21,9 +21,19 @@
  *                            "a").
  *   limited Huffman lengths  brute-force optimum on tiny alphabets and
  *                            Kraft equality.
- *   L4..L9 block encoder     zlib inflate of every output (this encoder is the
- *                            HIP path's own design, not orot's; orot's sizes at
- *                            these levels are the ratio reference only).
+ *   lazy_parse (L4..L9)      the same golden token digests: orot's standard
+ *                            path runs through lazy_parse with its own match
+ *                            functions.
+ *   L6-class M(x)            brute force of its definition, with and without
+ *                            a dictionary, and the 32 KiB window edge.
+ *   L6-class tokens          lazy_parse over M(x), and a Python restatement of
+ *                            orot's lazy rule over the same M(x).
+ *   L4..L12 block encoder    forced stored / fixed / dynamic emissions: each
+ *                            inflates and the choice is the smallest; zlib
+ *                            and orot's inflater on every kind of output; the
+ *                            size within 3% of orot's at every level 4-12 (the
+ *                            encoder is the HIP path's own design, not orot's,
+ *                            so orot's sizes are the ratio reference only).
  */
 #include "orot_oracle.h"
 
@@ -311,43 +321,86 @@ static void insert16(const uint8_t* src, int pos, int n, uint32_t* head, uint32_
     head[h] = (uint32_t)pos;
 }
 
-static size_t lz77_std(const uint8_t* src, int n, const lz_cfg* c, uint32_t* head, uint32_t* prev, uint32_t* out) {
+/* The lazy parse of the standard path (lz77.cpp:435-496), over two match
+ * functions of the position alone (A.3: before the probe at p every q < p
+ * with q + 4 <= n has been inserted, in increasing order, whatever the parse
+ * did):
+ *   main(p)  = the best match at p with all q < p inserted;
+ *   probe(p) = the lazy probe at p + 1 with that same table (p itself not
+ *              yet inserted).
+ * A length below 3 means no match.  orot's standard path and the HIP L6-class
+ * matcher differ only in these two functions, so the orot token digests of
+ * levels 4-9 pin this loop for both. */
+typedef struct {
+    void* ctx;
+    int (*main)(void* ctx, int p, int* dist);
+    int (*probe)(void* ctx, int p, int* dist);
+} lz_matches;
+
+static size_t lazy_parse(const uint8_t* src, int n, int lazy_depth, const lz_matches* m, uint32_t* out) {
     size_t nt = 0;
     int pos = 0;
-    const int lazy_chain = (c->max_chain >> 1) > 1 ? (c->max_chain >> 1) : 1;
     while (pos < n) {
         if (pos + 3 > n) {
             out[nt++] = src[pos++];
             continue;
         }
         int bd = 0;
-        int bl = match_find(src, pos, n, head, prev, c->max_chain, c->nice_len, c->miss_limit, &bd);
+        int bl = m->main(m->ctx, pos, &bd);
         if (bl < 3) {
-            insert16(src, pos, n, head, prev);
             out[nt++] = src[pos++];
             continue;
         }
-        /* lazy probe at pos+1 sees the table before pos is inserted (lz77.cpp:460-480) */
-        int k = c->lazy_depth;
+        /* a literal is emitted while the probe at pos + 1 beats the match by
+         * more than one byte, at most lazy_depth times (lz77.cpp:460-480) */
+        int k = lazy_depth;
         while (k > 0 && pos + bl + 1 < n && pos + 1 + 3 <= n) {
             int nd = 0;
-            const int nl = match_find(src, pos + 1, n, head, prev, lazy_chain, c->nice_len, c->miss_limit, &nd);
-            if (nl > bl + 1) {
-                insert16(src, pos, n, head, prev);
-                out[nt++] = src[pos++];
-                bl = nl;
-                bd = nd;
-                --k;
-            } else {
-                break;
-            }
+            const int nl = m->probe(m->ctx, pos, &nd);
+            if (nl <= bl + 1) break;
+            out[nt++] = src[pos++];
+            bl = nl;
+            bd = nd;
+            --k;
         }
         out[nt++] = ORACLE_TOK_MATCH | ((uint32_t)bl << 16) | (uint32_t)bd;
-        const int end = pos + bl;
-        for (int i = pos; i < end && i + 3 <= n; ++i) insert16(src, i, n, head, prev);
-        pos = end;
+        pos += bl;
     }
     return nt;
+}
+
+/* orot's match functions: the hash chains are brought up to date (every
+ * q < p inserted) before each probe. */
+typedef struct {
+    const uint8_t* src;
+    int n;
+    const lz_cfg* c;
+    uint32_t* head;
+    uint32_t* prev;
+    int inserted; /* positions [0, inserted) are in the chains */
+} std_ctx;
+
+static void std_sync(std_ctx* s, int p) {
+    for (; s->inserted < p; ++s->inserted) insert16(s->src, s->inserted, s->n, s->head, s->prev);
+}
+
+static int std_main(void* ctx, int p, int* dist) {
+    std_ctx* s = (std_ctx*)ctx;
+    std_sync(s, p);
+    return match_find(s->src, p, s->n, s->head, s->prev, s->c->max_chain, s->c->nice_len, s->c->miss_limit, dist);
+}
+
+static int std_probe(void* ctx, int p, int* dist) {
+    std_ctx* s = (std_ctx*)ctx;
+    std_sync(s, p);
+    const int lazy_chain = (s->c->max_chain >> 1) > 1 ? (s->c->max_chain >> 1) : 1;
+    return match_find(s->src, p + 1, s->n, s->head, s->prev, lazy_chain, s->c->nice_len, s->c->miss_limit, dist);
+}
+
+static size_t lz77_std(const uint8_t* src, int n, const lz_cfg* c, uint32_t* head, uint32_t* prev, uint32_t* out) {
+    std_ctx s = {src, n, c, head, prev, 0};
+    const lz_matches m = {&s, std_main, std_probe};
+    return lazy_parse(src, n, c->lazy_depth, &m, out);
 }
 
 size_t oracle_lz77(const uint8_t* src, size_t n, int level, uint32_t* tokens) {
@@ -463,30 +516,39 @@ void oracle_l6_match(const uint8_t* src, size_t n, int level, uint32_t* mlen, ui
     oracle_l6_match_h(src, 0, n, level, mlen, mdist);
 }
 
+/* The L6-class match functions: M(x) from a table, the probe at p + 1 being
+ * M(p + 1). */
+typedef struct {
+    const uint32_t* ml;
+    const uint32_t* md;
+    int n;
+} l6_ctx;
+
+static int l6_main(void* ctx, int p, int* dist) {
+    const l6_ctx* s = (const l6_ctx*)ctx;
+    *dist = (int)s->md[p];
+    return (int)s->ml[p];
+}
+
+static int l6_probe(void* ctx, int p, int* dist) {
+    const l6_ctx* s = (const l6_ctx*)ctx;
+    if (p + 1 >= s->n) {
+        *dist = 0;
+        return 0;
+    }
+    *dist = (int)s->md[p + 1];
+    return (int)s->ml[p + 1];
+}
+
 size_t oracle_l6_lz77_h(const uint8_t* src, size_t halo, size_t n, int level, uint32_t* tokens) {
     const l6_cfg* c = &L6LEVELS[clamp_level(level)];
     if (n == 0) return 0;
-    const uint8_t* s = src + halo;
     uint32_t* ml = (uint32_t*)calloc(2 * n, sizeof(uint32_t));
     uint32_t* md = ml + n;
     oracle_l6_match_h(src, halo, n, level, ml, md);
-    size_t nt = 0, pos = 0;
-    while (pos < n) {
-        if (pos + 3 > n || ml[pos] < 3) {
-            tokens[nt++] = s[pos++];
-            continue;
-        }
-        uint32_t b = ml[pos], d = md[pos];
-        int k = c->lazy_depth;
-        while (k > 0 && pos + b + 1 < n && pos + 1 + 3 <= n && ml[pos + 1] > b + 1) {
-            tokens[nt++] = s[pos++];
-            b = ml[pos];
-            d = md[pos];
-            --k;
-        }
-        tokens[nt++] = ORACLE_TOK_MATCH | (b << 16) | d;
-        pos += b;
-    }
+    l6_ctx s = {ml, md, (int)n};
+    const lz_matches m = {&s, l6_main, l6_probe};
+    const size_t nt = lazy_parse(src + halo, (int)n, c->lazy_depth, &m, tokens);
     free(ml);
     return nt;
 }
@@ -782,6 +844,29 @@ static void emit_dynamic(bitbuf* b, const uint32_t* t, size_t nt, const uint8_t*
     canon_codes(ll, 288, llc);
     canon_codes(dl, 32, dlc);
     emit_tokens(b, t, nt, ll, llc, dl, dlc);
+}
+
+void oracle_encode_block_forced(const uint32_t* tokens, size_t ntok, const uint8_t* raw, int is_last, int btype,
+                                uint8_t* dst, uint64_t* bitpos) {
+    bitbuf b = {dst, *bitpos};
+    blk_stats s;
+    block_stats(tokens, ntok, &s);
+    if (btype == 0) {
+        emit_stored(&b, raw, (size_t)s.raw, is_last);
+    } else if (btype == 1) {
+        uint8_t fl[288], fd[32];
+        uint16_t flc[288], fdc[32];
+        fixed_tables(fl, flc, fd, fdc);
+        put_bits(&b, (uint32_t)(is_last ? 1 : 0), 1);
+        put_bits(&b, 1, 2);
+        emit_tokens(&b, tokens, ntok, fl, flc, fd, fdc);
+    } else {
+        uint8_t ll[288], dl[32];
+        oracle_limited_lengths(s.lit, 288, 15, ll);
+        oracle_limited_lengths(s.dist, 32, 15, dl);
+        emit_dynamic(&b, tokens, ntok, ll, dl, is_last);
+    }
+    *bitpos = b.pos;
 }
 
 void oracle_encode_blocks_dyn(const uint32_t* tokens, size_t ntok, const uint8_t* raw, size_t raw_len, int is_last,

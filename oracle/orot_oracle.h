@@ -87,6 +87,13 @@ int oracle_encode_block_fixed(const uint32_t* tokens, size_t ntok, const uint8_t
 void oracle_encode_blocks_dyn(const uint32_t* tokens, size_t ntok, const uint8_t* raw, size_t raw_len,
                               int is_last, int max_syms, uint8_t* dst, uint64_t* bitpos);
 
+/* One block of the given type (0 stored, 1 fixed, 2 dynamic with the
+ * package-merge lengths oracle_encode_blocks_dyn uses), whatever the choice
+ * rule would pick: the emitted sizes the choice is checked against.  Same
+ * bit-buffer convention as above. */
+void oracle_encode_block_forced(const uint32_t* tokens, size_t ntok, const uint8_t* raw, int is_last, int btype,
+                                uint8_t* dst, uint64_t* bitpos);
+
 /* Optimal length-limited Huffman code lengths (package-merge) with the
  * deterministic tie order (freq, symbol).  One active symbol gets length 1.
  * Returns the number of active symbols. */

@@ -418,6 +418,9 @@ deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_i
             if (ok) s->sync_marker();
             if (ok && flush == DEFLATE_FULL_FLUSH) s->hist = s->dict_in = 0;  // no matches reach back past it
         } else {
+            // a full flush with nothing to write (right after a sync flush)
+            // still ends the dictionary: the marker is already out
+            if (flush == DEFLATE_FULL_FLUSH) s->hist = s->dict_in = 0;
             return DEFLATE_OK;
         }
         if (!ok) {

@@ -82,13 +82,13 @@ def main() -> int:
                 t = orc.ref_lz77(data, level)
                 out["tokens"].append({"kind": kind, "n": n, "seed": seed, "level": level, "count": int(t.size),
                                       "sha256": sha(t.astype("<u4").tobytes())})
-    # 4. orot's sizes at levels 4-9 (ratio reference of the L6-class path) and
+    # 4. orot's sizes at levels 4-12 (ratio reference of the L6-class path) and
     #    whether zlib accepts orot's own output there
     for kind in corpus.GENERATORS:
         for n in (300_000, 4_000_000):
             seed = 555 + n
             data = corpus.generate(kind, n, seed)
-            for level in (4, 6, 9):
+            for level in range(4, 13):
                 comp = orc.ref_parallel_compress(data, level, 0, 0, threads=8)
                 try:
                     ok = zlib.decompress(comp, -15) == data.tobytes()

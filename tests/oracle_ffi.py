@@ -45,6 +45,14 @@ def oracle() -> ctypes.CDLL:
         L.oracle_l6_lz77.argtypes = [_vp, _sz, ctypes.c_int, _vp]
         L.oracle_l6_match.restype = None
         L.oracle_l6_match.argtypes = [_vp, _sz, ctypes.c_int, _vp, _vp]
+        L.oracle_l6_match_h.restype = None
+        L.oracle_l6_match_h.argtypes = [_vp, _sz, _sz, ctypes.c_int, _vp, _vp]
+        L.oracle_l6_lz77_h.restype = _sz
+        L.oracle_l6_lz77_h.argtypes = [_vp, _sz, _sz, ctypes.c_int, _vp]
+        L.oracle_encode_blocks_dyn.restype = None
+        L.oracle_encode_blocks_dyn.argtypes = [_vp, _sz, _vp, _sz, ctypes.c_int, ctypes.c_int, _vp, _vp]
+        L.oracle_encode_block_forced.restype = None
+        L.oracle_encode_block_forced.argtypes = [_vp, _sz, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
         L.oracle_encode_block_fixed.restype = ctypes.c_int
         L.oracle_encode_block_fixed.argtypes = [_vp, _sz, _vp, _sz, ctypes.c_int, _vp, _vp]
         L.oracle_limited_lengths.restype = ctypes.c_int
@@ -142,6 +150,50 @@ def l6_match(data, level: int):
     return ml[:a.size], md[:a.size]
 
 
+def l6_match_h(buf, halo: int, level: int):
+    """M(x) for the len(buf) - halo positions after a dictionary of `halo`
+    bytes at the start of buf."""
+    a = _u8(buf)
+    n = a.size - halo
+    ml = np.zeros(max(1, n), dtype=np.uint32)
+    md = np.zeros(max(1, n), dtype=np.uint32)
+    oracle().oracle_l6_match_h(a.ctypes.data, halo, n, level, ml.ctypes.data, md.ctypes.data)
+    return ml[:n], md[:n]
+
+
+def l6_lz77_h(buf, halo: int, level: int) -> np.ndarray:
+    a = _u8(buf)
+    n = a.size - halo
+    t = np.zeros(max(1, n), dtype=np.uint32)
+    k = oracle().oracle_l6_lz77_h(a.ctypes.data, halo, n, level, t.ctypes.data)
+    return t[:k]
+
+
+def encode_blocks_dyn(tokens, raw, is_last: int = 1, max_syms: int = 16384):
+    """The L6-class block encoder from a byte-aligned start: (bytes, bits)."""
+    t = np.ascontiguousarray(tokens, dtype=np.uint32)
+    r = _u8(raw)
+    dst = np.zeros(r.size + 8 * t.size // 7 + 4096, dtype=np.uint8)
+    bp = np.zeros(1, dtype=np.uint64)
+    oracle().oracle_encode_blocks_dyn(t.ctypes.data, t.size, r.ctypes.data, r.size, is_last, max_syms,
+                                      dst.ctypes.data, bp.ctypes.data)
+    bits = int(bp[0])
+    return dst[:(bits + 7) // 8].tobytes(), bits
+
+
+def encode_block_forced(tokens, raw, btype: int, is_last: int = 1):
+    """One block of type btype (0 stored, 1 fixed, 2 dynamic) from a
+    byte-aligned start: (bytes, bits)."""
+    t = np.ascontiguousarray(tokens, dtype=np.uint32)
+    r = _u8(raw)
+    dst = np.zeros(r.size + 8 * t.size // 7 + 4096, dtype=np.uint8)
+    bp = np.zeros(1, dtype=np.uint64)
+    oracle().oracle_encode_block_forced(t.ctypes.data, t.size, r.ctypes.data, is_last, btype, dst.ctypes.data,
+                                        bp.ctypes.data)
+    bits = int(bp[0])
+    return dst[:(bits + 7) // 8].tobytes(), bits
+
+
 def limited_lengths(freq, max_bits: int) -> np.ndarray:
     f = np.ascontiguousarray(freq, dtype=np.uint32)
     lens = np.zeros(f.size, dtype=np.uint8)
@@ -165,6 +217,44 @@ def ref_lz77(data, level: int) -> np.ndarray:
     t = np.zeros(max(1, a.size), dtype=np.uint32)
     n = ref().ref_lz77(a.ctypes.data, a.size, level, t.ctypes.data)
     return t[:n]
+
+
+def ref_decompress(comp: bytes, fmt: int, cap: int):
+    """orot's own deflate_decompress (src/api/deflate_api.cpp:138-215 with
+    its built-in inflater, src/decompress/decompressor.cpp): (result, bytes)."""
+    L = ref()
+    L.deflate_decompress.restype = ctypes.c_int
+    L.deflate_decompress.argtypes = [_vp, _sz, _vp, _sz, ctypes.POINTER(_sz), ctypes.c_int]
+    src = np.frombuffer(comp, dtype=np.uint8) if comp else np.zeros(1, dtype=np.uint8)
+    out = np.zeros(max(1, cap), dtype=np.uint8)
+    got = _sz(0)
+    r = L.deflate_decompress(src.ctypes.data, len(comp), out.ctypes.data, cap, ctypes.byref(got), fmt)
+    return r, out[:got.value].tobytes()
+
+
+def _ref_decompress_child(conn, comp, fmt, cap):
+    conn.send(ref_decompress(comp, fmt, cap))
+    conn.close()
+
+
+def ref_decompress_guarded(comp: bytes, fmt: int, cap: int, timeout: float = 60.0):
+    """ref_decompress in a child process: orot's inflater can loop or crash
+    on valid streams that hold a stored block after a Huffman block, so a
+    test must not call it in its own process.  Returns (result, bytes), or
+    None when the child did not answer within `timeout` seconds."""
+    import multiprocessing as mp
+
+    ctx = mp.get_context("fork")
+    a, b = ctx.Pipe(duplex=False)
+    p = ctx.Process(target=_ref_decompress_child, args=(b, comp, fmt, cap))
+    p.start()
+    b.close()
+    got = a.recv() if a.poll(timeout) else None
+    p.join(1.0)
+    if p.is_alive():
+        p.kill()
+        p.join()
+    return got
 
 
 def ref_stream(data, level: int, fmt: int, one_call: bool = False) -> bytes:

@@ -141,6 +141,21 @@ def test_full_flush_is_a_restart_point(amd, level):
     assert inflate(head + tail, 0) == data.tobytes()
 
 
+@pytest.mark.parametrize("level", [6, 9])
+def test_full_flush_after_sync_flush_is_a_restart_point(amd, level):
+    """A full flush straight after a sync flush (no input between them)
+    writes no second marker but still ends the dictionary: the bytes written
+    after it inflate on their own."""
+    data = mixed(2_000_000, 78)
+    cut = 900_001
+    with amd.Compressor(level, 0) as c:
+        head = c.write(data[:cut], amd.Flush.SYNC_FLUSH) + c.flush(True)
+        tail = c.write(data[cut:], amd.Flush.NO_FLUSH) + c.finish()
+    assert head.endswith(b"\x00\x00\xff\xff")
+    assert inflate(tail, 0) == data[cut:].tobytes()
+    assert inflate(head + tail, 0) == data.tobytes()
+
+
 def test_stream_flush_only(amd):
     """A sync flush before any input, flushes with nothing pending, and an
     empty stream finished after a flush."""
