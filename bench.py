@@ -1,17 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X parallel DEFLATE compressor.
 
-Default workload (one GPU): the plain GPU path at level 1, raw DEFLATE, on one
-256 MiB buffer made of four 64 MiB segments -- all-zero bytes, Zipf-word
-text, uniform random bytes and 64-byte records -- in orot's default 128 KiB
-chunks.  A step is one deflate_parallel_compress_device call over the whole
-buffer, resident in HBM.  The output is checked once (untimed) to inflate
-back to the input through zlib, and its size is compared with orot's own
-output (which it reproduces byte for byte).
+Default workload (one GPU): the batched path at level 1, zlib framing, on 4096
+independent buffers with sizes log-uniform in [4 KiB, 4 MiB] (2.53 GB in
+total) and content drawn uniformly from four generators -- all-zero bytes,
+Zipf-word text, uniform random bytes and 64-byte records.  A step is one
+graph-submitted batch of all 4096 buffers, resident in HBM, in the
+configuration the autotuner keeps.  Every output is checked once (untimed) to
+inflate back to its input through zlib, to equal that buffer compressed alone,
+and its size is compared with orot's own output for the same buffer.  The
+plain path (one deflate_parallel_compress_device call per buffer) is timed on
+the same buffers, and so is the host-buffer path (deflate_batch_compress,
+PCIe copies included).
 
 Prints one JSON line with the metric, per-step latency percentiles, the
-roofline of the dominant kernel (LZ77 + block emission) and orot's CPU path
-timed on this host.
+roofline of the dominant kernels (the L1 LZ stage), and orot's parallel CPU
+path timed on this host on orot's own configuration (16 MiB Zipf text, L1,
+gzip, 128 KiB chunks) with all host threads and with one.
 """
 from __future__ import annotations
 
@@ -39,6 +44,7 @@ HBM_PEAK = 8.0e12
 # rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the dominant kernel on the
 # default workload, scaled by the same counters on a plain copy kernel.
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "l1_lz_stage_256MiB_pmc.txt")
+BATCH_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "l1_lz_stage_batch_4096_pmc.txt")
 # the L1-class LZ stage at 12-bit hashes and chunks up to 128 KiB: candidate
 # scan, then match / parse / emission
 L1_STAGE = "l1_candidate_kernel + l1_barrier_kernel"
@@ -153,11 +159,25 @@ class OrotCPU:
             self.ctx = None
 
 
+def cpu_model() -> str:
+    """The host CPU's model name as lscpu reports it."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except Exception:
+        return ""
+    for line in out.splitlines():
+        if line.startswith("Model name:"):
+            return line.split(":", 1)[1].strip()
+    return ""
+
+
 def cpu_baseline(data: np.ndarray, level: int = 1, fmt: int = 0, chunk: int = 131072, sample: str = "",
-                 budget_s: float = 15.0):
+                 budget_s: float = 15.0, threads: int = 0):
     """orot's parallel compressor on the host cores: median of up to 10 runs
     over the whole buffer after one warm-up."""
-    threads = cpu_threads()
+    threads = threads or cpu_threads()
     cpu = OrotCPU(level, fmt, chunk, threads)
     out = np.empty(data.size + data.size // 8 + 4096, dtype=np.uint8)
     cpu.compress(data, out)  # warm-up
@@ -176,6 +196,20 @@ def cpu_baseline(data: np.ndarray, level: int = 1, fmt: int = 0, chunk: int = 13
         "kind": cpu.kind,
         "sample": f"{sample}, median of {len(times)} runs after 1 warm-up ({clen} B out)",
     }
+
+
+def cpu_config1() -> dict:
+    """orot's parallel L1 on its own CPU configuration: one 16 MiB Zipf-word
+    text buffer, gzip, 128 KiB chunks (128 chunks), with every host thread
+    this process may use and with one thread."""
+    data = corpus.generate("text", 16 << 20, 0xC0F1)
+    sample = "orot parallel L1 gzip, 128 KiB chunks, one 16 MiB Zipf-word text buffer"
+    rec = cpu_baseline(data, 1, 2, 131072, sample)
+    one = cpu_baseline(data, 1, 2, 131072, sample, threads=1)
+    rec["one_thread"] = {"value": one["value"], "unit": "GB/s", "cores": 1, "sample": one["sample"]}
+    rec["cpu_model"] = cpu_model()
+    rec["host_cpus"] = os.cpu_count()
+    return rec
 
 
 def cpu_multi(bufs, level: int, fmt: int, budget_s: float = 20.0):
@@ -487,7 +521,47 @@ def batch_specs():
             for i, (k, n) in enumerate(zip(kinds, sizes))]
 
 
-def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds) -> dict:
+def host_batch_seconds(bufs, level: int, fmt: int, reps: int = 2) -> float:
+    """Best of `reps` deflate_batch_compress calls over the host buffers
+    (H2D copies, the batch, D2H copies of the outputs)."""
+    n = len(bufs)
+    outs = [np.empty(orot_amd.compress_bound(b.size, 2) + 64 * (b.size // 65535 + 4), dtype=np.uint8) for b in bufs]
+    in_p = (ctypes.c_void_p * n)(*[b.ctypes.data for b in bufs])
+    out_p = (ctypes.c_void_p * n)(*[o.ctypes.data for o in outs])
+    in_s = (ctypes.c_size_t * n)(*[b.size for b in bufs])
+    out_c = (ctypes.c_size_t * n)(*[o.size for o in outs])
+    got = (ctypes.c_size_t * n)()
+    best = None
+    with orot_amd.BatchCompressor(level, fmt, 0, 0) as hb:
+        for _ in range(reps + 1):  # the first call grows the arenas
+            t0 = time.perf_counter()
+            if orot_amd.lib().deflate_batch_compress(hb._ctx, in_p, in_s, out_p, out_c, got, n) != 0:
+                raise SystemExit(f"deflate_batch_compress: {orot_amd.last_error()}")
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+    return best
+
+
+def batch_only(args, level: int, fmt: int, d_in, d_out, items, n: int) -> dict:
+    """Profiling run: only the graph-submitted batch in the default
+    configuration (no plain path, autotuner or CPU leg), warm-up then steps."""
+    with orot_amd.BatchCompressor(level, fmt, 0, orot_amd.BATCH_GRAPH) as bc:
+        for _ in range(args.warmup):
+            bc.compress_device(d_in.ptr, d_out.ptr, items)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out_sizes = bc.compress_device(d_in.ptr, d_out.ptr, items)
+        total = time.perf_counter() - t0
+    rec = base_record(args, n * args.steps / total / 1e9, total / args.steps * 1e3,
+                      {"workload": "batch only (profiling)", "level": level, "buffers": len(items),
+                       "input_bytes": n})
+    rec["compressed_bytes"] = int(np.sum(out_sizes))
+    d_in.free()
+    d_out.free()
+    return rec
+
+
+def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds, traffic_summary=None) -> dict:
     """Many buffers: the plain path (one deflate_parallel_compress_device call
     per buffer, orot's default chunks) and the batch path (one submission of
     every buffer: graph replay, autotuned configuration).  A step is one
@@ -510,6 +584,9 @@ def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds) -> 
 
     def digests(out_sizes):
         return [hashlib.sha256(d_out.download(int(k), int(o)).tobytes()).digest() for k, o in zip(out_sizes, out_off)]
+
+    if args.batch_only:
+        return batch_only(args, level, fmt, d_in, d_out, items, n)
 
     # plain path: one call per buffer, orot's default configuration
     pc = orot_amd.ParallelCompressor(level, fmt, 0, 0)
@@ -564,6 +641,9 @@ def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds) -> 
     clen = int(np.sum(out_sizes))
     plain_clen = int(sum(plain_sizes))
 
+    # host buffers in and out: deflate_batch_compress, PCIe copies included
+    host_t = host_batch_seconds(bufs, level, fmt, reps=2)
+
     # stage times of the tuned configuration (direct launches, HIP events)
     with orot_amd.BatchCompressor(level, fmt, 0, 0) as bt:
         bt.set_config(tuned["block_size"], tuned["hash_bits"], tuned["max_chain"], tuned["nice_len"],
@@ -598,6 +678,7 @@ def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds) -> 
         "alone_equals_in_mix": mix_equal,
         "size_vs_default_config": round(clen / plain_clen, 5),
         "autotune_s": round(t_tune, 3),
+        "host_path_gbps": round(n / host_t / 1e9, 3),
         "stage_ms_tuned": [round(x, 4) for x in stages],
         "roofline": {
             "bound": "hbm",
@@ -606,12 +687,13 @@ def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds) -> 
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(b_alg / front / HBM_PEAK, 5),
-            "traffic": None,
+            "traffic": measured_traffic(traffic_summary) if traffic_summary else None,
         },
     })
     if not args.no_cpu:
         base, orot_sizes = cpu_multi(bufs, level, fmt)
-        rec["cpu_baseline"] = base
+        rec["cpu_baseline"] = cpu_config1() if level <= 3 else base
+        rec["cpu_batch"] = base
         m = len(orot_sizes)
         by_kind = {}
         for i in range(m):
@@ -640,10 +722,19 @@ def bench_l6(args) -> dict:
 def bench_batch(args) -> dict:
     """Level 1, zlib: 4096 buffers of 4 KiB - 4 MiB, mixed content."""
     specs = batch_specs()
-    return bench_multi(args, specs, 1, 1,
-                       "batched L1-class greedy, zlib, 4096 buffers log-uniform in [4 KiB, 4 MiB], content "
-                       "uniform over zeros / Zipf text / random / records, one graph-submitted batch, autotuned",
-                       lambda i: specs[i][0])
+    rec = bench_multi(args, specs, 1, 1,
+                      "batched L1-class greedy, zlib, 4096 buffers log-uniform in [4 KiB, 4 MiB], content "
+                      "uniform over zeros / Zipf text / random / records, one graph-submitted batch, autotuned",
+                      lambda i: specs[i][0], BATCH_TRAFFIC_SUMMARY)
+    if not args.batch_only and not args.no_plain256:
+        # the plain-path line of the 256 MiB four-segment buffer (L1, raw), no CPU leg
+        sub = argparse.Namespace(**vars(args))
+        sub.no_cpu = True
+        sub.steps = min(args.steps, 10)
+        p = bench_l1(sub)
+        rec["plain_256MiB"] = {k: p[k] for k in ("value", "ms_per_step", "p50_ms", "p99_ms", "compressed_bytes",
+                                                 "inflate_ok", "matches_orot_bytes", "roofline")}
+    return rec
 
 
 # Serving workload: Poisson arrivals, sizes log-uniform in [1 KiB, 8 MiB],
@@ -786,14 +877,18 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
-    ap.add_argument("--workload", choices=("l1", "text16", "l6", "batch", "serve", "stream"), default="l1",
-                    help="l1: 256 MiB four-segment buffer at level 1 (default); "
+    ap.add_argument("--workload", choices=("l1", "text16", "l6", "batch", "serve", "stream"), default="batch",
+                    help="batch: 4096 mixed buffers of 4 KiB - 4 MiB at level 1, zlib, batched (default); "
+                         "l1: 256 MiB four-segment buffer at level 1, raw, one plain call; "
                          "text16: 16 MiB Zipf text at level 1, gzip (orot's own CPU configuration); "
                          "l6: 256 x 8 MiB text/records buffers at level 6, gzip, batched; "
-                         "batch: 4096 mixed buffers of 4 KiB - 4 MiB at level 1, zlib, batched; "
                          "serve: Poisson request stream through the continuous-batching server; "
                          "stream: the l1 buffer through the streaming compressor, host to host, 4 MiB writes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch / l6: time only the graph-submitted batch (for profilers)")
+    ap.add_argument("--no-plain256", action="store_true",
+                    help="batch: skip the extra plain-path line on the 256 MiB four-segment buffer")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run the copy kernel used to scale HBM counters")
     args = ap.parse_args()
