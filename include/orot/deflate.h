@@ -4,10 +4,9 @@
  *
  * The types, values and entry points below keep the meaning of orot's
  * include/orot/deflate.h; each declaration cites the orot line it replaces.
- * Entry points of orot that are not part of the GPU path (the streaming
- * decompressor, deflate_set_allocator) are not exported by this library.
- * deflate_compress is served by the chunked GPU pipeline (its stream differs
- * from orot's single-block one); deflate_decompress by a GPU inflate.
+ * Compression runs on the GPU.  The decompress side of orot's host-buffer
+ * API (deflate_decompress, the streaming decompressor) is served by a host
+ * inflater, since it is not the GPU compress path.
  *
  * Deviations from orot, all deliberate:
  *   - gzip trailers carry the IEEE CRC-32 on every host (orot's x86 build
@@ -61,12 +60,26 @@ typedef enum deflate_flush {
     DEFLATE_FINISH = 3      /* final block and trailer */
 } deflate_flush;
 
+/* Custom allocator (orot include/orot/deflate.h:49-54). */
+typedef struct deflate_allocator {
+    void* (*alloc)(void* opaque, size_t size, size_t alignment);
+    void (*free)(void* opaque, void* ptr);
+    void* opaque;
+} deflate_allocator;
+
 /* Levels (orot include/orot/deflate.h:35-39); values are clamped to 0..12. */
 #define DEFLATE_LEVEL_STORE 0
 #define DEFLATE_LEVEL_FAST 1
 #define DEFLATE_LEVEL_DEFAULT 6
 #define DEFLATE_LEVEL_BETTER 9
 #define DEFLATE_LEVEL_MAX 12
+
+/* orot include/orot/deflate.h:61, src/api/deflate_api.cpp:75-77: a non-NULL
+ * allocator (with both functions set) replaces the current one; NULL is
+ * ignored.  Set it before first use.  It serves the host memory of the
+ * decompress side (inflate_stream_new, deflate_decompress); device memory and
+ * pinned staging come from the HIP runtime. */
+void deflate_set_allocator(const deflate_allocator* alloc);
 
 /* Upper bound of the compressed size (orot include/orot/deflate.h:66,
  * src/api/deflate_api.cpp:81-98): n + n/4096 + n/16384 + n/2^25 + 13, plus 6
@@ -89,9 +102,8 @@ size_t deflate_compress(const void* in, size_t in_size, void* out, size_t out_ca
  * *actual_out_size set; DEFLATE_DATA_ERROR for a corrupt or truncated stream,
  * a bad header or a checksum / ISIZE mismatch; DEFLATE_NEED_OUTPUT when the
  * output does not fit out_capacity; DEFLATE_PARAM_ERROR for bad arguments;
- * DEFLATE_MEM_ERROR when the GPU is unusable.  In this build the GPU inflate
- * is switched off: after the argument checks the call returns
- * DEFLATE_MEM_ERROR, with the reason in orot_gpu_last_error(). */
+ * DEFLATE_MEM_ERROR when host memory runs out.  Served by the host inflater;
+ * bytes after the end of the stream are ignored. */
 deflate_result deflate_decompress(const void* in, size_t in_size, void* out, size_t out_capacity,
                                   size_t* actual_out_size, deflate_format format);
 
@@ -137,6 +149,22 @@ void deflate_stream_free(deflate_stream* s);
 deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_in, size_t* avail_in,
                                        uint8_t** next_out, size_t* avail_out, deflate_flush flush);
 
+/* Streaming decompressor (orot include/orot/deflate.h:111-118,
+ * src/api/stream_api.cpp:253-380), on the host.  inflate_stream_new returns
+ * NULL on allocation failure or an unknown format.  inflate_stream_decompress
+ * takes all of the input it is given (held until decoded) and returns
+ * DEFLATE_OK (more input needed), DEFLATE_NEED_OUTPUT (call again with more
+ * output space), DEFLATE_STREAM_END (the stream and its trailer are
+ * complete; bytes of this call after the end stay in *next_in and
+ * *avail_in), DEFLATE_DATA_ERROR (corrupt data, a bad header or a checksum /
+ * ISIZE mismatch; the stream stays failed) or DEFLATE_PARAM_ERROR (NULL
+ * pointers, or a compressor stream).  Unlike orot's inflater, a stored
+ * block that follows a Huffman block decodes correctly. */
+deflate_stream* inflate_stream_new(deflate_format format);
+void inflate_stream_free(deflate_stream* s);
+deflate_result inflate_stream_decompress(deflate_stream* s, const uint8_t** next_in, size_t* avail_in,
+                                         uint8_t** next_out, size_t* avail_out);
+
 /* Checksums (orot include/orot/deflate.h:149-150). */
 uint32_t deflate_adler32(uint32_t initial, const void* data, size_t len);
 uint32_t deflate_crc32(uint32_t initial, const void* data, size_t len);
@@ -154,7 +182,7 @@ namespace orot {
 namespace deflate {
 
 /* C++ wrappers with the names and behaviour of orot's header-only wrappers
- * (orot include/orot/deflate.h:188-211, 344-387). */
+ * (orot include/orot/deflate.h:188-389). */
 enum class Format : int { Raw = DEFLATE_FORMAT_RAW, Zlib = DEFLATE_FORMAT_ZLIB, Gzip = DEFLATE_FORMAT_GZIP };
 
 struct Level {
@@ -242,6 +270,48 @@ inline uint32_t adler32(std::span<const uint8_t> data, uint32_t init = 1) {
 inline uint32_t crc32(std::span<const uint8_t> data, uint32_t init = 0) {
     return deflate_crc32(init, data.data(), data.size());
 }
+
+/* orot include/orot/deflate.h:219-236: the whole-buffer bound, then one call. */
+inline std::vector<uint8_t> compress(std::span<const uint8_t> input, int level = Level::Default,
+                                     Format format = Format::Zlib) {
+    std::vector<uint8_t> out(deflate_compress_bound(input.size(), static_cast<deflate_format>(format)));
+    const size_t n = deflate_compress(input.data(), input.size(), out.data(), out.size(), level,
+                                      static_cast<deflate_format>(format));
+    if (n == 0) throw Error(DEFLATE_BUF_ERROR, "deflate::compress: output buffer overflow");
+    out.resize(n);
+    return out;
+}
+
+/* orot include/orot/deflate.h:308-342: feed() returns the bytes written and
+ * sets done at the end of the stream. */
+class Decompressor {
+public:
+    explicit Decompressor(Format fmt = Format::Zlib) : s_(inflate_stream_new(static_cast<deflate_format>(fmt))) {
+        if (!s_) throw Error(DEFLATE_MEM_ERROR, "Decompressor: allocation failed");
+    }
+    ~Decompressor() { inflate_stream_free(s_); }
+    Decompressor(const Decompressor&) = delete;
+    Decompressor& operator=(const Decompressor&) = delete;
+
+    size_t feed(std::span<const uint8_t> in, std::span<uint8_t> out, bool& done) {
+        const uint8_t* next_in = in.data();
+        size_t avail_in = in.size();
+        uint8_t* next_out = out.data();
+        size_t avail_out = out.size();
+        const deflate_result r = inflate_stream_decompress(s_, &next_in, &avail_in, &next_out, &avail_out);
+        if (r == DEFLATE_STREAM_END) {
+            done = true;
+        } else if (r < 0) {
+            throw Error(r, "Decompressor::feed failed");
+        } else {
+            done = false;
+        }
+        return out.size() - avail_out;
+    }
+
+private:
+    deflate_stream* s_;
+};
 
 /* orot include/orot/deflate.h:238-256: one call into a max_output buffer. */
 inline std::vector<uint8_t> decompress(std::span<const uint8_t> input, Format format = Format::Zlib,

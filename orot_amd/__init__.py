@@ -55,6 +55,10 @@ EXPORTED_SYMBOLS = (
     "deflate_stream_new",
     "deflate_stream_free",
     "deflate_stream_compress",
+    "inflate_stream_new",
+    "inflate_stream_free",
+    "inflate_stream_decompress",
+    "deflate_set_allocator",
     "deflate_server_new",
     "deflate_server_free",
     "deflate_server_submit_device",
@@ -104,6 +108,16 @@ class BatchConfig(ctypes.Structure):
 
     _fields_ = [("block_size", ctypes.c_size_t), ("hash_bits", ctypes.c_int), ("max_chain", ctypes.c_int),
                 ("nice_len", ctypes.c_int), ("lazy_depth", ctypes.c_int)]
+
+
+ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t)
+FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p)
+
+
+class Allocator(ctypes.Structure):
+    """deflate_allocator (orot include/orot/deflate.h:49-54)."""
+
+    _fields_ = [("alloc", ALLOC_FN), ("free", FREE_FN), ("opaque", ctypes.c_void_p)]
 
 
 class Format(enum.IntEnum):
@@ -191,6 +205,15 @@ def lib() -> ctypes.CDLL:
     L.deflate_stream_compress.restype = ctypes.c_int
     L.deflate_stream_compress.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(sz), ctypes.POINTER(vp),
                                           ctypes.POINTER(sz), ctypes.c_int]
+    L.inflate_stream_new.restype = vp
+    L.inflate_stream_new.argtypes = [ctypes.c_int]
+    L.inflate_stream_free.restype = None
+    L.inflate_stream_free.argtypes = [vp]
+    L.inflate_stream_decompress.restype = ctypes.c_int
+    L.inflate_stream_decompress.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(sz), ctypes.POINTER(vp),
+                                            ctypes.POINTER(sz)]
+    L.deflate_set_allocator.restype = None
+    L.deflate_set_allocator.argtypes = [ctypes.POINTER(Allocator)]
     L.deflate_crc32.argtypes = [u32, vp, sz]
     L.deflate_adler32.restype = u32
     L.deflate_adler32.argtypes = [u32, vp, sz]
@@ -302,6 +325,62 @@ def decompress(data, fmt: int = Format.GZIP, max_output: int = 256 << 20) -> byt
             cap = min(max_output, cap * 2)
             continue
         raise Error(r, f"deflate::decompress failed ({last_error()})")
+
+
+class Decompressor:
+    """orot::deflate::Decompressor (orot include/orot/deflate.h:308-342) over
+    inflate_stream_new / inflate_stream_decompress."""
+
+    def __init__(self, fmt: int = Format.ZLIB):
+        self._s = lib().inflate_stream_new(int(fmt))
+        if not self._s:
+            raise Error(Result.MEM_ERROR, "Decompressor: allocation failed")
+
+    def call(self, data, out: np.ndarray):
+        """One inflate_stream_decompress call: (result, input bytes used,
+        output bytes written)."""
+        a = _as_u8(data)
+        ni = ctypes.c_void_p(a.ctypes.data if a.size else None)
+        ai = ctypes.c_size_t(a.size)
+        no = ctypes.c_void_p(out.ctypes.data if out.size else None)
+        ao = ctypes.c_size_t(out.size)
+        r = lib().inflate_stream_decompress(self._s, ctypes.byref(ni), ctypes.byref(ai), ctypes.byref(no),
+                                            ctypes.byref(ao))
+        return int(r), a.size - ai.value, out.size - ao.value
+
+    def feed(self, data, out_chunk: int = 1 << 20) -> tuple[bytes, bool]:
+        """All of `data` through the stream, draining NEED_OUTPUT: (bytes
+        out, stream ended).  Raises Error on a failure code."""
+        a = _as_u8(data)
+        got, pos = [], 0
+        buf = np.empty(out_chunk, dtype=np.uint8)
+        while True:
+            r, used, n = self.call(a[pos:], buf)
+            pos += used
+            got.append(buf[:n].tobytes())
+            if r < 0:
+                raise Error(r, f"Decompressor::feed failed ({r})")
+            if r == Result.STREAM_END:
+                return b"".join(got), True
+            if r == Result.OK:
+                return b"".join(got), False
+
+    def close(self) -> None:
+        if self._s:
+            lib().inflate_stream_free(self._s)
+            self._s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def batch_decompress_device(d_in: int, d_out: int, items, fmt: int = Format.GZIP):

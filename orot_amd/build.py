@@ -27,7 +27,7 @@ COMMON = [
 ]
 
 SOURCES = ["deflate_kernels.hip", "deflate_l6.hip", "inflate.hip", "engine.cpp", "deflate_api.cpp",
-           "inflate_api.cpp", "server.cpp", "stream_api.cpp", "noalloc.hip"]
+           "inflate_api.cpp", "server.cpp", "stream_api.cpp", "noalloc.hip", "inflate_host.cpp"]
 
 
 def _run(cmd, verbose):
@@ -39,6 +39,20 @@ def _run(cmd, verbose):
         raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
     if verbose and out.strip():
         print(out)
+
+
+CPP_TESTS = [("tests/cpp/wrappers.cpp", "tests/cpp/wrappers")]
+
+
+def build_cpp_tests(verbose: bool = False) -> None:
+    """Test programs written against include/orot/deflate.h, linked with the
+    library (rpath relative to the program, so the tree can move)."""
+    for src, exe in CPP_TESTS:
+        out = os.path.join(ROOT, exe)
+        _run(["g++", "-std=c++20", "-O2", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+              os.path.join(ROOT, src), "-o", out + ".tmp", "-L" + HERE, "-lorot_amd",
+              "-Wl,-rpath,$ORIGIN/../../orot_amd", "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"], verbose)
+        os.replace(out + ".tmp", out)
 
 
 def build(verbose: bool = False) -> str:
@@ -53,6 +67,7 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
+    build_cpp_tests(verbose)
     return LIB
 
 

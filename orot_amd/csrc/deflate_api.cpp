@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -37,6 +38,26 @@ namespace orot_gpu {
 
 namespace {
 thread_local std::string g_last_error;
+
+std::mutex g_alloc_mu;
+deflate_allocator g_alloc = {
+    [](void*, size_t size, size_t align) -> void* {
+        void* p = nullptr;
+        return ::posix_memalign(&p, align < sizeof(void*) ? sizeof(void*) : align, size ? size : 1) == 0 ? p : nullptr;
+    },
+    [](void*, void* ptr) { ::free(ptr); },
+    nullptr,
+};
+}  // namespace
+
+deflate_allocator host_allocator() {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    return g_alloc;
+}
+
+void set_allocator(const deflate_allocator& a) {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    g_alloc = a;
 }
 
 void set_error(const char* where, hipError_t e) { g_last_error = std::string(where) + ": " + hipGetErrorString(e); }
@@ -199,6 +220,14 @@ int autotune(Engine& eng, const uint8_t* d_in, uint8_t* d_out, const deflate_bat
 }  // namespace
 
 extern "C" {
+
+// orot include/orot/deflate.h:49-61, src/api/deflate_api.cpp:20-77: a non-NULL
+// allocator replaces the current one (NULL is ignored).  Here it serves the
+// host memory of the decompress side (inflate_stream_new, deflate_decompress);
+// device and pinned memory come from the HIP runtime.
+void deflate_set_allocator(const deflate_allocator* alloc) {
+    if (alloc && alloc->alloc && alloc->free) orot_gpu::set_allocator(*alloc);
+}
 
 size_t deflate_compress_bound(size_t n, deflate_format format) {
     size_t b = n + (n >> 12) + (n >> 14) + (n >> 25) + 13;

@@ -1,14 +1,13 @@
-// inflate_api.cpp -- C-ABI host layer of the GPU inflate: deflate_decompress
-// (orot src/api/deflate_api.cpp:138-215), its device-resident variant and the
-// batched device entry point of deflate_gpu.h.  There is no CPU inflate: a
-// missing or failing GPU makes every entry point fail, with a message in
-// orot_gpu_last_error().
+// inflate_api.cpp -- C-ABI host layer of the decompress side: orot's
+// deflate_decompress (src/api/deflate_api.cpp:138-215) on the host inflater
+// of inflate_host.cpp, and the device-resident and batched GPU inflate entry
+// points of deflate_gpu.h.
 //
-// The inflate kernel is switched off (kInflateEnabled): on its first GPU run
-// it did not finish on a stored-block stream that the same kernel source
+// The GPU inflate kernel is switched off (kInflateEnabled): on its first GPU
+// run it did not finish on a stored-block stream that the same kernel source
 // decodes correctly under the lane-emulating CPU harness of
-// tests/test_inflate_emu.py.  Until the cause is found every entry point
-// returns its failure value after the argument checks.
+// tests/test_inflate_emu.py.  Until the cause is found the two GPU entry
+// points return their failure value after the argument checks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +19,7 @@
 #include "../../include/orot/deflate_gpu.h"
 #include "deflate_gpu.h"
 #include "host_common.h"
+#include "inflate_host.h"
 
 using orot_gpu::InflateItem;
 using orot_gpu::InflateResult;
@@ -138,32 +138,30 @@ deflate_result finish(const InflateResult& r, size_t* actual) {
 
 extern "C" {
 
+// orot's whole-buffer inflate of host buffers, on the host inflater
+// (inflate_host.cpp): the decompress side is not the GPU compress path.
 deflate_result deflate_decompress(const void* in, size_t in_size, void* out, size_t out_capacity,
                                   size_t* actual_out_size, deflate_format format) {
     if ((!in && in_size) || (!out && out_capacity) || !valid_format(format)) return DEFLATE_PARAM_ERROR;
-    if (inflate_disabled()) return DEFLATE_MEM_ERROR;
-    auto& s = inf();
-    std::lock_guard<std::mutex> lk(s.mu);
-    if (!s.init()) return DEFLATE_MEM_ERROR;
-    if (!grow(s.d_in, s.in_cap, in_size, "hipMalloc(inflate in)") ||
-        !grow(s.d_out, s.out_cap, out_capacity, "hipMalloc(inflate out)"))
-        return DEFLATE_MEM_ERROR;
-    hipError_t e = in_size ? hipMemcpyAsync(s.d_in, in, in_size, hipMemcpyHostToDevice, s.stream) : hipSuccess;
-    if (e != hipSuccess) {
-        set_error("inflate: copy in", e);
+    orot_gpu::HostInflater* h = orot_gpu::HostInflater::create((int)format);
+    if (!h) {
+        set_error_msg("deflate_decompress: allocation failed");
         return DEFLATE_MEM_ERROR;
     }
-    std::vector<InflateResult> res;
-    if (!s.run(s.d_in, s.d_out, {InflateItem{0, in_size, 0, out_capacity}}, (int)format, res))
-        return DEFLATE_MEM_ERROR;
-    if (res[0].status == DEFLATE_OK && res[0].out_size) {
-        e = hipMemcpy(out, s.d_out, (size_t)res[0].out_size, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) {
-            set_error("inflate: copy out", e);
-            return DEFLATE_MEM_ERROR;
-        }
+    size_t got = 0;
+    const deflate_result r = h->whole(static_cast<const uint8_t*>(in), in_size, static_cast<uint8_t*>(out),
+                                      out_capacity, &got);
+    orot_gpu::HostInflater::destroy(h);
+    if (r == DEFLATE_STREAM_END) {
+        if (actual_out_size) *actual_out_size = got;
+        return DEFLATE_OK;
     }
-    return finish(res[0], actual_out_size);
+    if (r == DEFLATE_NEED_OUTPUT) {
+        set_error_msg("deflate_decompress: output does not fit out_capacity");
+        return DEFLATE_NEED_OUTPUT;
+    }
+    set_error_msg(r == DEFLATE_OK ? "deflate_decompress: truncated stream" : "deflate_decompress: corrupt stream");
+    return DEFLATE_DATA_ERROR;
 }
 
 deflate_result deflate_decompress_device(const void* d_in, size_t in_size, void* d_out, size_t out_capacity,

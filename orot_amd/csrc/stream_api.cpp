@@ -35,6 +35,7 @@
 #include "deflate_gpu.h"
 #include "engine.h"
 #include "host_common.h"
+#include "inflate_host.h"
 
 using orot_gpu::BufferSpec;
 using orot_gpu::Engine;
@@ -144,6 +145,9 @@ private:
 }  // namespace
 
 struct deflate_stream {
+    // a streaming decompressor (inflate_stream_new) holds only this; the
+    // engine and buffers below stay unused
+    orot_gpu::HostInflater* inflater = nullptr;
     Engine eng;
     int level_arg = 0, format_arg = 0;  // as passed to deflate_stream_new (the reuse key)
     int format = 0;
@@ -169,6 +173,7 @@ struct deflate_stream {
     bool failed = false;
 
     ~deflate_stream() {
+        orot_gpu::HostInflater::destroy(inflater);
         if (inflight) {
             uint64_t got = 0;
             (void)eng.wait(&got);
@@ -361,6 +366,10 @@ deflate_stream* deflate_stream_new(int level, deflate_format format) {
 
 void deflate_stream_free(deflate_stream* s) {
     if (!s) return;
+    if (s->inflater) {
+        delete s;
+        return;
+    }
     if (s->inflight) {
         uint64_t got = 0;
         s->inflight = false;
@@ -378,7 +387,7 @@ void deflate_stream_free(deflate_stream* s) {
 
 deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_in, size_t* avail_in,
                                        uint8_t** next_out, size_t* avail_out, deflate_flush flush) {
-    if (!s || !next_in || !avail_in || !next_out || !avail_out || (*avail_in && !*next_in) ||
+    if (!s || s->inflater || !next_in || !avail_in || !next_out || !avail_out || (*avail_in && !*next_in) ||
         (*avail_out && !*next_out) || flush < DEFLATE_NO_FLUSH || flush > DEFLATE_FINISH)
         return DEFLATE_PARAM_ERROR;
     if (s->failed) return DEFLATE_MEM_ERROR;
@@ -427,6 +436,38 @@ deflate_result deflate_stream_compress(deflate_stream* s, const uint8_t** next_i
             s->failed = true;
             return DEFLATE_MEM_ERROR;
         }
+    }
+}
+
+// ---- streaming decompressor (orot src/api/stream_api.cpp:253-380) ----------
+
+deflate_stream* inflate_stream_new(deflate_format format) {
+    if (format != DEFLATE_FORMAT_RAW && format != DEFLATE_FORMAT_ZLIB && format != DEFLATE_FORMAT_GZIP) {
+        set_error_msg("inflate_stream_new: unknown format");
+        return nullptr;
+    }
+    auto* s = new (std::nothrow) deflate_stream;
+    if (!s) return nullptr;
+    s->inflater = orot_gpu::HostInflater::create((int)format);
+    if (!s->inflater) {
+        delete s;
+        set_error_msg("inflate_stream_new: allocation failed");
+        return nullptr;
+    }
+    return s;
+}
+
+void inflate_stream_free(deflate_stream* s) { deflate_stream_free(s); }
+
+deflate_result inflate_stream_decompress(deflate_stream* s, const uint8_t** next_in, size_t* avail_in,
+                                         uint8_t** next_out, size_t* avail_out) {
+    if (!s || !s->inflater || !next_in || !avail_in || !next_out || !avail_out || (*avail_in && !*next_in) ||
+        (*avail_out && !*next_out))
+        return DEFLATE_PARAM_ERROR;
+    try {
+        return s->inflater->stream(next_in, avail_in, next_out, avail_out);
+    } catch (const std::bad_alloc&) {
+        return DEFLATE_MEM_ERROR;
     }
 }
 

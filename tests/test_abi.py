@@ -22,7 +22,7 @@ def _declared():
         c_part = text.split("#ifdef __cplusplus\n#include")[0]
         for m in re.finditer(r"^\s*[A-Za-z_][\w \*]*?\b(\w+)\s*\(", c_part, re.M):
             name = m.group(1)
-            if name.startswith(("deflate_", "orot_gpu_")):
+            if name.startswith(("deflate_", "inflate_", "orot_gpu_")):
                 names.add(name)
     return names
 
@@ -132,10 +132,10 @@ def test_stream_arguments_and_no_gpu():
     assert r.returncode == 0
 
 
-def test_decompress_arguments_and_no_gpu():
-    """deflate_decompress rejects bad arguments with DEFLATE_PARAM_ERROR before
-    touching the GPU, and otherwise reports DEFLATE_MEM_ERROR: there is no CPU
-    inflate behind the ABI, and the GPU inflate is switched off."""
+def test_decompress_arguments_and_host_inflate():
+    """deflate_decompress rejects bad arguments with DEFLATE_PARAM_ERROR and
+    inflates host buffers on the host, with no GPU needed; the GPU inflate
+    entry points (device and batch) stay switched off."""
     import subprocess
     import sys
 
@@ -149,7 +149,8 @@ def test_decompress_arguments_and_no_gpu():
         "     L.deflate_decompress(buf,5,buf,64,ctypes.byref(got),7)]\n"
         "s=zlib.compress(b'abc')\n"
         "r=L.deflate_decompress(s,len(s),buf,64,ctypes.byref(got),1)\n"
-        "ok=all(b==-5 for b in bad) and r==-2\n"
+        "dev=L.deflate_decompress_device(s,len(s),buf,64,ctypes.byref(got),1)\n"
+        "ok=all(b==-5 for b in bad) and r==0 and buf.raw[:3]==b'abc' and dev==-2\n"
         "sys.exit(0 if ok else 1)\n"
     )
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env={**os.environ, "PYTHONPATH": ROOT})
