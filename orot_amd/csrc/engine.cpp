@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -152,7 +153,11 @@ bool Engine::init(const EngineConfig& cfg) {
         set_error("hipEventCreate", e);
         return false;
     }
-    exch_ok_ = exchange_order_ok(stream_);
+    // OROT_AMD_BALLOT_SCAN=1 takes the ballot match-any variant of the
+    // candidate scans even where same-address exchanges apply in lane order
+    // (both variants give the same codes; this keeps the fallback tested)
+    const char* force_ballot = std::getenv("OROT_AMD_BALLOT_SCAN");
+    exch_ok_ = !(force_ballot && force_ballot[0] == '1') && exchange_order_ok(stream_);
     e = set_kernel_attributes_l1(lds_max_);
     if (e == hipSuccess) e = set_kernel_attributes_l6(lds_max_);
     if (e != hipSuccess) {
