@@ -14,7 +14,9 @@
  *   - deflate_parallel_compress returns 0 instead of writing past
  *     out_capacity when the compressed stream does not fit;
  *   - num_threads is accepted and ignored (the GPU decides its parallelism);
- *   - levels 1-3 accept block sizes up to 256 KiB (L1) / 128 KiB (L2, L3);
+ *   - block sizes are accepted up to 1 GiB; levels 4-12 (whose bytes are
+ *     not orot's at any block size) run a block above 1 MiB as 1 MiB pieces,
+ *     each with the 32 KiB before it as its dictionary;
  *   - deflate_decompress reports an output buffer that is too small as
  *     DEFLATE_NEED_OUTPUT, as orot does with its libdeflate backend (its
  *     built-in inflater reports DEFLATE_DATA_ERROR);
@@ -87,12 +89,14 @@ void deflate_set_allocator(const deflate_allocator* alloc);
 size_t deflate_compress_bound(size_t in_size, deflate_format format);
 
 /* Whole-buffer compression (orot include/orot/deflate.h:69-77,
- * src/api/deflate_api.cpp:100-136).  Returns 0 when out_capacity is below
- * deflate_compress_bound(in_size, format), as orot does; the gzip header's XFL
- * byte is 2 for levels >= 9, 4 for levels <= 2, else 0
- * (src/formats/gzip_wrapper.cpp:31).  The stream is produced by the same
- * chunked GPU pipeline as deflate_parallel_compress (the level's default
- * chunk size), not as orot's single block. */
+ * src/api/deflate_api.cpp:100-136, src/compress/block_compressor.cpp:16-73).
+ * Returns 0 when out_capacity is below deflate_compress_bound(in_size,
+ * format), as orot does; the gzip header's XFL byte is 2 for levels >= 9, 4
+ * for levels <= 2, else 0 (src/formats/gzip_wrapper.cpp:31).  Levels 0-3 give
+ * orot's bytes: the whole input is one block (stored blocks at level 0, and
+ * an input with no match at all is stored), for inputs of up to 1 GiB.
+ * Levels 4-12 give the L6-class stream of deflate_parallel_compress with the
+ * level's default block size. */
 size_t deflate_compress(const void* in, size_t in_size, void* out, size_t out_capacity, int level,
                         deflate_format format);
 
@@ -108,9 +112,11 @@ deflate_result deflate_decompress(const void* in, size_t in_size, void* out, siz
                                   size_t* actual_out_size, deflate_format format);
 
 /* Parallel compressor (orot include/orot/deflate.h:120-145,
- * src/api/stream_api.cpp:384-406).  block_size 0 selects 128 KiB for levels
- * <= 3, 512 KiB for levels <= 6 and 1 MiB above.  Returns NULL when no GPU is
- * usable or on allocation failure. */
+ * src/api/stream_api.cpp:384-406, src/parallel/parallel_compressor.cpp:22-28).
+ * block_size 0 selects 128 KiB for levels <= 3, 512 KiB for levels <= 6 and
+ * 1 MiB above; any block size up to 1 GiB is accepted (levels 0-3 give orot's
+ * bytes at every one).  Returns NULL when no GPU is usable, on allocation
+ * failure or for a block size above 1 GiB. */
 typedef struct deflate_parallel_ctx deflate_parallel_ctx;
 
 deflate_parallel_ctx* deflate_parallel_new(int level, deflate_format format, int num_threads, size_t block_size);

@@ -952,20 +952,36 @@ static size_t default_block_size(int level) {
 }
 
 static size_t parallel_compress_impl(int level, int format, int threads, size_t block_size, const uint8_t* src,
-                                     size_t n, uint8_t* dst, size_t cap, int l6gpu);
+                                     size_t n, uint8_t* dst, size_t cap, int l6gpu, int whole);
 
 size_t oracle_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src, size_t n,
                                 uint8_t* dst, size_t cap) {
-    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, 0);
+    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, 0, 0);
 }
+
+/* The L6 class takes chunks of at most 1 MiB: a larger block runs as 1 MiB
+ * pieces, each with the 32 KiB before it as its dictionary. */
+#define L6_MAX_CHUNK ((size_t)1 << 20)
 
 size_t oracle_gpu_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src,
                                     size_t n, uint8_t* dst, size_t cap) {
-    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, clamp_level(level) >= 4);
+    const int l6 = clamp_level(level) >= 4;
+    if (l6 && block_size > L6_MAX_CHUNK) block_size = L6_MAX_CHUNK;
+    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, l6, 0);
+}
+
+size_t oracle_whole_compress(int level, int format, const uint8_t* src, size_t n, uint8_t* dst, size_t cap) {
+    const int lv = clamp_level(level);
+    if (cap < oracle_compress_bound(n, format)) return 0; /* raw_deflate.cpp:20, zlib/gzip wrappers */
+    size_t r;
+    if (lv >= 4) r = oracle_gpu_parallel_compress(lv, format, 1, 0, src, n, dst, cap);
+    else r = parallel_compress_impl(lv, format, 1, n ? n : 1, src, n, dst, cap, 0, 1);
+    if (r && format == 2) dst[8] = (uint8_t)(lv >= 9 ? 2 : (lv <= 2 ? 4 : 0)); /* gzip_wrapper.cpp:31 */
+    return r;
 }
 
 static size_t parallel_compress_impl(int level, int format, int threads, size_t block_size, const uint8_t* src,
-                                     size_t n, uint8_t* dst, size_t cap, int l6gpu) {
+                                     size_t n, uint8_t* dst, size_t cap, int l6gpu, int whole) {
     const int lv = clamp_level(level);
     if (LEVELS[lv].bt4 && !l6gpu) return 0;
     if (block_size == 0) block_size = default_block_size(level);
@@ -1023,6 +1039,11 @@ static size_t parallel_compress_impl(int level, int format, int threads, size_t 
             for (size_t i = 0; i < nb; ++i) {
                 const int last = i == nb - 1;
                 if (lv == 0) {
+                    bitbuf b = {tmp, bp};
+                    emit_stored(&b, jobs[i].src, jobs[i].len, last);
+                    bp = b.pos;
+                } else if (whole && jobs[i].ntok == jobs[i].len) {
+                    /* whole buffer, no match at all: stored (block_compressor.cpp:60-61) */
                     bitbuf b = {tmp, bp};
                     emit_stored(&b, jobs[i].src, jobs[i].len, last);
                     bp = b.pos;

@@ -72,6 +72,14 @@ size_t oracle_l6_lz77_h(const uint8_t* src, size_t halo, size_t n, int level, ui
 size_t oracle_gpu_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src,
                                     size_t n, uint8_t* dst, size_t cap);
 
+/* orot's whole-buffer deflate_compress (src/api/deflate_api.cpp:100-136,
+ * src/compress/block_compressor.cpp:16-73, the zlib and gzip wrappers):
+ * 0 when cap is below the bound; levels 0-3: the whole input as one chunk,
+ * stored when it holds no match, orot's bytes (IEEE CRC-32); levels 4-12: the
+ * HIP path's L6-class stream with the level's default chunks.  gzip XFL by
+ * level. */
+size_t oracle_whole_compress(int level, int format, const uint8_t* src, size_t n, uint8_t* dst, size_t cap);
+
 /* Single-chunk block emission with orot's Fixed/Stored hint
  * (src/deflate/deflate_block.cpp:389-474, hint Fixed).  Appends to a bit
  * buffer starting at bit offset *bitpos of dst (dst must be zero-filled).

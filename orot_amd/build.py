@@ -27,7 +27,8 @@ COMMON = [
 ]
 
 SOURCES = ["deflate_kernels.hip", "deflate_l6.hip", "inflate.hip", "engine.cpp", "deflate_api.cpp",
-           "inflate_api.cpp", "server.cpp", "stream_api.cpp", "noalloc.hip", "inflate_host.cpp"]
+           "inflate_api.cpp", "server.cpp", "stream_api.cpp", "noalloc.hip", "inflate_host.cpp",
+           "deflate_big.hip"]
 
 
 def _run(cmd, verbose):

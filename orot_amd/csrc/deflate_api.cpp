@@ -305,10 +305,35 @@ int deflate_parallel_stage_times(const deflate_parallel_ctx* ctx, float* ms, int
 // ---- whole buffer ----------------------------------------------------------
 
 // deflate_compress (orot include/orot/deflate.h:69-77, src/api/deflate_api.cpp:
-// 100-136 -> src/formats/{raw_deflate,zlib_wrapper,gzip_wrapper}.cpp): orot's
-// rules for this entry point -- 0 when out_capacity is below the bound, the
-// gzip XFL byte by level (gzip_wrapper.cpp:31) -- over the chunked GPU
-// pipeline with the level's default chunk size.
+// 100-136 -> src/formats/{raw_deflate,zlib_wrapper,gzip_wrapper}.cpp, src/
+// compress/block_compressor.cpp:16-73): 0 when out_capacity is below the
+// bound; the gzip XFL byte by level (gzip_wrapper.cpp:31).  Levels 0-3 give
+// orot's bytes: the whole input is one chunk (one block, or stored blocks),
+// and an input with no match is stored.  Levels 4-12 run the L6-class
+// pipeline with the level's default chunks.
+namespace {
+
+// Chunk capacity of a whole-buffer context for an input of n bytes: powers of
+// two from 64 KiB, so that contexts re-lay their arenas only when an input
+// needs a larger one.
+uint32_t whole_bucket(size_t n) {
+    uint32_t b = 1u << 16;
+    while (b < n && b < orot_gpu::MAX_CHUNK_BIG) b <<= 1;
+    return b;
+}
+
+deflate_parallel_ctx* new_ctx(const EngineConfig& cfg) {
+    auto* ctx = new (std::nothrow) deflate_parallel_ctx;
+    if (!ctx) return nullptr;
+    if (!ctx->eng.init(cfg)) {
+        delete ctx;
+        return nullptr;
+    }
+    return ctx;
+}
+
+}  // namespace
+
 size_t deflate_compress(const void* in, size_t n, void* out, size_t cap, int level, deflate_format format) {
     if (!out || (!in && n)) return 0;
     if (cap < deflate_compress_bound(n, format)) {
@@ -317,16 +342,37 @@ size_t deflate_compress(const void* in, size_t n, void* out, size_t cap, int lev
     }
     const int lv = level < 0 ? 0 : (level > 12 ? 12 : level);
     const int fmt = (format == DEFLATE_FORMAT_ZLIB || format == DEFLATE_FORMAT_GZIP) ? (int)format : 0;
+    const bool whole = lv <= 3;
+    if (whole && n > orot_gpu::MAX_CHUNK_BIG) {
+        set_error_msg("deflate_compress: levels 0-3 take inputs of at most 1 GiB");
+        return 0;
+    }
     // one context per (level, format), created on first use
     static std::mutex mu;
     static deflate_parallel_ctx* ctxs[13][3] = {};
     deflate_parallel_ctx* ctx;
     {
         std::lock_guard<std::mutex> lk(mu);
-        if (!ctxs[lv][fmt]) ctxs[lv][fmt] = deflate_parallel_new(lv, (deflate_format)fmt, 0, 0);
+        if (!ctxs[lv][fmt]) {
+            EngineConfig cfg;
+            cfg.level = lv;
+            cfg.format = fmt;
+            cfg.whole = whole;
+            cfg.chunk = whole ? whole_bucket(n) : 0;
+            ctxs[lv][fmt] = new_ctx(cfg);
+        }
         ctx = ctxs[lv][fmt];
     }
     if (!ctx) return 0;
+    if (whole) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        const uint32_t b = whole_bucket(n);
+        if (ctx->eng.config().chunk < n || ctx->eng.config().chunk > 4 * (uint64_t)b) {
+            EngineConfig cfg = ctx->eng.config();
+            cfg.chunk = b;
+            if (!ctx->eng.set_config(cfg)) return 0;
+        }
+    }
     const size_t got = deflate_parallel_compress(ctx, in, n, out, cap);
     if (got && fmt == DEFLATE_FORMAT_GZIP) static_cast<uint8_t*>(out)[8] = lv >= 9 ? 2 : (lv <= 2 ? 4 : 0);
     return got;

@@ -118,6 +118,7 @@ struct Plan {
     Result* d_result;       // nbufs
     hipStream_t stream;
     hipEvent_t ev[8];       // optional timing events (nullptr = off)
+    int whole;              // levels 1-3: one chunk per buffer with orot's whole-buffer rules (all-literal -> stored)
     bool exch_ok;           // LDS exchanges of one wave apply in lane order
     size_t lds_max;         // largest dynamic LDS allocation per workgroup
     int cus;                // compute units
@@ -130,6 +131,23 @@ hipError_t check_exchange_order_at(hipStream_t stream, uint32_t* d_word, bool* o
 
 // Largest chunk the L1-class kernels accept for the given hash width.
 uint32_t max_chunk_for_hash_bits(int hash_bits);
+
+// Levels 1-3 with chunks above the single-workgroup kernels' limit (or a
+// whole-buffer call) take the big-chunk path of deflate_big.hip: token words
+// and candidates (d_tok, d_aux: big_position_stride per chunk), the
+// token-start bitmap (d_pd: big_bitmap_words 32-bit words per chunk) and the
+// segment tables (d_blocks: big_segtab_words 32-bit words per chunk).
+constexpr uint32_t MAX_CHUNK_BIG = 1u << 30;
+constexpr uint32_t BIG_SEG = 65536;  // positions per candidate segment
+bool l1_big_chunk(uint32_t chunk, int hash_bits);
+__host__ __device__ constexpr uint64_t big_position_stride(uint32_t chunk) { return ((uint64_t)chunk + 7u) & ~7ull; }
+__host__ __device__ constexpr uint64_t big_bitmap_words(uint32_t chunk) { return ((uint64_t)chunk + 31u) / 32u + 1u; }
+__host__ __device__ constexpr uint32_t big_segs(uint32_t chunk) { return (chunk + BIG_SEG - 1) / BIG_SEG; }
+__host__ __device__ constexpr uint64_t big_segtab_words(uint32_t chunk, int hash_bits) {
+    return (uint64_t)big_segs(chunk) << hash_bits;
+}
+hipError_t set_kernel_attributes_big();
+hipError_t launch_l1_big(const Plan& p, int whole);
 
 // Per-chunk stride (32-bit words) of the L1-class token words: the chunk
 // plus one partial range of the interleaved layout.

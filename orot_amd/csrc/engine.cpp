@@ -78,10 +78,9 @@ bool resolve_config(EngineConfig& cfg) {
             set_error_msg("hash_bits must be 12 or 14 for levels 1-3");
             return false;
         }
-        const uint32_t maxc = max_chunk_for_hash_bits(cfg.hash_bits);
-        if (cfg.chunk > maxc) {
-            set_error_msg("block_size " + std::to_string(cfg.chunk) + " exceeds the GPU limit " +
-                          std::to_string(maxc) + " for level " + std::to_string(lv));
+        // above the single-workgroup kernels' limit the big-chunk path runs
+        if (cfg.chunk > MAX_CHUNK_BIG) {
+            set_error_msg("block_size " + std::to_string(cfg.chunk) + " exceeds the GPU limit of 1 GiB");
             return false;
         }
     } else if (lv >= 4) {
@@ -91,12 +90,11 @@ bool resolve_config(EngineConfig& cfg) {
             set_error_msg("invalid matcher parameters");
             return false;
         }
-        if (cfg.chunk > l6_max_chunk()) {
-            set_error_msg("block_size " + std::to_string(cfg.chunk) + " exceeds the GPU limit " +
-                          std::to_string(l6_max_chunk()) + " for level " + std::to_string(lv));
-            return false;
-        }
-    } else if (cfg.chunk > (1u << 30)) {
+        // Larger L6-class blocks run as pieces of the kernels' largest chunk,
+        // each with the 32 KiB before it as its dictionary (the class's own
+        // definition; its bytes are not orot's at any block size).
+        if (cfg.chunk > l6_max_chunk()) cfg.chunk = l6_max_chunk();
+    } else if (cfg.chunk > MAX_CHUNK_BIG) {
         set_error_msg("block_size exceeds 1 GiB");
         return false;
     }
@@ -160,6 +158,7 @@ bool Engine::init(const EngineConfig& cfg) {
     exch_ok_ = !(force_ballot && force_ballot[0] == '1') && exchange_order_ok(stream_);
     e = set_kernel_attributes_l1(lds_max_);
     if (e == hipSuccess) e = set_kernel_attributes_l6(lds_max_);
+    if (e == hipSuccess) e = set_kernel_attributes_big();
     if (e != hipSuccess) {
         set_error("hipFuncSetAttribute", e);
         return false;
@@ -185,8 +184,9 @@ void Engine::drop_graph() {
 bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     const bool l6 = cfg_.level >= 4;
     const bool l1 = cfg_.level >= 1;
+    const bool big = l1 && !l6 && (cfg_.whole || l1_big_chunk(cfg_.chunk, cfg_.hash_bits));
     if (nchunks <= chunk_cap_ && nbufs <= buf_cap_ && arena_chunk_ == cfg_.chunk && (!l6 || arena_l6_) &&
-        (!l1 || d_tok_))
+        (!l1 || d_tok_) && (!big || arena_big_))
         return true;
     // grow by at least half again, so a stream of growing batches settles quickly
     const uint32_t ncap = std::max(nchunks, (arena_chunk_ == cfg_.chunk ? chunk_cap_ + chunk_cap_ / 2 : 0u));
@@ -209,12 +209,21 @@ bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     h_res_ = nullptr;
     chunk_cap_ = buf_cap_ = 0;
     const uint32_t chunk = cfg_.chunk;
-    const size_t per_pos = (size_t)ncap * (l6 ? l6_position_stride(chunk) : l1_position_stride(chunk)) + 8;
+    const size_t per_pos =
+        (size_t)ncap * (l6 ? l6_position_stride(chunk) : (big ? big_position_stride(chunk) : l1_position_stride(chunk))) +
+        8;
     bool ok = true;
     if (l1) {
+        // big chunks keep their token-start bitmap where the others keep 16-bit codes
+        const size_t pd = big ? std::max(per_pos, (size_t)ncap * big_bitmap_words(chunk) * 2 + 8) : per_pos;
         ok = ok && dev_alloc(d_tok_, per_pos, "hipMalloc(token words)") &&
-             dev_alloc(d_pd_, per_pos, "hipMalloc(candidates / chain steps)") &&
+             dev_alloc(d_pd_, pd, "hipMalloc(candidates / chain steps)") &&
              dev_alloc(d_stage_, (size_t)ncap * stage_words_for_chunk(chunk), "hipMalloc(staging)");
+    }
+    if (big) {
+        ok = ok && dev_alloc(d_aux_, per_pos, "hipMalloc(candidates)") &&
+             dev_alloc(d_blocks_, (size_t)ncap * big_segtab_words(chunk, cfg_.hash_bits) * 4,
+                       "hipMalloc(segment tables)");
     }
     if (l6) {
         const size_t mb = l6_max_blocks(chunk);
@@ -237,6 +246,7 @@ bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     buf_cap_ = bcap;
     arena_chunk_ = chunk;
     arena_l6_ = l6;
+    arena_big_ = big;
     ++gen_;
     return true;
 }
@@ -297,6 +307,7 @@ Plan Engine::make_plan(const uint8_t* d_in, uint8_t* d_out, uint32_t nchunks, ui
     p.hash_bits = cfg_.hash_bits;
     p.l6 = cfg_.l6;
     p.format = cfg_.format;
+    p.whole = cfg_.whole ? 1 : 0;
     p.d_tok = d_tok_;
     p.d_pd = d_pd_;
     p.d_aux = d_aux_;

@@ -30,6 +30,7 @@ struct EngineConfig {
     uint32_t chunk = 0;   // DEFLATE block size (0: orot's default for the level)
     int hash_bits = 0;    // L1 class: 12 or 14 (0: the level's own)
     L6Params l6{};        // L6 class (max_chain 0: the level's own)
+    bool whole = false;   // levels 0-3: orot's whole-buffer rules (chunk must cover each buffer)
 };
 
 // Fills the zero fields of cfg with the level's defaults and checks the
@@ -100,6 +101,7 @@ private:
     uint32_t buf_cap_ = 0;
     uint32_t arena_chunk_ = 0;     // chunk size the arenas were sized for
     bool arena_l6_ = false;
+    bool arena_big_ = false;
     uint64_t gen_ = 0;             // bumped when any arena moves
 
     // last submit

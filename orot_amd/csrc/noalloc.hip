@@ -92,20 +92,26 @@ Layout layout_for(const EngineConfig& cfg, uint32_t nchunks) {
         return at;
     };
     const bool l1 = cfg.level >= 1, l6 = cfg.level >= 4;
+    const bool big = l1 && !l6 && l1_big_chunk(cfg.chunk, cfg.hash_bits);
     const size_t per_pos =
-        (size_t)nchunks * (l6 ? l6_position_stride(cfg.chunk) : l1_position_stride(cfg.chunk)) + 8;
+        (size_t)nchunks *
+            (l6 ? l6_position_stride(cfg.chunk) : (big ? big_position_stride(cfg.chunk) : l1_position_stride(cfg.chunk))) +
+        8;
+    const size_t pd = big ? (size_t)nchunks * big_bitmap_words(cfg.chunk) * 4 + 16 : per_pos * sizeof(uint16_t);
     const size_t mb = l6 ? l6_max_blocks(cfg.chunk) : 0;
     L.hdr = take(sizeof(BatchHdr));
     L.bufs = take(sizeof(BufDesc));
     L.desc = take((size_t)nchunks * sizeof(ChunkDesc));
     L.order = take((size_t)nchunks * sizeof(uint32_t));
     L.tok = l1 ? take(per_pos * sizeof(uint32_t)) : 0;
-    L.pd = l1 ? take(per_pos * sizeof(uint16_t)) : 0;
+    L.pd = l1 ? take(pd) : 0;
     L.stage = l1 ? take((size_t)nchunks * stage_words_for_chunk(cfg.chunk) * sizeof(uint32_t)) : 0;
-    L.aux = l6 ? take(per_pos * sizeof(uint32_t)) : 0;
+    L.aux = (l6 || big) ? take(per_pos * sizeof(uint32_t)) : 0;
     L.ntok = l6 ? take((size_t)nchunks * sizeof(uint32_t)) : 0;
     L.blkpos = l6 ? take((size_t)nchunks * mb * sizeof(uint32_t)) : 0;
-    L.blocks = l6 ? take((size_t)nchunks * mb * l6_block_record_bytes()) : 0;
+    L.blocks = l6    ? take((size_t)nchunks * mb * l6_block_record_bytes())
+               : big ? take((size_t)nchunks * big_segtab_words(cfg.chunk, cfg.hash_bits) * 4)
+                     : 0;
     L.info = take((size_t)nchunks * sizeof(ChunkInfo));
     L.res = take(sizeof(Result));
     L.total = o;
@@ -156,6 +162,7 @@ DeviceScratch* device_scratch() {
         d.lds_max = (size_t)v;
     e = set_kernel_attributes_l1(d.lds_max);
     if (e == hipSuccess) e = set_kernel_attributes_l6(d.lds_max);
+    if (e == hipSuccess) e = set_kernel_attributes_big();
     if (e != hipSuccess) {
         set_error("deflate_parallel_compress_device_noalloc: hipFuncSetAttribute", e);
         return nullptr;
@@ -232,10 +239,12 @@ size_t deflate_parallel_compress_device_noalloc(int level, deflate_format format
     p.d_tok = cfg.level >= 1 ? reinterpret_cast<uint32_t*>(base + L.tok) : nullptr;
     p.d_pd = cfg.level >= 1 ? reinterpret_cast<uint16_t*>(base + L.pd) : nullptr;
     p.d_stage = cfg.level >= 1 ? reinterpret_cast<uint32_t*>(base + L.stage) : nullptr;
-    p.d_aux = cfg.level >= 4 ? reinterpret_cast<uint32_t*>(base + L.aux) : nullptr;
+    const bool big = cfg.level >= 1 && cfg.level <= 3 && l1_big_chunk(cfg.chunk, cfg.hash_bits);
+    p.d_aux = (cfg.level >= 4 || big) ? reinterpret_cast<uint32_t*>(base + L.aux) : nullptr;
     p.d_ntok = cfg.level >= 4 ? reinterpret_cast<uint32_t*>(base + L.ntok) : nullptr;
     p.d_blkpos = cfg.level >= 4 ? reinterpret_cast<uint32_t*>(base + L.blkpos) : nullptr;
-    p.d_blocks = cfg.level >= 4 ? static_cast<void*>(base + L.blocks) : nullptr;
+    p.d_blocks = (cfg.level >= 4 || big) ? static_cast<void*>(base + L.blocks) : nullptr;
+    p.whole = 0;
     p.stage_words = stage_words_for_chunk(cfg.chunk);
     p.d_info = reinterpret_cast<ChunkInfo*>(base + L.info);
     p.d_result = reinterpret_cast<Result*>(base + L.res);

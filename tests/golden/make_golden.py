@@ -96,9 +96,28 @@ def main() -> int:
                     ok = False
                 out["levels_l6"].append({"kind": kind, "n": n, "seed": seed, "level": level, "format": 0,
                                          "block": 0, "size": len(comp), "zlib_ok": ok})
+    # 5. orot's whole-buffer deflate_compress (one block per input) at levels 0-3
+    out["whole"] = []
+    for kind in corpus.GENERATORS + ("mixed",):
+        for n in (0, 1, 5, 4096, 300_000, 2_000_000):
+            seed = 888 + n
+            if kind == "mixed":
+                q = n // 4
+                data = np.concatenate([corpus.generate(k, q if i < 3 else n - 3 * q, seed + i)
+                                       for i, k in enumerate(corpus.GENERATORS)])
+            else:
+                data = corpus.generate(kind, n, seed)
+            for level in (0, 1, 2, 3):
+                for fmt in (0, 1, 2):
+                    comp = orc.ref_compress(data, level, fmt)
+                    e = {"kind": kind, "n": n, "seed": seed, "level": level, "format": fmt, "size": len(comp),
+                         "sha256": sha(comp)}
+                    if n <= 4096:
+                        e["hex"] = comp.hex()
+                    out["whole"].append(e)
     with open(os.path.join(HERE, "orot_fixtures.json"), "w") as f:
         json.dump(out, f, indent=0)
-    # 5. the benchmark workload (256 MiB, four 64 MiB segments, L1 raw, 128 KiB)
+    # 6. the benchmark workload (256 MiB, four 64 MiB segments, L1 raw, 128 KiB)
     import bench  # noqa: E402
 
     data = bench.workload()

@@ -49,6 +49,8 @@ def oracle() -> ctypes.CDLL:
         L.oracle_l6_match_h.argtypes = [_vp, _sz, _sz, ctypes.c_int, _vp, _vp]
         L.oracle_l6_lz77_h.restype = _sz
         L.oracle_l6_lz77_h.argtypes = [_vp, _sz, _sz, ctypes.c_int, _vp]
+        L.oracle_whole_compress.restype = _sz
+        L.oracle_whole_compress.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _sz, _vp, _sz]
         L.oracle_encode_blocks_dyn.restype = None
         L.oracle_encode_blocks_dyn.argtypes = [_vp, _sz, _vp, _sz, ctypes.c_int, ctypes.c_int, _vp, _vp]
         L.oracle_encode_block_forced.restype = None
@@ -131,6 +133,18 @@ def gpu_parallel_compress(data, level: int, fmt: int, block_size: int = 0, threa
                                               out.ctypes.data, cap)
     if n == 0:
         raise RuntimeError("oracle_gpu_parallel_compress failed")
+    return out[:n].tobytes()
+
+
+def whole_compress(data, level: int, fmt: int) -> bytes:
+    """orot's whole-buffer deflate_compress as the HIP path computes it
+    (orot's bytes at levels 0-3)."""
+    a = _u8(data)
+    cap = a.size + (a.size >> 12) + (a.size >> 14) + (a.size >> 25) + 13 + 18
+    out = np.zeros(cap, dtype=np.uint8)
+    n = oracle().oracle_whole_compress(level, fmt, a.ctypes.data, a.size, out.ctypes.data, cap)
+    if n == 0:
+        raise RuntimeError("oracle_whole_compress failed")
     return out[:n].tobytes()
 
 

@@ -575,3 +575,38 @@ def test_orot_inflates_gpu_path_multichunk(kind):
         assert zlib.decompress(comp, 31) == data.tobytes()
         got = orc.ref_decompress_guarded(comp, 2, data.size)
         assert got is not None and got[0] == 0 and got[1] == data.tobytes(), (level, block, got and got[0])
+
+
+# ---- block sizes beyond orot's defaults and the whole-buffer entry point,
+# against orot's own build and orot's golden whole-buffer bytes
+@pytest.mark.skipif(not orc.ref_available(), reason="orot not built in this checkout (oracle/_ref)")
+@pytest.mark.parametrize("block", [131073, 600_000, 1 << 20, 3_000_017])
+def test_oracle_big_blocks_equal_orot_build(block):
+    data = np.concatenate([corpus.generate(k, 1_000_000, 13 + i) for i, k in enumerate(corpus.GENERATORS)])
+    for level in (1, 2, 3):
+        for fmt in (0, 2):
+            assert orc.parallel_compress(data, level, fmt, block, threads=4) == \
+                orc.ref_parallel_compress(data, level, fmt, block, threads=4), (level, fmt)
+
+
+def test_golden_whole_buffer(fixtures):
+    for e in fixtures["whole"]:
+        n, seed = e["n"], e["seed"]
+        if e["kind"] == "mixed":
+            q = n // 4
+            data = np.concatenate([corpus.generate(k, q if i < 3 else n - 3 * q, seed + i)
+                                   for i, k in enumerate(corpus.GENERATORS)])
+        else:
+            data = corpus.generate(e["kind"], n, seed)
+        got = orc.whole_compress(data, e["level"], e["format"])
+        if "hex" in e:
+            assert got.hex() == e["hex"], e
+        assert len(got) == e["size"] and hashlib.sha256(got).hexdigest() == e["sha256"], e
+
+
+def test_whole_buffer_closed_forms():
+    """orot stores an input with no match (block_compressor.cpp:60-61): the
+    empty input is one empty stored block, "a" a one-byte stored block."""
+    assert orc.whole_compress(b"", 1, 0) == bytes.fromhex("010000ffff")
+    assert orc.whole_compress(b"a", 1, 0) == bytes.fromhex("010100feff61")
+    assert orc.whole_compress(b"a", 6, 0) == bytes.fromhex("4b0400")  # L6 class: the chunked pipeline
