@@ -56,6 +56,7 @@ typedef struct deflate_batch_config {
     int max_chain;
     int nice_len;
     int lazy_depth;
+    int dictionary; /* levels 1-3: 1 = each chunk sees the 32 KiB before it (not orot's bytes); 0 = off */
 } deflate_batch_config;
 
 int deflate_batch_get_config(deflate_batch_ctx* ctx, deflate_batch_config* cfg);

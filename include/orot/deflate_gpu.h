@@ -34,6 +34,15 @@ size_t deflate_parallel_compress_device(deflate_parallel_ctx* ctx, const void* d
 size_t deflate_parallel_compress_device_noalloc(int level, deflate_format format, size_t block_size, const void* d_in,
                                                 size_t in_size, void* d_out, size_t out_capacity);
 
+/* No counterpart in orot's parallel API (orot's lz77_insert_dict,
+ * src/deflate/lz77.cpp:43-55, is never called by its parallel path).  At
+ * levels 1-3, on = 1 lets each chunk after the first of its buffer match into
+ * the 32 KiB before it: the chunk's LZ77 runs with those bytes in its hash
+ * table, its tokens cover its own bytes, and back-references may reach into
+ * the previous chunk.  Smaller output, not orot's bytes; off by default.
+ * Levels 4-12 always use the dictionary.  Returns 0, or -1 for a NULL ctx. */
+int deflate_parallel_set_dictionary(deflate_parallel_ctx* ctx, int on);
+
 /* Per-stage device time of the last call on ctx, in milliseconds, measured
  * with HIP events on the context's stream: [0] checksum, [1] LZ77 + block
  * emission, [2] offset scan + framing, [3] assembly.  Timing is recorded

@@ -282,6 +282,57 @@ static size_t lz77_fast(const uint8_t* src, int n, int bits, uint32_t* head, uin
     return nt;
 }
 
+/* The fast path over a chunk that follows `halo` bytes of its buffer (the
+ * L1-class dictionary mode of the HIP path; orot's own dictionary insertion,
+ * lz77.cpp:43-55, hashes with 16 bits into tables the fast path reads with
+ * hash_bits and stores dictionary offsets, so orot has no usable L1-class
+ * dictionary to follow).  Positions count from the dictionary's first byte,
+ * which is the empty sentinel as position 0 is; every dictionary position is
+ * in the table before the chunk's first probe, exactly as every earlier chunk
+ * position is; tokens cover the chunk only. */
+static size_t lz77_fast_h(const uint8_t* src, int halo, int n, int bits, uint32_t* head, uint32_t* out) {
+    const int nx = halo + n;
+    for (int q = 0; q < halo && q + 4 <= nx; ++q) head[hash4(src + q, bits)] = (uint32_t)q;
+    size_t nt = 0;
+    int pos = halo;
+    while (pos < nx) {
+        int len = 0, dist = 0;
+        if (nx - pos >= 4) {
+            const uint32_t h = hash4(src + pos, bits);
+            const uint32_t cur = head[h];
+            head[h] = (uint32_t)pos;
+            if (cur != 0) {
+                const int d = (pos - (int)cur) & WIN_MASK;
+                if (d != 0 && src[cur] == src[pos]) {
+                    const int l = lcp(src + pos, src + pos - d, nx - pos < 258 ? nx - pos : 258);
+                    if (l >= 3) {
+                        len = l;
+                        dist = d;
+                    }
+                }
+            }
+        }
+        if (len < 3) {
+            out[nt++] = src[pos++];
+        } else {
+            out[nt++] = ORACLE_TOK_MATCH | ((uint32_t)len << 16) | (uint32_t)dist;
+            for (int i = pos + 1; i < pos + len && i + 4 <= nx; ++i) head[hash4(src + i, bits)] = (uint32_t)i;
+            pos += len;
+        }
+    }
+    return nt;
+}
+
+size_t oracle_l1_lz77_h(const uint8_t* src, size_t halo, size_t n, int level, uint32_t* tokens) {
+    const lz_cfg* c = &LEVELS[clamp_level(level)];
+    if (!c->fast_path || n == 0) return 0;
+    uint32_t* head = (uint32_t*)calloc((size_t)1 << c->hash_bits, sizeof(uint32_t));
+    if (!head) return 0;
+    const size_t r = lz77_fast_h(src, (int)halo, (int)n, c->hash_bits, head, tokens);
+    free(head);
+    return r;
+}
+
 /* Chain walk of the standard path (lz77.cpp:66-156).  The 8-byte and
  * byte-at-best_len rejects of the original only skip candidates that cannot
  * beat best_len, so they are omitted without changing the result. */
@@ -939,8 +990,16 @@ static void* worker(void* arg) {
         pthread_mutex_unlock(&q->mu);
         if (i >= q->njobs) break;
         chunk_job* j = &q->jobs[i];
-        j->ntok = j->l6gpu ? oracle_l6_lz77_h(j->src - j->halo, j->halo, j->len, j->level, j->tokens)
-                           : oracle_lz77(j->src, j->len, j->level, j->tokens);
+        if (j->l6gpu) {
+            j->ntok = oracle_l6_lz77_h(j->src - j->halo, j->halo, j->len, j->level, j->tokens);
+        } else if (j->halo) {
+            const int bits = LEVELS[clamp_level(j->level)].hash_bits;
+            uint32_t* head = (uint32_t*)calloc((size_t)1 << bits, sizeof(uint32_t));
+            j->ntok = head ? lz77_fast_h(j->src - j->halo, (int)j->halo, (int)j->len, bits, head, j->tokens) : 0;
+            free(head);
+        } else {
+            j->ntok = oracle_lz77(j->src, j->len, j->level, j->tokens);
+        }
     }
     return NULL;
 }
@@ -952,11 +1011,18 @@ static size_t default_block_size(int level) {
 }
 
 static size_t parallel_compress_impl(int level, int format, int threads, size_t block_size, const uint8_t* src,
-                                     size_t n, uint8_t* dst, size_t cap, int l6gpu, int whole);
+                                     size_t n, uint8_t* dst, size_t cap, int l6gpu, int whole, int l1dict);
 
 size_t oracle_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src, size_t n,
                                 uint8_t* dst, size_t cap) {
-    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, 0, 0);
+    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, 0, 0, 0);
+}
+
+size_t oracle_l1_dict_compress(int level, int format, int threads, size_t block_size, const uint8_t* src, size_t n,
+                               uint8_t* dst, size_t cap) {
+    const int lv = clamp_level(level);
+    if (lv < 1 || lv > 3) return oracle_gpu_parallel_compress(level, format, threads, block_size, src, n, dst, cap);
+    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, 0, 0, 1);
 }
 
 /* The L6 class takes chunks of at most 1 MiB: a larger block runs as 1 MiB
@@ -967,7 +1033,7 @@ size_t oracle_gpu_parallel_compress(int level, int format, int threads, size_t b
                                     size_t n, uint8_t* dst, size_t cap) {
     const int l6 = clamp_level(level) >= 4;
     if (l6 && block_size > L6_MAX_CHUNK) block_size = L6_MAX_CHUNK;
-    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, l6, 0);
+    return parallel_compress_impl(level, format, threads, block_size, src, n, dst, cap, l6, 0, 0);
 }
 
 size_t oracle_whole_compress(int level, int format, const uint8_t* src, size_t n, uint8_t* dst, size_t cap) {
@@ -975,13 +1041,13 @@ size_t oracle_whole_compress(int level, int format, const uint8_t* src, size_t n
     if (cap < oracle_compress_bound(n, format)) return 0; /* raw_deflate.cpp:20, zlib/gzip wrappers */
     size_t r;
     if (lv >= 4) r = oracle_gpu_parallel_compress(lv, format, 1, 0, src, n, dst, cap);
-    else r = parallel_compress_impl(lv, format, 1, n ? n : 1, src, n, dst, cap, 0, 1);
+    else r = parallel_compress_impl(lv, format, 1, n ? n : 1, src, n, dst, cap, 0, 1, 0);
     if (r && format == 2) dst[8] = (uint8_t)(lv >= 9 ? 2 : (lv <= 2 ? 4 : 0)); /* gzip_wrapper.cpp:31 */
     return r;
 }
 
 static size_t parallel_compress_impl(int level, int format, int threads, size_t block_size, const uint8_t* src,
-                                     size_t n, uint8_t* dst, size_t cap, int l6gpu, int whole) {
+                                     size_t n, uint8_t* dst, size_t cap, int l6gpu, int whole, int l1dict) {
     const int lv = clamp_level(level);
     if (LEVELS[lv].bt4 && !l6gpu) return 0;
     if (block_size == 0) block_size = default_block_size(level);
@@ -994,7 +1060,7 @@ static size_t parallel_compress_impl(int level, int format, int threads, size_t 
         jobs[i].len = n ? (n - i * block_size < block_size ? n - i * block_size : block_size) : 0;
         jobs[i].level = level;
         jobs[i].l6gpu = l6gpu;
-        jobs[i].halo = l6gpu ? (i * block_size < L6_HALO ? i * block_size : L6_HALO) : 0;
+        jobs[i].halo = (l6gpu || l1dict) ? (i * block_size < L6_HALO ? i * block_size : L6_HALO) : 0;
         jobs[i].tokens = (uint32_t*)calloc(jobs[i].len ? jobs[i].len : 1, sizeof(uint32_t));
         if (!jobs[i].tokens) ok = 0;
     }

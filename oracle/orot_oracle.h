@@ -72,6 +72,17 @@ size_t oracle_l6_lz77_h(const uint8_t* src, size_t halo, size_t n, int level, ui
 size_t oracle_gpu_parallel_compress(int level, int format, int threads, size_t block_size, const uint8_t* src,
                                     size_t n, uint8_t* dst, size_t cap);
 
+/* Levels 1-3 in the HIP path's dictionary mode: each chunk after the first
+ * of its buffer sees the up to 32 KiB before it (its positions are in the
+ * hash table before the chunk's first probe; tokens cover the chunk only).
+ * Other levels: as oracle_gpu_parallel_compress. */
+size_t oracle_l1_dict_compress(int level, int format, int threads, size_t block_size, const uint8_t* src, size_t n,
+                               uint8_t* dst, size_t cap);
+
+/* The dictionary mode's token stream of one chunk (levels 1-3): src points
+ * at the halo bytes before the chunk (halo <= 32768), the chunk follows. */
+size_t oracle_l1_lz77_h(const uint8_t* src, size_t halo, size_t n, int level, uint32_t* tokens);
+
 /* orot's whole-buffer deflate_compress (src/api/deflate_api.cpp:100-136,
  * src/compress/block_compressor.cpp:16-73, the zlib and gzip wrappers):
  * 0 when cap is below the bound; levels 0-3: the whole input as one chunk,

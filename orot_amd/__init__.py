@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = (
     "deflate_crc32",
     "deflate_parallel_compress_device",
     "deflate_parallel_compress_device_noalloc",
+    "deflate_parallel_set_dictionary",
     "deflate_parallel_set_timing",
     "deflate_parallel_stage_times",
     "deflate_decompress_device",
@@ -107,7 +108,7 @@ class BatchConfig(ctypes.Structure):
     """deflate_batch_config (include/orot/deflate_batch.h)."""
 
     _fields_ = [("block_size", ctypes.c_size_t), ("hash_bits", ctypes.c_int), ("max_chain", ctypes.c_int),
-                ("nice_len", ctypes.c_int), ("lazy_depth", ctypes.c_int)]
+                ("nice_len", ctypes.c_int), ("lazy_depth", ctypes.c_int), ("dictionary", ctypes.c_int)]
 
 
 ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t)
@@ -184,6 +185,8 @@ def lib() -> ctypes.CDLL:
     L.deflate_parallel_compress.argtypes = [vp, vp, sz, vp, sz]
     L.deflate_parallel_compress_device.restype = sz
     L.deflate_parallel_compress_device.argtypes = [vp, vp, sz, vp, sz]
+    L.deflate_parallel_set_dictionary.restype = ctypes.c_int
+    L.deflate_parallel_set_dictionary.argtypes = [vp, ctypes.c_int]
     L.deflate_parallel_compress_device_noalloc.restype = sz
     L.deflate_parallel_compress_device_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, sz, vp, sz, vp, sz]
     L.deflate_parallel_set_timing.restype = None
@@ -421,6 +424,12 @@ class ParallelCompressor:
         """deflate_parallel_compress_device on device pointers."""
         return int(lib().deflate_parallel_compress_device(self._ctx, d_in, n, d_out, cap))
 
+    def set_dictionary(self, on: bool = True) -> None:
+        """deflate_parallel_set_dictionary: levels 1-3 match into the 32 KiB
+        before each chunk (not orot's bytes)."""
+        if lib().deflate_parallel_set_dictionary(self._ctx, 1 if on else 0) != 0:
+            raise Error(Result.PARAM_ERROR, f"deflate_parallel_set_dictionary: {last_error()}")
+
     def set_timing(self, on: bool = True) -> None:
         lib().deflate_parallel_set_timing(self._ctx, 1 if on else 0)
 
@@ -641,11 +650,11 @@ class BatchCompressor:
         if lib().deflate_batch_get_config(self._ctx, ctypes.byref(c)) != 0:
             raise Error(Result.PARAM_ERROR, last_error())
         return {"block_size": c.block_size, "hash_bits": c.hash_bits, "max_chain": c.max_chain,
-                "nice_len": c.nice_len, "lazy_depth": c.lazy_depth}
+                "nice_len": c.nice_len, "lazy_depth": c.lazy_depth, "dictionary": c.dictionary}
 
     def set_config(self, block_size: int, hash_bits: int = 0, max_chain: int = 0, nice_len: int = 0,
-                   lazy_depth: int = 0) -> None:
-        c = BatchConfig(block_size, hash_bits, max_chain, nice_len, lazy_depth)
+                   lazy_depth: int = 0, dictionary: int = 0) -> None:
+        c = BatchConfig(block_size, hash_bits, max_chain, nice_len, lazy_depth, dictionary)
         if lib().deflate_batch_set_config(self._ctx, ctypes.byref(c)) != 0:
             raise Error(Result.PARAM_ERROR, f"deflate_batch_set_config: {last_error()}")
 

@@ -291,6 +291,14 @@ size_t deflate_parallel_compress_device(deflate_parallel_ctx* ctx, const void* d
     return ctx->run(static_cast<const uint8_t*>(d_in), n, static_cast<uint8_t*>(d_out), cap);
 }
 
+int deflate_parallel_set_dictionary(deflate_parallel_ctx* ctx, int on) {
+    if (!ctx) return -1;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    EngineConfig c = ctx->eng.config();
+    c.l1_dict = on != 0;
+    return ctx->eng.set_config(c) ? 0 : -1;
+}
+
 void deflate_parallel_set_timing(deflate_parallel_ctx* ctx, int on) {
     if (!ctx) return;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -475,6 +483,7 @@ int deflate_batch_get_config(deflate_batch_ctx* ctx, deflate_batch_config* cfg) 
     cfg->max_chain = c.l6.max_chain;
     cfg->nice_len = c.l6.nice_len;
     cfg->lazy_depth = c.l6.lazy_depth;
+    cfg->dictionary = c.l1_dict ? 1 : 0;
     return 0;
 }
 
@@ -487,6 +496,7 @@ int deflate_batch_set_config(deflate_batch_ctx* ctx, const deflate_batch_config*
     c.l6.max_chain = cfg->max_chain;
     c.l6.nice_len = cfg->nice_len;
     c.l6.lazy_depth = cfg->lazy_depth;
+    c.l1_dict = cfg->dictionary != 0;
     if (!ctx->eng.set_config(c)) return -1;
     ctx->autotune_pending = false;
     return 0;

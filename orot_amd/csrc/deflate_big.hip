@@ -38,28 +38,42 @@ namespace orot_gpu {
 namespace {
 
 constexpr int BIG_THREADS = 1024;
+
+// Dictionary mode (ext = BIG_DICT): a chunk that is not the first of its
+// buffer sees the up to 32 KiB before it (a stream piece's first chunk: the
+// stream's bytes before the piece).  Positions then count from the
+// dictionary's first byte and per-position arrays are laid out with room for
+// it; tokens cover the chunk only.
+__device__ __forceinline__ uint32_t big_halo(uint32_t ext, uint32_t c, const ChunkDesc* desc, const BufDesc* bufs) {
+    if (!ext) return 0;
+    const BufDesc& B = bufs[desc[c].buf & CHUNK_BUF];
+    const uint64_t rel = desc[c].in_off - B.in_off + B.hist;
+    return (uint32_t)min<uint64_t>(rel, ext);
+}
 constexpr int BIG_WAVES = BIG_THREADS / 64;
 constexpr uint32_t BIG_RUN = 64;  // positions per thread of the match kernel
 
 
 template <int HB>
 __global__ __launch_bounds__(64) void big_cand_kernel(const uint8_t* __restrict__ src, const ChunkDesc* __restrict__ desc,
-                                                      const BatchHdr* __restrict__ hdr, uint32_t chunk,
-                                                      uint32_t* __restrict__ cand, uint32_t* __restrict__ segtab) {
+                                                      const BufDesc* __restrict__ bufs, const BatchHdr* __restrict__ hdr,
+                                                      uint32_t chunk, uint32_t ext, uint32_t* __restrict__ cand,
+                                                      uint32_t* __restrict__ segtab) {
     constexpr uint32_t TAB = 1u << HB;
     extern __shared__ uint32_t tab[];
     const int lane = threadIdx.x;
-    const uint32_t segs = big_segs(chunk);
+    const uint32_t segs = big_segs(chunk + ext);
     const uint64_t items = (uint64_t)hdr->nchunks * segs;
     for (uint64_t it = blockIdx.x; it < items; it += gridDim.x) {
         const uint32_t c = (uint32_t)(it / segs), sg = (uint32_t)(it % segs);
-        const uint32_t n = desc[c].len;
+        const uint32_t H = big_halo(ext, c, desc, bufs);
+        const uint32_t n = H + desc[c].len;       // positions of the dictionary and the chunk
         const uint32_t nh = n >= 4 ? n - 3 : 0;  // positions p with p + 4 <= n are hashed
         const uint32_t a = sg * BIG_SEG;
         if (a >= nh) continue;  // the carry kernel reads only segments below nh
         const uint32_t b = min(a + BIG_SEG, nh);
-        const uint8_t* s = src + desc[c].in_off;
-        uint32_t* Q = cand + (uint64_t)c * big_position_stride(chunk);
+        const uint8_t* s = src + desc[c].in_off - H;
+        uint32_t* Q = cand + (uint64_t)c * big_position_stride(chunk + ext);
         for (uint32_t i = lane; i < TAB; i += 64) tab[i] = 0;
         __syncthreads();
         uint32_t wnext = a + (uint32_t)lane < b ? ld32u(s + a + lane) : 0u;
@@ -95,15 +109,16 @@ __global__ __launch_bounds__(64) void big_cand_kernel(const uint8_t* __restrict_
 
 template <int HB>
 __global__ __launch_bounds__(256) void big_carry_kernel(const ChunkDesc* __restrict__ desc,
-                                                        const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                        const BufDesc* __restrict__ bufs,
+                                                        const BatchHdr* __restrict__ hdr, uint32_t chunk, uint32_t ext,
                                                         uint32_t* __restrict__ segtab) {
     constexpr uint32_t TAB = 1u << HB;
-    const uint32_t segs = big_segs(chunk);
+    const uint32_t segs = big_segs(chunk + ext);
     const uint64_t items = (uint64_t)hdr->nchunks * TAB;
     for (uint64_t it = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; it < items;
          it += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t c = (uint32_t)(it / TAB), h = (uint32_t)(it % TAB);
-        const uint32_t n = desc[c].len;
+        const uint32_t n = big_halo(ext, c, desc, bufs) + desc[c].len;
         const uint32_t nh = n >= 4 ? n - 3 : 0;
         const uint32_t used = (nh + BIG_SEG - 1) / BIG_SEG;
         uint32_t* col = segtab + (uint64_t)c * segs * TAB + h;
@@ -118,25 +133,27 @@ __global__ __launch_bounds__(256) void big_carry_kernel(const ChunkDesc* __restr
 
 template <int HB>
 __global__ __launch_bounds__(256) void big_match_kernel(const uint8_t* __restrict__ src, const ChunkDesc* __restrict__ desc,
-                                                        const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                        const BufDesc* __restrict__ bufs,
+                                                        const BatchHdr* __restrict__ hdr, uint32_t chunk, uint32_t ext,
                                                         const uint32_t* __restrict__ cand,
                                                         const uint32_t* __restrict__ carry,
                                                         uint32_t* __restrict__ tokw) {
     constexpr uint32_t TAB = 1u << HB;
-    const uint32_t segs = big_segs(chunk);
+    const uint32_t segs = big_segs(chunk + ext);
     const uint32_t runs = (chunk + BIG_RUN - 1) / BIG_RUN;
     const uint64_t items = (uint64_t)hdr->nchunks * runs;
     for (uint64_t it = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; it < items;
          it += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t c = (uint32_t)(it / runs), r = (uint32_t)(it % runs);
-        const uint32_t n = desc[c].len;
-        const uint32_t r0 = r * BIG_RUN;
+        const uint32_t H = big_halo(ext, c, desc, bufs);
+        const uint32_t n = H + desc[c].len;  // positions count from the dictionary's first byte
+        const uint32_t r0 = H + r * BIG_RUN;  // the chunk's positions only
         if (r0 >= n) continue;
         const uint32_t r1 = min(n, r0 + BIG_RUN);
         const uint32_t nh = n >= 4 ? n - 3 : 0;
         const uint32_t lim = min(n, r1 + 257u);  // compares reach past the run for its last tokens
-        const uint8_t* s = src + desc[c].in_off;
-        const uint64_t base = (uint64_t)c * big_position_stride(chunk);
+        const uint8_t* s = src + desc[c].in_off - H;
+        const uint64_t base = (uint64_t)c * big_position_stride(chunk + ext);
         const uint32_t* Q = cand + base;
         const uint32_t* carry_c = carry + (uint64_t)c * segs * TAB;
         uint32_t* T = tokw + base;
@@ -201,8 +218,9 @@ __global__ __launch_bounds__(256) void big_match_kernel(const uint8_t* __restric
 __device__ __forceinline__ bool bm_test(const uint32_t* bm, uint32_t p) { return (bm[p >> 5] >> (p & 31)) & 1u; }
 
 __global__ __launch_bounds__(BIG_THREADS) void big_parse_kernel(const ChunkDesc* __restrict__ desc,
+                                                                const BufDesc* __restrict__ bufs,
                                                                 const BatchHdr* __restrict__ hdr, uint32_t chunk,
-                                                                const uint32_t* __restrict__ tokw,
+                                                                uint32_t ext, const uint32_t* __restrict__ tokw,
                                                                 uint32_t* __restrict__ bitmap,
                                                                 uint32_t* __restrict__ stage, uint64_t stage_words,
                                                                 ChunkInfo* __restrict__ info, int whole) {
@@ -218,8 +236,8 @@ __global__ __launch_bounds__(BIG_THREADS) void big_parse_kernel(const ChunkDesc*
     const uint32_t nchunks = hdr->nchunks;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const uint32_t n = desc[c].len;
-        const uint64_t base = (uint64_t)c * big_position_stride(chunk);
-        const uint32_t* T = tokw + base;
+        // token words of the chunk's positions (after its dictionary's)
+        const uint32_t* T = tokw + (uint64_t)c * big_position_stride(chunk + ext) + big_halo(ext, c, desc, bufs);
         uint32_t* bm = bitmap + (uint64_t)c * big_bitmap_words(chunk);
         // segments of S positions (a multiple of 32: each owns whole bitmap words)
         const uint32_t S = max(32u, (((n + BIG_THREADS - 1) / BIG_THREADS) + 31u) & ~31u);
@@ -441,7 +459,8 @@ hipError_t set_kernel_attributes_big() {
 }
 
 hipError_t launch_l1_big(const Plan& p, int whole) {
-    const uint32_t segs = big_segs(p.chunk);
+    const uint32_t ext = p.l1_dict ? BIG_DICT : 0u;
+    const uint32_t segs = big_segs(p.chunk + ext);
     uint32_t* cand = p.d_aux;
     uint32_t* segtab = reinterpret_cast<uint32_t*>(p.d_blocks);
     uint32_t* bitmap = reinterpret_cast<uint32_t*>(p.d_pd);
@@ -453,22 +472,24 @@ hipError_t launch_l1_big(const Plan& p, int whole) {
     const unsigned gm = grid_for(p.persistent_grid, (uint32_t)std::min<uint64_t>((p.nchunks * runs + 255) / 256, wide), wide);
     const unsigned gp = grid_for(p.persistent_grid, p.nchunks, (uint32_t)p.cus);
     if (p.hash_bits == 12) {
-        hipLaunchKernelGGL(big_cand_kernel<12>, dim3(gc), dim3(64), 4u << 12, p.stream, p.d_in, p.d_desc, p.d_hdr,
-                           p.chunk, cand, segtab);
-        hipLaunchKernelGGL(big_carry_kernel<12>, dim3(gk), dim3(256), 0, p.stream, p.d_desc, p.d_hdr, p.chunk, segtab);
-        hipLaunchKernelGGL(big_match_kernel<12>, dim3(gm), dim3(256), 0, p.stream, p.d_in, p.d_desc, p.d_hdr, p.chunk,
-                           cand, segtab, p.d_tok);
+        hipLaunchKernelGGL(big_cand_kernel<12>, dim3(gc), dim3(64), 4u << 12, p.stream, p.d_in, p.d_desc, p.d_bufs,
+                           p.d_hdr, p.chunk, ext, cand, segtab);
+        hipLaunchKernelGGL(big_carry_kernel<12>, dim3(gk), dim3(256), 0, p.stream, p.d_desc, p.d_bufs, p.d_hdr, p.chunk,
+                           ext, segtab);
+        hipLaunchKernelGGL(big_match_kernel<12>, dim3(gm), dim3(256), 0, p.stream, p.d_in, p.d_desc, p.d_bufs, p.d_hdr,
+                           p.chunk, ext, cand, segtab, p.d_tok);
     } else if (p.hash_bits == 14) {
-        hipLaunchKernelGGL(big_cand_kernel<14>, dim3(gc), dim3(64), 4u << 14, p.stream, p.d_in, p.d_desc, p.d_hdr,
-                           p.chunk, cand, segtab);
-        hipLaunchKernelGGL(big_carry_kernel<14>, dim3(gk), dim3(256), 0, p.stream, p.d_desc, p.d_hdr, p.chunk, segtab);
-        hipLaunchKernelGGL(big_match_kernel<14>, dim3(gm), dim3(256), 0, p.stream, p.d_in, p.d_desc, p.d_hdr, p.chunk,
-                           cand, segtab, p.d_tok);
+        hipLaunchKernelGGL(big_cand_kernel<14>, dim3(gc), dim3(64), 4u << 14, p.stream, p.d_in, p.d_desc, p.d_bufs,
+                           p.d_hdr, p.chunk, ext, cand, segtab);
+        hipLaunchKernelGGL(big_carry_kernel<14>, dim3(gk), dim3(256), 0, p.stream, p.d_desc, p.d_bufs, p.d_hdr, p.chunk,
+                           ext, segtab);
+        hipLaunchKernelGGL(big_match_kernel<14>, dim3(gm), dim3(256), 0, p.stream, p.d_in, p.d_desc, p.d_bufs, p.d_hdr,
+                           p.chunk, ext, cand, segtab, p.d_tok);
     } else {
         return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(big_parse_kernel, dim3(gp), dim3(BIG_THREADS), 0, p.stream, p.d_desc, p.d_hdr, p.chunk,
-                       p.d_tok, bitmap, p.d_stage, p.stage_words, p.d_info, whole);
+    hipLaunchKernelGGL(big_parse_kernel, dim3(gp), dim3(BIG_THREADS), 0, p.stream, p.d_desc, p.d_bufs, p.d_hdr, p.chunk,
+                       ext, p.d_tok, bitmap, p.d_stage, p.stage_words, p.d_info, whole);
     return hipGetLastError();
 }
 

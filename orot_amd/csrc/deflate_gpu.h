@@ -119,6 +119,7 @@ struct Plan {
     hipStream_t stream;
     hipEvent_t ev[8];       // optional timing events (nullptr = off)
     int whole;              // levels 1-3: one chunk per buffer with orot's whole-buffer rules (all-literal -> stored)
+    int l1_dict;            // levels 1-3: each chunk sees the 32 KiB before it (big-chunk path, arrays with BIG_DICT room)
     bool exch_ok;           // LDS exchanges of one wave apply in lane order
     size_t lds_max;         // largest dynamic LDS allocation per workgroup
     int cus;                // compute units
@@ -139,6 +140,7 @@ uint32_t max_chunk_for_hash_bits(int hash_bits);
 // segment tables (d_blocks: big_segtab_words 32-bit words per chunk).
 constexpr uint32_t MAX_CHUNK_BIG = 1u << 30;
 constexpr uint32_t BIG_SEG = 65536;  // positions per candidate segment
+constexpr uint32_t BIG_DICT = 32768;  // dictionary room per chunk in dictionary mode
 bool l1_big_chunk(uint32_t chunk, int hash_bits);
 __host__ __device__ constexpr uint64_t big_position_stride(uint32_t chunk) { return ((uint64_t)chunk + 7u) & ~7ull; }
 __host__ __device__ constexpr uint64_t big_bitmap_words(uint32_t chunk) { return ((uint64_t)chunk + 31u) / 32u + 1u; }

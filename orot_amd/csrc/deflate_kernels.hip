@@ -1669,7 +1669,7 @@ hipError_t launch_wide(const Plan& p) {
 }
 
 hipError_t launch_l1_class(const Plan& p) {
-    if (p.whole || l1_big_chunk(p.chunk, p.hash_bits)) return launch_l1_big(p, p.whole);
+    if (p.whole || p.l1_dict || l1_big_chunk(p.chunk, p.hash_bits)) return launch_l1_big(p, p.whole);
     if (p.hash_bits == 12) {
         if (p.chunk <= LC_MAX_CHUNK) return p.exch_ok ? launch_barrier<12, true>(p) : launch_barrier<12, false>(p);
         if (p.chunk <= MAX_CHUNK_L1) return launch_wide<12>(p);

@@ -184,9 +184,10 @@ void Engine::drop_graph() {
 bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     const bool l6 = cfg_.level >= 4;
     const bool l1 = cfg_.level >= 1;
-    const bool big = l1 && !l6 && (cfg_.whole || l1_big_chunk(cfg_.chunk, cfg_.hash_bits));
+    const bool dict = l1 && !l6 && cfg_.l1_dict;
+    const bool big = l1 && !l6 && (cfg_.whole || dict || l1_big_chunk(cfg_.chunk, cfg_.hash_bits));
     if (nchunks <= chunk_cap_ && nbufs <= buf_cap_ && arena_chunk_ == cfg_.chunk && (!l6 || arena_l6_) &&
-        (!l1 || d_tok_) && (!big || arena_big_))
+        (!l1 || d_tok_) && (!big || arena_big_) && (!dict || arena_dict_))
         return true;
     // grow by at least half again, so a stream of growing batches settles quickly
     const uint32_t ncap = std::max(nchunks, (arena_chunk_ == cfg_.chunk ? chunk_cap_ + chunk_cap_ / 2 : 0u));
@@ -209,8 +210,10 @@ bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     h_res_ = nullptr;
     chunk_cap_ = buf_cap_ = 0;
     const uint32_t chunk = cfg_.chunk;
+    const uint32_t ext = dict ? BIG_DICT : 0u;  // dictionary room per chunk
     const size_t per_pos =
-        (size_t)ncap * (l6 ? l6_position_stride(chunk) : (big ? big_position_stride(chunk) : l1_position_stride(chunk))) +
+        (size_t)ncap *
+            (l6 ? l6_position_stride(chunk) : (big ? big_position_stride(chunk + ext) : l1_position_stride(chunk))) +
         8;
     bool ok = true;
     if (l1) {
@@ -222,7 +225,7 @@ bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     }
     if (big) {
         ok = ok && dev_alloc(d_aux_, per_pos, "hipMalloc(candidates)") &&
-             dev_alloc(d_blocks_, (size_t)ncap * big_segtab_words(chunk, cfg_.hash_bits) * 4,
+             dev_alloc(d_blocks_, (size_t)ncap * big_segtab_words(chunk + ext, cfg_.hash_bits) * 4,
                        "hipMalloc(segment tables)");
     }
     if (l6) {
@@ -247,6 +250,7 @@ bool Engine::ensure_arenas(uint32_t nchunks, uint32_t nbufs) {
     arena_chunk_ = chunk;
     arena_l6_ = l6;
     arena_big_ = big;
+    arena_dict_ = dict;
     ++gen_;
     return true;
 }
@@ -308,6 +312,7 @@ Plan Engine::make_plan(const uint8_t* d_in, uint8_t* d_out, uint32_t nchunks, ui
     p.l6 = cfg_.l6;
     p.format = cfg_.format;
     p.whole = cfg_.whole ? 1 : 0;
+    p.l1_dict = (cfg_.l1_dict && cfg_.level >= 1 && cfg_.level <= 3) ? 1 : 0;
     p.d_tok = d_tok_;
     p.d_pd = d_pd_;
     p.d_aux = d_aux_;

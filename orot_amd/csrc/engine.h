@@ -31,6 +31,7 @@ struct EngineConfig {
     int hash_bits = 0;    // L1 class: 12 or 14 (0: the level's own)
     L6Params l6{};        // L6 class (max_chain 0: the level's own)
     bool whole = false;   // levels 0-3: orot's whole-buffer rules (chunk must cover each buffer)
+    bool l1_dict = false; // levels 1-3: each chunk sees the 32 KiB before it
 };
 
 // Fills the zero fields of cfg with the level's defaults and checks the
@@ -102,6 +103,7 @@ private:
     uint32_t arena_chunk_ = 0;     // chunk size the arenas were sized for
     bool arena_l6_ = false;
     bool arena_big_ = false;
+    bool arena_dict_ = false;
     uint64_t gen_ = 0;             // bumped when any arena moves
 
     // last submit

@@ -245,6 +245,7 @@ size_t deflate_parallel_compress_device_noalloc(int level, deflate_format format
     p.d_blkpos = cfg.level >= 4 ? reinterpret_cast<uint32_t*>(base + L.blkpos) : nullptr;
     p.d_blocks = (cfg.level >= 4 || big) ? static_cast<void*>(base + L.blocks) : nullptr;
     p.whole = 0;
+    p.l1_dict = 0;
     p.stage_words = stage_words_for_chunk(cfg.chunk);
     p.d_info = reinterpret_cast<ChunkInfo*>(base + L.info);
     p.d_result = reinterpret_cast<Result*>(base + L.res);

@@ -49,6 +49,10 @@ def oracle() -> ctypes.CDLL:
         L.oracle_l6_match_h.argtypes = [_vp, _sz, _sz, ctypes.c_int, _vp, _vp]
         L.oracle_l6_lz77_h.restype = _sz
         L.oracle_l6_lz77_h.argtypes = [_vp, _sz, _sz, ctypes.c_int, _vp]
+        L.oracle_l1_lz77_h.restype = _sz
+        L.oracle_l1_lz77_h.argtypes = [_vp, _sz, _sz, ctypes.c_int, _vp]
+        L.oracle_l1_dict_compress.restype = _sz
+        L.oracle_l1_dict_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _sz, _vp, _sz, _vp, _sz]
         L.oracle_whole_compress.restype = _sz
         L.oracle_whole_compress.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _sz, _vp, _sz]
         L.oracle_encode_blocks_dyn.restype = None
@@ -133,6 +137,26 @@ def gpu_parallel_compress(data, level: int, fmt: int, block_size: int = 0, threa
                                               out.ctypes.data, cap)
     if n == 0:
         raise RuntimeError("oracle_gpu_parallel_compress failed")
+    return out[:n].tobytes()
+
+
+def l1_lz77_h(buf, halo: int, level: int) -> np.ndarray:
+    a = _u8(buf)
+    n = a.size - halo
+    t = np.zeros(max(1, n), dtype=np.uint32)
+    k = oracle().oracle_l1_lz77_h(a.ctypes.data, halo, n, level, t.ctypes.data)
+    return t[:k]
+
+
+def l1_dict_compress(data, level: int, fmt: int, block_size: int = 0, threads: int = 1) -> bytes:
+    """Levels 1-3 in dictionary mode (each chunk sees the 32 KiB before it)."""
+    a = _u8(data)
+    cap = a.size + a.size // 8 + 4096
+    out = np.zeros(cap, dtype=np.uint8)
+    n = oracle().oracle_l1_dict_compress(level, fmt, threads, block_size, a.ctypes.data, a.size, out.ctypes.data,
+                                         cap)
+    if n == 0:
+        raise RuntimeError("oracle_l1_dict_compress failed")
     return out[:n].tobytes()
 
 

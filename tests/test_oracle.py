@@ -610,3 +610,53 @@ def test_whole_buffer_closed_forms():
     assert orc.whole_compress(b"", 1, 0) == bytes.fromhex("010000ffff")
     assert orc.whole_compress(b"a", 1, 0) == bytes.fromhex("010100feff61")
     assert orc.whole_compress(b"a", 6, 0) == bytes.fromhex("4b0400")  # L6 class: the chunked pipeline
+
+
+# ---- the L1-class dictionary mode (the HIP path's own, off by default): its
+# token stream against a brute force of its definition (every dictionary and
+# chunk position before p is in the table; orot's fast-path rules otherwise)
+def _greedy_from(buf, cand, start):
+    n = len(buf)
+    toks = []
+    p = start
+    while p < n:
+        l = 0
+        if p + 4 <= n and cand[p]:
+            m = min(258, n - p)
+            d = cand[p]
+            while l < m and buf[p + l] == buf[p - d + l]:
+                l += 1
+        if l >= 3:
+            toks.append(0x80000000 | (l << 16) | cand[p])
+            p += l
+        else:
+            toks.append(buf[p])
+            p += 1
+    return toks
+
+
+@pytest.mark.parametrize("level,bits", [(1, 12), (3, 14)])
+@pytest.mark.parametrize("kind", corpus.GENERATORS)
+@pytest.mark.parametrize("halo", [0, 5, 4000, 32768])
+def test_l1_dict_tokens_bruteforce(level, bits, kind, halo):
+    dic = corpus.generate("text", halo, 3).tobytes()
+    body = corpus.generate(kind, 30_000, 4).tobytes()
+    buf = dic + body[:1000] + dic[-min(halo, 2000):] + body[1000:]
+    got = [int(t) for t in orc.l1_lz77_h(buf, halo, level)]
+    assert got == _greedy_from(buf, _brute_candidates(buf, bits), halo)
+    if halo == 0:
+        assert got == [int(t) for t in orc.lz77(buf, level)]
+
+
+def test_l1_dict_mode_stream():
+    """Dictionary mode: every chunk after the first starts with the 32 KiB
+    before it; the stream inflates, is smaller than orot's at small blocks,
+    and equals orot's bytes for a single chunk."""
+    data = np.concatenate([corpus.generate(k, 400_000, 5 + i) for i, k in enumerate(corpus.GENERATORS)])
+    for level in (1, 2):
+        d = orc.l1_dict_compress(data, level, 2, 65536, threads=4)
+        p = orc.parallel_compress(data, level, 2, 65536, threads=4)
+        assert zlib.decompress(d, 31) == data.tobytes()
+        assert len(d) < len(p)
+        one = data[:60_000]
+        assert orc.l1_dict_compress(one, level, 1, 65536) == orc.parallel_compress(one, level, 1, 65536)
