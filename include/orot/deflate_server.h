@@ -2,12 +2,16 @@
  * orot/deflate_server.h -- a compression server on one MI355X.
  *
  * No counterpart in orot.  Requests arrive one at a time (submit returns at
- * once); a worker thread admits them while their buffers fit an HBM budget,
- * batches whatever has been admitted whenever a lane (one HIP stream and its
- * pipeline, per level and format) is idle, and keeps up to `lanes_per_level`
- * batches per level on the device at once.  Requests that arrive while others
- * run join the next batch of their level: continuous batching.  Each output
- * is the bytes deflate_parallel_compress gives for that input alone.
+ * once).  The queues (one ring per level and format), the pool of
+ * `hbm_budget` bytes with its allocator, and the admission counters live in
+ * device memory: whenever a lane (one HIP stream and its pipeline, per level
+ * and format) is idle, an admission kernel takes the longest run of that
+ * queue's oldest requests that fits the pool and the batch limits, places
+ * their buffers in the pool and writes the lane's batch tables; the pipeline
+ * then runs on the lane's stream.  Up to `lanes_per_level` batches per level
+ * and format are on the device at once; requests that arrive while others
+ * run join a later batch of their level: continuous batching.  Each output is
+ * the bytes deflate_parallel_compress gives for that input alone.
  */
 #ifndef OROT_DEFLATE_SERVER_H
 #define OROT_DEFLATE_SERVER_H
@@ -25,6 +29,8 @@ typedef struct deflate_server_config {
     size_t max_batch_bytes;    /* input bytes per batch; 0: 256 MiB */
     int max_batch_requests;    /* 0: 4096 */
     int lanes_per_level;       /* batches of one level in flight at once; 0: 2 */
+    size_t lane_arena_bytes;   /* device working memory of one lane; it bounds the
+                                  blocks of one batch (and of one request); 0: 2 GiB */
 } deflate_server_config;
 
 /* Completion record. */
@@ -47,11 +53,13 @@ void deflate_server_free(deflate_server* srv);
 
 /* Queue one request.  d_in: input already in device memory (it must stay
  * valid until the request completes).  Returns its id (> 0), or 0 when the
- * request can never fit the budget or the arguments are invalid. */
+ * request can never fit (the budget, or a lane's blocks) or the arguments are
+ * invalid. */
 uint64_t deflate_server_submit_device(deflate_server* srv, const void* d_in, size_t in_size, int level,
                                       deflate_format format);
-/* Same with host input: copied into the server's pool when admitted (the
- * host buffer must stay valid until the request completes). */
+/* Same with host input: copied into the server's pool when admitted, where
+ * the copy stays until release (the host buffer must stay valid until the
+ * request completes). */
 uint64_t deflate_server_submit(deflate_server* srv, const void* in, size_t in_size, int level,
                                deflate_format format);
 
@@ -67,8 +75,9 @@ size_t deflate_server_read(deflate_server* srv, uint64_t id, void* out, size_t o
 int deflate_server_release(deflate_server* srv, uint64_t id);
 
 /* Counters: [0] submitted, [1] completed, [2] queued (not yet admitted),
- * [3] in flight, [4] pool bytes in use, [5] batches launched,
- * [6] largest queue seen.  Returns the number written. */
+ * [3] in flight, [4] pool bytes in use (read from the device allocator after
+ * the releases so far are applied), [5] batches launched, [6] largest queue
+ * seen.  Returns the number written. */
 int deflate_server_stats(deflate_server* srv, uint64_t* values, int max_values);
 
 #ifdef __cplusplus

@@ -93,7 +93,8 @@ class ServerConfig(ctypes.Structure):
     """deflate_server_config (include/orot/deflate_server.h)."""
 
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_batch_bytes", ctypes.c_size_t),
-                ("max_batch_requests", ctypes.c_int), ("lanes_per_level", ctypes.c_int)]
+                ("max_batch_requests", ctypes.c_int), ("lanes_per_level", ctypes.c_int),
+                ("lane_arena_bytes", ctypes.c_size_t)]
 
 
 class Completion(ctypes.Structure):
@@ -693,8 +694,8 @@ class Server:
     """Continuous-batching compression server (include/orot/deflate_server.h)."""
 
     def __init__(self, hbm_budget: int = 0, max_batch_bytes: int = 0, max_batch_requests: int = 0,
-                 lanes_per_level: int = 0):
-        cfg = ServerConfig(hbm_budget, max_batch_bytes, max_batch_requests, lanes_per_level)
+                 lanes_per_level: int = 0, lane_arena_bytes: int = 0):
+        cfg = ServerConfig(hbm_budget, max_batch_bytes, max_batch_requests, lanes_per_level, lane_arena_bytes)
         self._srv = lib().deflate_server_new(ctypes.byref(cfg))
         if not self._srv:
             raise Error(Result.MEM_ERROR, f"Server: {last_error()}")

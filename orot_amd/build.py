@@ -28,7 +28,7 @@ COMMON = [
 
 SOURCES = ["deflate_kernels.hip", "deflate_l6.hip", "inflate.hip", "engine.cpp", "deflate_api.cpp",
            "inflate_api.cpp", "server.cpp", "stream_api.cpp", "noalloc.hip", "inflate_host.cpp",
-           "deflate_big.hip"]
+           "deflate_big.hip", "server_kernels.hip"]
 
 
 def _run(cmd, verbose):

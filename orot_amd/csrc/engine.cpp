@@ -428,6 +428,55 @@ bool Engine::submit(const uint8_t* d_in, uint8_t* d_out, const BufferSpec* bufs,
     return true;
 }
 
+uint64_t Engine::arena_bytes_per_chunk(const EngineConfig& c) {
+    const bool l6 = c.level >= 4;
+    const bool l1 = c.level >= 1;
+    const bool dict = l1 && !l6 && c.l1_dict;
+    const bool big = l1 && !l6 && (c.whole || dict || l1_big_chunk(c.chunk, c.hash_bits));
+    const uint32_t ext = dict ? BIG_DICT : 0u;
+    const uint64_t stride =
+        l6 ? l6_position_stride(c.chunk) : (big ? big_position_stride(c.chunk + ext) : l1_position_stride(c.chunk));
+    uint64_t b = sizeof(ChunkInfo) + sizeof(ChunkDesc) + 4;
+    if (l1) {
+        b += 4 * stride + 2 * (big ? std::max<uint64_t>(stride, 2 * big_bitmap_words(c.chunk)) : stride) +
+             4 * stage_words_for_chunk(c.chunk);
+    }
+    if (big) b += 4 * stride + 4 * big_segtab_words(c.chunk + ext, c.hash_bits);
+    if (l6) {
+        const uint64_t mb = l6_max_blocks(c.chunk);
+        b += 4 * stride + 4 + 4 * mb + mb * l6_block_record_bytes();
+    }
+    return b;
+}
+
+bool Engine::prepare_device_tables(uint32_t chunk_cap, uint32_t buf_cap, DeviceTables* out) {
+    if (!ensure_arenas(chunk_cap, buf_cap)) return false;
+    out->hdr = reinterpret_cast<BatchHdr*>(d_tables_);
+    out->bufs = reinterpret_cast<BufDesc*>(d_tables_ + HDR_BYTES);
+    out->desc = reinterpret_cast<ChunkDesc*>(d_tables_ + desc_offset(buf_cap_));
+    out->order = reinterpret_cast<uint32_t*>(d_tables_ + order_offset(buf_cap_, chunk_cap_));
+    out->results = d_res_;
+    return true;
+}
+
+bool Engine::launch_device_tables(const uint8_t* d_in, uint8_t* d_out, uint32_t nchunks, uint32_t nbufs) {
+    last_nbufs_ = 0;
+    timed_ = false;
+    if (nchunks > chunk_cap_ || nbufs > buf_cap_) {
+        set_error_msg("device-built batch exceeds the arenas");
+        return false;
+    }
+    if (nbufs == 0) return true;
+    hipError_t e = launch_parallel_compress(make_plan(d_in, d_out, nchunks, nbufs, false));
+    if (e == hipSuccess) e = hipEventRecord(done_, stream_);
+    if (e != hipSuccess) {
+        set_error("launch (device tables)", e);
+        return false;
+    }
+    last_graph_ = false;
+    return true;
+}
+
 bool Engine::wait(uint64_t* sizes) {
     hipError_t e = hipEventSynchronize(done_);
     if (e != hipSuccess) {

@@ -72,6 +72,23 @@ public:
     hipStream_t stream() const { return stream_; }
     int compute_units() const { return cus_; }
 
+    // Batches whose tables a kernel writes in device memory (the server's
+    // admission).  prepare_device_tables sizes the arenas for exactly
+    // chunk_cap chunks and buf_cap buffers and says where the tables go;
+    // launch_device_tables enqueues the pipeline over them, nchunks / nbufs
+    // being the counts the tables hold (they size the grids).
+    struct DeviceTables {
+        BatchHdr* hdr;
+        BufDesc* bufs;
+        ChunkDesc* desc;
+        uint32_t* order;
+        const Result* results;
+    };
+    bool prepare_device_tables(uint32_t chunk_cap, uint32_t buf_cap, DeviceTables* out);
+    bool launch_device_tables(const uint8_t* d_in, uint8_t* d_out, uint32_t nchunks, uint32_t nbufs);
+    // Device bytes the arenas take per chunk of a resolved configuration.
+    static uint64_t arena_bytes_per_chunk(const EngineConfig& cfg);
+
 private:
     bool ensure_arenas(uint32_t nchunks, uint32_t nbufs);
     bool build_tables(const BufferSpec* bufs, size_t count, uint32_t& nchunks);

@@ -1,23 +1,33 @@
 // server.cpp -- continuous-batching compression server (deflate_server.h).
 //
-// One device pool of hbm_budget bytes holds every admitted request's output
-// (and its input, for host submissions).  A worker thread repeats:
-//   retire  -- lanes whose batch finished hand their requests to the
-//              completion queue (sizes come from the device-side results);
-//   launch  -- every idle lane takes the requests of its (level, format)
-//              queue that fit the pool, in arrival order, up to the batch
-//              limits, and enqueues them as one batch on its own stream.
-// Lanes of different levels, and several lanes of one level, run on the GPU
-// at once.
+// The queues, the pool allocator and the admission live in device memory
+// (server_dev.h, server_kernels.hip).  API threads only append to the
+// worker's inboxes (submit, release) and read completed records; the worker
+// thread, which holds the server's mutex only to exchange those lists,
+// repeats:
+//   push   -- new requests go to their queue's ring in device memory and new
+//             releases to the release ring (copies from pinned staging on
+//             the control stream);
+//   admit  -- for each queue with requests and an idle lane the admission
+//             kernel takes the longest run of its oldest requests that fits
+//             the pool and the batch limits and writes the lane's tables;
+//   launch -- host inputs are copied to the pool offsets the admission chose,
+//             then the lane's pipeline and completion kernel run on the
+//             lane's own stream;
+//   retire -- lanes whose batch finished hand their requests to the
+//             completion queue.
+// Lanes of different levels and formats, and several lanes of one, run on
+// the GPU at once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
-#include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -25,92 +35,85 @@
 #include "../../include/orot/deflate_server.h"
 #include "engine.h"
 #include "host_common.h"
+#include "server_dev.h"
 
-using orot_gpu::BufferSpec;
-using orot_gpu::Engine;
-using orot_gpu::EngineConfig;
+using namespace orot_gpu;
 
 namespace {
 
-constexpr uint64_t ALIGN = 256;
+constexpr uint32_t RING = 8192;          // requests per queue ring
+constexpr uint32_t STAGE_SLOTS = RING;   // new requests pushed per round
+constexpr int NKEYS = 13 * 4;            // level * 4 + format
 
-uint64_t align_up(uint64_t v) { return (v + ALIGN - 1) & ~(ALIGN - 1); }
-
-// First-fit allocator over [0, size).
-class Pool {
-public:
-    void reset(uint64_t size) {
-        free_.clear();
-        if (size) free_[0] = size;
-        used_ = 0;
-    }
-    bool alloc(uint64_t n, uint64_t& off) {
-        n = align_up(std::max<uint64_t>(n, 1));
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < n) continue;
-            off = it->first;
-            const uint64_t rest = it->second - n;
-            free_.erase(it);
-            if (rest) free_[off + n] = rest;
-            used_ += n;
-            return true;
-        }
-        return false;
-    }
-    void release(uint64_t off, uint64_t n) {
-        n = align_up(std::max<uint64_t>(n, 1));
-        used_ -= n;
-        auto it = free_.emplace(off, n).first;
-        auto next = std::next(it);
-        if (next != free_.end() && it->first + it->second == next->first) {
-            it->second += next->second;
-            free_.erase(next);
-        }
-        if (it != free_.begin()) {
-            auto prev = std::prev(it);
-            if (prev->first + prev->second == it->first) {
-                prev->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-    uint64_t used() const { return used_; }
-
-private:
-    std::map<uint64_t, uint64_t> free_;
-    uint64_t used_ = 0;
-};
+uint64_t align_up(uint64_t v) { return (v + SRV_ALIGN - 1) & ~(SRV_ALIGN - 1); }
 
 struct Request {
     uint64_t id = 0;
     int level = 1;
     int format = 0;
-    const uint8_t* d_in = nullptr;  // device input (caller's, or inside the pool)
+    const uint8_t* d_in = nullptr;  // device input (caller's)
     const void* h_in = nullptr;     // host input, copied at admission
     uint64_t in_size = 0;
-    uint64_t in_off = 0;            // pool offset of a copied host input
-    bool in_pool = false;
-    uint64_t out_off = 0;
     uint64_t cap = 0;
+    uint64_t need = 0;              // pool bytes
+    uint64_t out_off = 0;
+    uint64_t in_off = 0;
+    uint32_t rec = 0;               // device allocation record
     uint64_t out_size = 0;
     int status = 0;
     bool done = false;
     uint64_t t_submit = 0, t_launch = 0, t_done = 0;
 };
 
+// Device address of pinned host memory the kernels write.
+template <class T>
+T* device_view(T* h) {
+    void* d = nullptr;
+    return hipHostGetDevicePointer(&d, h, 0) == hipSuccess ? static_cast<T*>(d) : h;
+}
+
+template <class T>
+bool pinned_alloc(T*& p, size_t n) {
+    p = nullptr;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(n, 1) * sizeof(T), 0);
+    if (e != hipSuccess) {
+        p = nullptr;
+        set_error("hipHostMalloc(server)", e);
+        return false;
+    }
+    return true;
+}
+
 struct Lane {
     Engine eng;
-    int level = 1;
-    int format = 0;
+    Engine::DeviceTables tabs{};
+    LaneHead* h_head = nullptr;   // pinned: written by the admission kernel
+    LaneItem* h_items = nullptr;  // pinned: admission, then completion
+    hipEvent_t ev_done = nullptr;
     bool busy = false;
     std::vector<Request*> reqs;
-    std::vector<uint64_t> sizes;
+    ~Lane() {
+        if (eng.stream()) (void)hipStreamSynchronize(eng.stream());
+        if (ev_done) (void)hipEventDestroy(ev_done);
+        if (h_head) (void)hipHostFree(h_head);
+        if (h_items) (void)hipHostFree(h_items);
+    }
 };
 
-size_t default_chunk(int level) { return level <= 3 ? 131072 : (level <= 6 ? 524288 : 1048576); }
+struct Key {
+    EngineConfig cfg{};          // resolved
+    uint32_t chunk_cap = 0;      // chunks one batch (and one request) may have
+    uint32_t buf_cap = 0;        // requests one batch may have
+    SrvSlot* d_ring = nullptr;
+    uint64_t pushed = 0;         // requests copied to the device ring
+    uint64_t seen = 0;           // requests the host has seen admitted
+    bool stalled = false;        // the last admission took nothing: wait for a release
+    std::deque<Request*> backlog;  // submitted, not yet in the ring
+    std::vector<Request*> by_seq;  // the request at each ring position
+    std::vector<std::unique_ptr<Lane>> lanes;
+};
 
-uint64_t out_capacity(uint64_t n, int level, int format) {
-    const uint64_t chunk = default_chunk(level);
+uint64_t out_capacity(uint64_t n, uint32_t chunk, int format) {
     const uint64_t nchunks = n ? (n + chunk - 1) / chunk : 1;
     const uint64_t bound = deflate_compress_bound(n, (deflate_format)format) + 64;
     return std::max(bound, n + 6 * (nchunks + n / 65535 + 2) + 64);
@@ -120,19 +123,38 @@ uint64_t out_capacity(uint64_t n, int level, int format) {
 
 struct deflate_server {
     deflate_server_config cfg{};
+    int device = 0;
     uint8_t* d_pool = nullptr;
-    Pool pool;
+    uint8_t* d_state = nullptr;   // SrvPool | SrvRec[SRV_RECS] | rels[SRV_RELS] | heads[NKEYS]
+    SrvPool* d_pool_state = nullptr;
+    SrvRec* d_recs = nullptr;
+    uint32_t* d_rels = nullptr;
+    uint64_t* d_heads = nullptr;
+    hipStream_t control = nullptr;
+    // pinned staging of pushes, double-buffered
+    SrvSlot* h_slots[2] = {};
+    uint32_t* h_rels[2] = {};
+    hipEvent_t ev_stage[2] = {};
+    int stage_cur = 0;
+    LaneHead* h_stats = nullptr;
+    uint32_t rel_tail = 0;
+    Key keys[NKEYS];
+
+    // shared with the API threads (mu)
     std::mutex mu;
     std::condition_variable cv_work;
     std::condition_variable cv_done;
-    std::map<int, std::deque<Request*>> queues;  // key = level * 4 + format
-    std::map<int, std::vector<std::unique_ptr<Lane>>> lanes;
+    std::vector<Request*> inbox;
+    std::vector<uint32_t> rel_inbox;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> live;
     std::deque<Request*> completed;
     uint64_t next_id = 1;
-    uint64_t n_submitted = 0, n_completed = 0, n_queued = 0, n_inflight = 0, n_batches = 0, max_queue = 0;
+    uint64_t n_submitted = 0, n_completed = 0, n_inflight = 0, n_admitted = 0, n_batches = 0, max_queue = 0;
+    uint64_t pool_in_use = 0;
+    uint64_t stats_req = 0, stats_done = 0;
     bool stop = false;
-    uint64_t wake = 0;  // bumped by submit / release / stop: the worker's reason to look again
+    bool worker_exited = false;
+    uint64_t wake = 0;  // bumped by submit / release / stats / stop
     std::thread worker;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
@@ -141,171 +163,383 @@ struct deflate_server {
             .count();
     }
 
-    // retire finished batches (mu held)
+    ~deflate_server() {
+        for (auto& k : keys) {
+            k.lanes.clear();
+            if (k.d_ring) (void)hipFree(k.d_ring);
+        }
+        if (control) (void)hipStreamSynchronize(control);
+        for (int i = 0; i < 2; ++i) {
+            if (ev_stage[i]) (void)hipEventDestroy(ev_stage[i]);
+            if (h_slots[i]) (void)hipHostFree(h_slots[i]);
+            if (h_rels[i]) (void)hipHostFree(h_rels[i]);
+        }
+        if (h_stats) (void)hipHostFree(h_stats);
+        if (control) (void)hipStreamDestroy(control);
+        if (d_state) (void)hipFree(d_state);
+        if (d_pool) (void)hipFree(d_pool);
+    }
+
+    // Device state, staging and the per-queue limits (caller's thread).
+    bool init() {
+        for (int key = 0; key < NKEYS; ++key) {
+            Key& k = keys[key];
+            k.cfg.level = key / 4;
+            k.cfg.format = key % 4;
+            if (k.cfg.format == 3 || !resolve_config(k.cfg)) continue;
+            k.buf_cap = (uint32_t)std::min<uint64_t>({(uint64_t)cfg.max_batch_requests, SRV_MAX_BATCH, RING});
+            const uint64_t per = Engine::arena_bytes_per_chunk(k.cfg);
+            const uint64_t by_mem = std::max<uint64_t>(cfg.lane_arena_bytes / per, 1);
+            const uint64_t by_batch = (cfg.max_batch_bytes + k.cfg.chunk - 1) / k.cfg.chunk + k.buf_cap;
+            k.chunk_cap = (uint32_t)std::min<uint64_t>({by_mem, by_batch, 0x3FFFFFFFull});
+        }
+        const size_t state_bytes = sizeof(SrvPool) + (size_t)SRV_RECS * sizeof(SrvRec) + (size_t)SRV_RELS * 4 +
+                                   (size_t)NKEYS * 8;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_pool), cfg.hbm_budget);
+        if (e != hipSuccess) {
+            d_pool = nullptr;
+            set_error("hipMalloc(server pool)", e);
+            return false;
+        }
+        e = hipMalloc(reinterpret_cast<void**>(&d_state), state_bytes);
+        if (e != hipSuccess) {
+            d_state = nullptr;
+            set_error("hipMalloc(server state)", e);
+            return false;
+        }
+        d_pool_state = reinterpret_cast<SrvPool*>(d_state);
+        d_recs = reinterpret_cast<SrvRec*>(d_state + sizeof(SrvPool));
+        d_rels = reinterpret_cast<uint32_t*>(d_state + sizeof(SrvPool) + (size_t)SRV_RECS * sizeof(SrvRec));
+        d_heads = reinterpret_cast<uint64_t*>(d_state + sizeof(SrvPool) + (size_t)SRV_RECS * sizeof(SrvRec) +
+                                              (size_t)SRV_RELS * 4);
+        SrvPool init{};
+        init.budget = cfg.hbm_budget;
+        e = hipMemset(d_state, 0, state_bytes);
+        if (e == hipSuccess) e = hipMemcpy(d_pool_state, &init, sizeof(init), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&control, hipStreamNonBlocking);
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ev_stage[i], hipEventDisableTiming);
+        if (e != hipSuccess) {
+            set_error("server init", e);
+            return false;
+        }
+        for (int i = 0; i < 2; ++i)
+            if (!pinned_alloc(h_slots[i], STAGE_SLOTS) || !pinned_alloc(h_rels[i], SRV_RELS)) return false;
+        return pinned_alloc(h_stats, 1);
+    }
+
+    // ---- worker side -----------------------------------------------------
+
+    Lane* idle_lane(Key& k) {
+        for (auto& lp : k.lanes)
+            if (!lp->busy) return lp.get();
+        if ((int)k.lanes.size() >= cfg.lanes_per_level) return nullptr;
+        auto lp = std::make_unique<Lane>();
+        if (!lp->eng.init(k.cfg) || !lp->eng.prepare_device_tables(k.chunk_cap, k.buf_cap, &lp->tabs) ||
+            !pinned_alloc(lp->h_head, 1) || !pinned_alloc(lp->h_items, k.buf_cap) ||
+            hipEventCreateWithFlags(&lp->ev_done, hipEventDisableTiming) != hipSuccess)
+            return nullptr;
+        k.lanes.push_back(std::move(lp));
+        return k.lanes.back().get();
+    }
+
+    // Copies new requests into their rings and new releases into the release
+    // ring (control stream).  Returns whether anything was pushed.
+    bool push(std::vector<uint32_t>& rels) {
+        const int cur = stage_cur;
+        (void)hipEventSynchronize(ev_stage[cur]);  // the staging's previous copies are done
+        SrvSlot* st = h_slots[cur];
+        uint32_t used = 0;
+        hipError_t e = hipSuccess;
+        for (Key& k : keys) {
+            if (k.backlog.empty()) continue;
+            if (!k.d_ring) {
+                e = hipMalloc(reinterpret_cast<void**>(&k.d_ring), (size_t)RING * sizeof(SrvSlot));
+                if (e != hipSuccess) {
+                    k.d_ring = nullptr;
+                    break;
+                }
+                k.by_seq.assign(RING, nullptr);
+            }
+            const uint64_t first = k.pushed;
+            const uint32_t base = used;
+            while (!k.backlog.empty() && used < STAGE_SLOTS && k.pushed - k.seen < RING) {
+                Request* r = k.backlog.front();
+                k.backlog.pop_front();
+                SrvSlot s{};
+                s.in_addr = (uint64_t)(uintptr_t)r->d_in;
+                s.in_size = r->in_size;
+                s.cap = r->cap;
+                s.id = r->id;
+                s.flags = r->h_in ? SLOT_HOST_INPUT : 0u;
+                st[used++] = s;
+                k.by_seq[k.pushed % RING] = r;
+                ++k.pushed;
+            }
+            // the new range [first, pushed) of the ring, in at most two pieces
+            uint64_t seq = first;
+            uint32_t src = base;
+            while (seq < k.pushed && e == hipSuccess) {
+                const uint32_t pos = (uint32_t)(seq % RING);
+                const uint32_t m = (uint32_t)std::min<uint64_t>(k.pushed - seq, RING - pos);
+                e = hipMemcpyAsync(k.d_ring + pos, st + src, (size_t)m * sizeof(SrvSlot), hipMemcpyHostToDevice,
+                                   control);
+                seq += m;
+                src += m;
+            }
+            if (e != hipSuccess) break;
+        }
+        uint32_t nrel = 0;
+        if (e == hipSuccess && !rels.empty()) {
+            nrel = (uint32_t)std::min<size_t>(rels.size(), SRV_RELS);
+            std::copy(rels.begin(), rels.begin() + nrel, h_rels[cur]);
+            uint32_t done = 0;
+            while (done < nrel && e == hipSuccess) {
+                const uint32_t pos = (rel_tail + done) % SRV_RELS;
+                const uint32_t m = std::min(nrel - done, SRV_RELS - pos);
+                e = hipMemcpyAsync(d_rels + pos, h_rels[cur] + done, (size_t)m * 4, hipMemcpyHostToDevice, control);
+                done += m;
+            }
+            rel_tail += nrel;
+            rels.erase(rels.begin(), rels.begin() + nrel);
+        }
+        if (e == hipSuccess) e = hipEventRecord(ev_stage[cur], control);
+        stage_cur ^= 1;
+        if (e != hipSuccess) set_error("server push", e);
+        return used > 0 || nrel > 0;
+    }
+
+    AdmitArgs admit_args(Key& k, int key, Lane* L) {
+        AdmitArgs a{};
+        a.pool = d_pool_state;
+        a.recs = d_recs;
+        a.rels = d_rels;
+        a.rel_tail = rel_tail;
+        a.ring = k.d_ring;
+        a.head = d_heads + key;
+        a.tail = k.pushed;
+        a.ring_cap = RING;
+        a.chunk = k.cfg.chunk;
+        a.hdr = L->tabs.hdr;
+        a.bufs = L->tabs.bufs;
+        a.desc = L->tabs.desc;
+        a.order = L->tabs.order;
+        a.buf_cap = k.buf_cap;
+        a.chunk_cap = k.chunk_cap;
+        a.max_bytes = cfg.max_batch_bytes;
+        a.pool_addr = (uint64_t)(uintptr_t)d_pool;
+        a.lane = device_view(L->h_head);
+        a.items = device_view(L->h_items);
+        return a;
+    }
+
+    // Completes a lane's requests (status from the completion records, or
+    // failed) and hands them to the completion queue.
+    void finish(Lane& L, bool failed) {
+        const uint64_t t = now();
+        for (size_t i = 0; i < L.reqs.size(); ++i) {
+            Request* r = L.reqs[i];
+            r->out_size = failed ? 0 : L.h_items[i].out_size;
+            r->status = (!failed && L.h_items[i].status == 0 && r->out_size) ? 0 : -1;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (Request* r : L.reqs) {
+                r->done = true;
+                r->t_done = t;
+                completed.push_back(r);
+            }
+            n_completed += L.reqs.size();
+            n_inflight -= L.reqs.size();
+        }
+        cv_done.notify_all();
+        L.reqs.clear();
+        L.busy = false;
+    }
+
     bool retire() {
         bool any = false;
-        for (auto& kv : lanes)
-            for (auto& lp : kv.second) {
+        for (Key& k : keys)
+            for (auto& lp : k.lanes) {
                 Lane& L = *lp;
-                if (!L.busy || !L.eng.done()) continue;
-                L.sizes.resize(L.reqs.size());
-                const bool ok = L.eng.wait(L.sizes.data());
-                const uint64_t t = now();
-                for (size_t i = 0; i < L.reqs.size(); ++i) {
-                    Request* r = L.reqs[i];
-                    r->out_size = ok ? L.sizes[i] : 0;
-                    r->status = r->out_size ? 0 : -1;
-                    r->done = true;
-                    r->t_done = t;
-                    if (r->in_pool) {
-                        pool.release(r->in_off, r->in_size);
-                        r->in_pool = false;
-                    }
-                    completed.push_back(r);
-                }
-                n_completed += L.reqs.size();
-                n_inflight -= L.reqs.size();
-                L.reqs.clear();
-                L.busy = false;
+                if (!L.busy) continue;
+                const hipError_t q = hipEventQuery(L.ev_done);
+                if (q == hipErrorNotReady) continue;
+                finish(L, q != hipSuccess);
                 any = true;
             }
         return any;
     }
 
-    Lane* idle_lane(int key) {
-        auto& v = lanes[key];
-        for (auto& lp : v)
-            if (!lp->busy) return lp.get();
-        if ((int)v.size() >= cfg.lanes_per_level) return nullptr;
-        auto lp = std::make_unique<Lane>();
-        EngineConfig ec;
-        ec.level = key / 4;
-        ec.format = key % 4;
-        if (!lp->eng.init(ec)) return nullptr;
-        lp->level = ec.level;
-        lp->format = ec.format;
-        v.push_back(std::move(lp));
-        return v.back().get();
-    }
-
-    // form and enqueue batches on idle lanes (mu held)
-    bool launch() {
-        bool any = false;
-        for (auto& kv : queues) {
-            auto& q = kv.second;
-            while (!q.empty()) {
-                Lane* L = idle_lane(kv.first);
-                if (!L) break;
-                std::vector<Request*> take;
-                uint64_t bytes = 0;
-                while (!q.empty() && (int)take.size() < cfg.max_batch_requests &&
-                       (take.empty() || bytes + q.front()->in_size <= cfg.max_batch_bytes)) {
-                    Request* r = q.front();
-                    uint64_t out_off = 0, in_off = 0;
-                    if (!pool.alloc(r->cap, out_off)) break;
-                    if (r->h_in) {
-                        if (!pool.alloc(r->in_size, in_off)) {
-                            pool.release(out_off, r->cap);
-                            break;
-                        }
-                        r->in_off = in_off;
-                        r->in_pool = true;
-                        r->d_in = d_pool + in_off;
-                    }
-                    r->out_off = out_off;
-                    bytes += r->in_size;
-                    take.push_back(r);
-                    q.pop_front();
-                    --n_queued;
-                }
-                if (take.empty()) break;  // the queue head waits for memory
-                const uint8_t* base = take[0]->d_in;
-                for (Request* r : take) base = std::min(base, r->d_in);
-                std::vector<BufferSpec> specs(take.size());
-                bool ok = true;
-                for (size_t i = 0; i < take.size(); ++i) {
-                    Request* r = take[i];
-                    if (r->h_in && r->in_size) {
-                        ok = ok && hipMemcpyAsync(const_cast<uint8_t*>(r->d_in), r->h_in, r->in_size,
-                                                  hipMemcpyHostToDevice, L->eng.stream()) == hipSuccess;
-                    }
-                    specs[i] = {(uint64_t)(r->d_in - base), r->in_size, r->out_off, r->cap};
-                }
-                ok = ok && L->eng.submit(base, d_pool, specs.data(), specs.size(), false);
-                const uint64_t t = now();
-                for (Request* r : take) r->t_launch = t;
-                n_inflight += take.size();
+    // One round of admissions: every queue with requests and an idle lane.
+    // Returns the number of requests admitted.
+    uint64_t admit_round() {
+        std::vector<std::pair<Key*, Lane*>> started;
+        for (int key = 0; key < NKEYS; ++key) {
+            Key& k = keys[key];
+            if (k.pushed == k.seen || k.stalled) continue;
+            Lane* L = idle_lane(k);
+            if (!L) continue;
+            if (launch_srv_admit(admit_args(k, key, L), control) != hipSuccess) continue;
+            L->busy = true;  // reserved until the admission is read
+            started.push_back({&k, L});
+        }
+        if (started.empty()) return 0;
+        const uint64_t t = now();
+        const hipError_t se = hipStreamSynchronize(control);
+        uint64_t admitted = 0, batches = 0;
+        for (auto& [kp, L] : started) {
+            Key& k = *kp;
+            L->busy = false;
+            if (se != hipSuccess) continue;
+            const LaneHead h = *L->h_head;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                pool_in_use = h.in_use;
+            }
+            if (h.count == 0) {
+                k.stalled = h.pending > 0;
+                continue;
+            }
+            L->reqs.resize(h.count);
+            bool ok = h.first_seq == k.seen;
+            for (uint32_t i = 0; i < h.count && ok; ++i) {
+                Request* r = k.by_seq[(h.first_seq + i) % RING];
+                ok = r && r->id == L->h_items[i].id;
+                L->reqs[i] = r;
+                if (!ok) break;
+                r->out_off = L->h_items[i].out_off;
+                r->in_off = L->h_items[i].in_off;
+                r->rec = L->h_items[i].rec;
+                r->t_launch = t;
+            }
+            k.seen += h.count;
+            admitted += h.count;
+            ++batches;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                n_admitted += h.count;
+                n_inflight += h.count;
                 ++n_batches;
-                L->reqs = std::move(take);
-                L->busy = true;
-                if (!ok) {
-                    // the lane's done event was not recorded: finish the batch as failed now
-                    (void)hipStreamSynchronize(L->eng.stream());
-                    for (Request* r : L->reqs) {
-                        r->status = -1;
-                        r->out_size = 0;
-                        r->done = true;
-                        r->t_done = now();
-                        if (r->in_pool) {
-                            pool.release(r->in_off, r->in_size);
-                            r->in_pool = false;
-                        }
-                        completed.push_back(r);
-                    }
-                    n_completed += L->reqs.size();
-                    n_inflight -= L->reqs.size();
-                    L->reqs.clear();
-                    L->busy = false;
-                }
-                any = true;
+            }
+            L->busy = true;
+            hipError_t e = ok ? hipSuccess : hipErrorInvalidValue;
+            if (!ok) set_error_msg("server: admission records do not match the queue");
+            const hipStream_t s = L->eng.stream();
+            for (Request* r : L->reqs)
+                if (e == hipSuccess && r->h_in && r->in_size)
+                    e = hipMemcpyAsync(d_pool + r->in_off, r->h_in, r->in_size, hipMemcpyHostToDevice, s);
+            if (e == hipSuccess && !L->eng.launch_device_tables(d_pool, d_pool, h.nchunks, h.count))
+                e = hipErrorLaunchFailure;
+            if (e == hipSuccess)
+                e = launch_srv_complete(L->tabs.hdr, L->tabs.results, device_view(L->h_items), d_pool_state, s);
+            if (e == hipSuccess) e = hipEventRecord(L->ev_done, s);
+            if (e != hipSuccess) {
+                // the batch is not (wholly) enqueued: finish it as failed once the stream is idle
+                (void)hipStreamSynchronize(s);
+                if (ok) set_error("server launch", e);
+                finish(*L, true);
             }
         }
-        return any;
+        (void)batches;
+        return admitted;
+    }
+
+    void refresh_stats() {
+        AdmitArgs a{};
+        a.pool = d_pool_state;
+        a.recs = d_recs;
+        a.rels = d_rels;
+        a.rel_tail = rel_tail;
+        a.lane = device_view(h_stats);
+        if (launch_srv_admit(a, control) == hipSuccess && hipStreamSynchronize(control) == hipSuccess) {
+            std::lock_guard<std::mutex> lk(mu);
+            pool_in_use = h_stats->in_use;
+        }
     }
 
     bool any_busy() const {
-        for (auto& kv : lanes)
-            for (auto& lp : kv.second)
+        for (const Key& k : keys)
+            for (auto& lp : k.lanes)
                 if (lp->busy) return true;
         return false;
     }
 
+    bool all_admitted_or_stalled() const {
+        for (const Key& k : keys)
+            if (!k.stalled && (!k.backlog.empty() || k.pushed != k.seen)) return false;
+        return true;
+    }
+
     void run() {
-        std::unique_lock<std::mutex> lk(mu);
+        (void)hipSetDevice(device);
+        std::vector<Request*> new_reqs;
+        std::vector<uint32_t> new_rels;
+        uint64_t stats_served = 0;
         for (;;) {
-            const uint64_t seen = wake;
-            const bool retired = retire();
-            if (retired) cv_done.notify_all();
-            const bool launched = launch();
+            uint64_t seen_wake, stats_ticket;
+            bool stopping;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                new_reqs.insert(new_reqs.end(), inbox.begin(), inbox.end());
+                inbox.clear();
+                new_rels.insert(new_rels.end(), rel_inbox.begin(), rel_inbox.end());
+                rel_inbox.clear();
+                stopping = stop;
+                stats_ticket = stats_req;
+                seen_wake = wake;
+            }
+            for (Request* r : new_reqs) keys[r->level * 4 + r->format].backlog.push_back(r);
+            new_reqs.clear();
+            if (!new_rels.empty())
+                for (Key& k : keys) k.stalled = false;
+            bool progress = push(new_rels);
+            progress = retire() || progress;
+            while (admit_round() > 0) progress = true;
+            if (stats_ticket > stats_served) {
+                refresh_stats();
+                stats_served = stats_ticket;
+                std::lock_guard<std::mutex> lk(mu);
+                stats_done = std::max(stats_done, stats_ticket);
+                cv_done.notify_all();
+            }
             const bool busy = any_busy();
-            // at stop, requests that can never be admitted (the pool is full of
-            // outputs nobody will release) are dropped with the server
-            if (stop && !busy && (n_queued == 0 || !launched)) break;
+            if (stopping && !busy && new_rels.empty() && all_admitted_or_stalled()) break;
+            std::unique_lock<std::mutex> lk(mu);
             if (busy) {
-                // a batch is on the GPU: check again shortly
-                cv_work.wait_for(lk, std::chrono::microseconds(50));
-            } else if (!launched && !retired) {
-                // nothing runs and the queue heads wait for memory (or the
-                // queues are empty): sleep until a submit, release or stop
-                cv_work.wait(lk, [&] { return stop || wake != seen; });
+                // a batch is on the GPU: look again shortly, or at once on news
+                cv_work.wait_for(lk, std::chrono::microseconds(50), [&] { return wake != seen_wake; });
+            } else if (!progress && new_rels.empty()) {
+                cv_work.wait(lk, [&] { return stop || wake != seen_wake; });
             }
         }
+        std::lock_guard<std::mutex> lk(mu);
+        worker_exited = true;
+        cv_done.notify_all();
     }
+
+    // ---- API side --------------------------------------------------------
 
     uint64_t submit(const void* d_in, const void* h_in, uint64_t n, int level, int format) {
         if (level < 0) level = 0;
         if (level > 12) level = 12;
         if (format != 1 && format != 2) format = 0;
+        const Key& k = keys[level * 4 + format];
         auto r = std::make_unique<Request>();
         r->level = level;
         r->format = format;
         r->d_in = static_cast<const uint8_t*>(d_in);
         r->h_in = h_in;
         r->in_size = n;
-        r->cap = out_capacity(n, level, format);
-        const uint64_t need = align_up(r->cap) + (h_in ? align_up(std::max<uint64_t>(n, 1)) : 0);
-        if (need > cfg.hbm_budget) {
-            orot_gpu::set_error_msg("deflate_server_submit: request larger than the HBM budget");
+        r->cap = out_capacity(n, k.cfg.chunk, format);
+        r->need = align_up(r->cap) + (h_in ? align_up(std::max<uint64_t>(n, 1)) : 0);
+        if (r->need > cfg.hbm_budget) {
+            set_error_msg("deflate_server_submit: request larger than the HBM budget");
+            return 0;
+        }
+        const uint64_t nch = n ? (n + k.cfg.chunk - 1) / k.cfg.chunk : 1;
+        if (nch > k.chunk_cap) {
+            set_error_msg("deflate_server_submit: request of " + std::to_string(nch) + " blocks; a lane holds " +
+                          std::to_string(k.chunk_cap) + " (lane_arena_bytes)");
             return 0;
         }
         std::lock_guard<std::mutex> lk(mu);
@@ -313,11 +547,10 @@ struct deflate_server {
         r->t_submit = now();
         Request* p = r.get();
         live[p->id] = std::move(r);
-        queues[level * 4 + format].push_back(p);
+        inbox.push_back(p);
         ++n_submitted;
-        ++n_queued;
+        max_queue = std::max(max_queue, n_submitted - n_admitted);
         ++wake;
-        max_queue = std::max(max_queue, n_queued);
         cv_work.notify_one();
         return p->id;
     }
@@ -333,20 +566,17 @@ deflate_server* deflate_server_new(const deflate_server_config* cfg) {
     if (!s->cfg.max_batch_bytes) s->cfg.max_batch_bytes = 256ull << 20;
     if (s->cfg.max_batch_requests <= 0) s->cfg.max_batch_requests = 4096;
     if (s->cfg.lanes_per_level <= 0) s->cfg.lanes_per_level = 2;
+    if (!s->cfg.lane_arena_bytes) s->cfg.lane_arena_bytes = 2ull << 30;
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-        orot_gpu::set_error_msg("deflate_server_new: no usable GPU");
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0 || hipGetDevice(&s->device) != hipSuccess) {
+        set_error_msg("deflate_server_new: no usable GPU");
         delete s;
         return nullptr;
     }
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->d_pool), s->cfg.hbm_budget);
-    if (e != hipSuccess) {
-        s->d_pool = nullptr;
-        orot_gpu::set_error("hipMalloc(server pool)", e);
+    if (!s->init()) {
         delete s;
         return nullptr;
     }
-    s->pool.reset(s->cfg.hbm_budget);
     s->worker = std::thread([s] { s->run(); });
     return s;
 }
@@ -360,8 +590,6 @@ void deflate_server_free(deflate_server* s) {
         s->cv_work.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
-    s->lanes.clear();
-    if (s->d_pool) (void)hipFree(s->d_pool);
     delete s;
 }
 
@@ -421,7 +649,7 @@ size_t deflate_server_read(deflate_server* s, uint64_t id, void* out, size_t cap
     if (n > cap) return 0;
     hipError_t e = hipMemcpy(out, src, n, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
-        orot_gpu::set_error("hipMemcpy(server output)", e);
+        set_error("hipMemcpy(server output)", e);
         return 0;
     }
     return n;
@@ -432,18 +660,23 @@ int deflate_server_release(deflate_server* s, uint64_t id) {
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->live.find(id);
     if (it == s->live.end() || !it->second->done) return -1;
-    s->pool.release(it->second->out_off, it->second->cap);
+    s->rel_inbox.push_back(it->second->rec);
     s->live.erase(it);
     ++s->wake;
-    s->cv_work.notify_one();  // freed memory may admit the queue head
+    s->cv_work.notify_one();  // freed memory may admit a queue head
     return 0;
 }
 
 int deflate_server_stats(deflate_server* s, uint64_t* v, int max) {
     if (!s || !v) return 0;
-    std::lock_guard<std::mutex> lk(s->mu);
-    const uint64_t vals[7] = {s->n_submitted, s->n_completed, s->n_queued, s->n_inflight,
-                              s->pool.used(), s->n_batches, s->max_queue};
+    std::unique_lock<std::mutex> lk(s->mu);
+    // the pool figure comes from the device, after the releases so far
+    const uint64_t ticket = ++s->stats_req;
+    ++s->wake;
+    s->cv_work.notify_one();
+    s->cv_done.wait(lk, [&] { return s->stats_done >= ticket || s->worker_exited; });
+    const uint64_t vals[7] = {s->n_submitted, s->n_completed, s->n_submitted - s->n_admitted, s->n_inflight,
+                              s->pool_in_use, s->n_batches, s->max_queue};
     const int k = std::min(max, 7);
     for (int i = 0; i < k; ++i) v[i] = vals[i];
     return k;

@@ -133,3 +133,73 @@ def test_arrivals_during_batches(amd):
             srv.release(rid)
         assert srv.stats()["batches"] > 2
     d_in.free()
+
+
+def test_pool_wraps_and_reclaims_out_of_order(amd):
+    """A pool a few requests wide: allocations wrap around its end, the
+    client releases completed requests in a shuffled order and holds some
+    back; outputs that are held at the same time never overlap, the pool
+    figure never exceeds the budget, and it returns to 0 at the end."""
+    rng = np.random.default_rng(17)
+    budget = 3 << 20
+    reqs = []
+    for i in range(80):
+        n = int(rng.integers(1024, 400_000))
+        reqs.append((corpus.generate(corpus.GENERATORS[i % 4], n, 300 + i), (1, 2)[i % 2], 2))
+    with amd.Server(hbm_budget=budget, lanes_per_level=2) as srv:
+        ids = [srv.submit(b, lv, fmt) for b, lv, fmt in reqs]
+        index = {rid: i for i, rid in enumerate(ids)}
+        held = {}
+        got = {}
+        t0 = time.time()
+        while len(got) < len(ids) and time.time() - t0 < 120:
+            for c in srv.poll(100):
+                assert c[3] == 0
+                got[c[0]] = srv.read(c[0], c[2])
+                held[c[0]] = (srv.output_ptr(c[0]), c[2])
+            spans = sorted(held.values())
+            for (p0, n0), (p1, _) in zip(spans, spans[1:]):
+                assert p0 + n0 <= p1, "outputs held at once overlap"
+            # release a shuffled part of what is held, keep the rest a while
+            keys = list(held)
+            rng.shuffle(keys)
+            for rid in keys[: max(1, len(keys) * 2 // 3)]:
+                srv.release(rid)
+                del held[rid]
+            assert srv.stats()["pool_bytes"] <= budget
+        assert len(got) == len(ids), srv.stats()
+        for rid in list(held):
+            srv.release(rid)
+        st = srv.stats()
+        assert st["pool_bytes"] == 0 and st["completed"] == len(ids) and st["queued"] == 0
+        for rid, i in index.items():
+            b, lv, fmt = reqs[i]
+            assert got[rid] == _want(b, lv, fmt), (i, b.size, lv)
+
+
+def test_pool_accounting_is_on_the_device(amd):
+    """Completed outputs hold pool bytes until they are released; the figure
+    is the device allocator's (it includes the host input's copy)."""
+    b = corpus.generate("text", 300_000, 5)
+    with amd.Server(hbm_budget=64 << 20) as srv:
+        rid = srv.submit(b, 1, 2)
+        done = _drain(srv, 1)
+        assert done[rid][3] == 0
+        used = srv.stats()["pool_bytes"]
+        assert used >= 2 * 300_000
+        assert srv.read(rid, done[rid][2]) == _want(b, 1, 2)
+        srv.release(rid)
+        assert srv.stats()["pool_bytes"] == 0
+
+
+def test_lane_capacity_bounds_a_request(amd):
+    """A request with more blocks than a lane's working memory holds is
+    refused at submission; one that fits runs."""
+    with amd.Server(hbm_budget=256 << 20, lane_arena_bytes=8 << 20) as srv:
+        with pytest.raises(amd.Error):
+            srv.submit(np.zeros(64 << 20, dtype=np.uint8), 1, 0)
+        b = corpus.generate("records", 200_000, 8)
+        rid = srv.submit(b, 1, 0)
+        done = _drain(srv, 1)
+        assert srv.read(rid, done[rid][2]) == _want(b, 1, 0)
+        srv.release(rid)
