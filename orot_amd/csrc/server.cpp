@@ -245,6 +245,9 @@ struct deflate_server {
     // Copies new requests into their rings and new releases into the release
     // ring (control stream).  Returns whether anything was pushed.
     bool push(std::vector<uint32_t>& rels) {
+        bool work = !rels.empty();
+        for (const Key& k : keys) work = work || (!k.backlog.empty() && k.pushed - k.seen < RING);
+        if (!work) return false;
         const int cur = stage_cur;
         (void)hipEventSynchronize(ev_stage[cur]);  // the staging's previous copies are done
         SrvSlot* st = h_slots[cur];
