@@ -89,10 +89,18 @@ def _gen(spec):
     return corpus.generate(kind, int(size), int(seed))
 
 
+_BUFFERS = {}  # buffers generated ahead of time, by their specs
+
+
 def make_buffers(specs):
     """Generate (kind, size, seed) buffers in a process pool.  Called before
-    anything touches the GPU, so the forked workers hold no device state."""
+    anything touches the GPU, so the forked workers hold no device state
+    (buffers a later leg needs are generated up front by pregenerate)."""
     from concurrent.futures import ProcessPoolExecutor
+
+    got = _BUFFERS.pop(tuple(specs), None)
+    if got is not None:
+        return got
 
     workers = max(1, min(cpu_threads(), 16))
     if workers == 1 or len(specs) < 4:
@@ -734,6 +742,28 @@ def bench_batch(args) -> dict:
         p = bench_l1(sub)
         rec["plain_256MiB"] = {k: p[k] for k in ("value", "ms_per_step", "p50_ms", "p99_ms", "compressed_bytes",
                                                  "inflate_ok", "matches_orot_bytes", "roofline")}
+    if not args.batch_only and not args.no_extras:
+        # the level-6 batch (orot's ratio at L6 from its host path over a sample)
+        sub = argparse.Namespace(**vars(args))
+        sub.steps = min(args.steps, 5)
+        sub.warmup = min(args.warmup, 1)
+        l6 = bench_l6(sub)
+        rec["l6_batch"] = {"workload": l6["config"]["workload"], "steps": sub.steps}
+        rec["l6_batch"].update({k: l6[k] for k in (
+            "value", "unit", "ms_per_step", "p50_ms", "p99_ms", "compressed_bytes", "ratio", "inflate_ok",
+            "plain_path_gbps", "speedup_vs_plain", "alone_equals_in_mix", "roofline") if k in l6})
+        if "ratio_vs_orot" in l6:
+            rec["l6_batch"]["ratio_vs_orot"] = l6["ratio_vs_orot"]
+            rec["l6_batch"]["cpu_batch"] = l6.get("cpu_batch")
+        # a shorter run of the serving stream
+        sub = argparse.Namespace(**vars(args))
+        sub.duration = EXTRA_SERVE_S
+        sub.no_cpu = True
+        sv = bench_serve(sub)
+        rec["serving"] = {"workload": sv["config"]["workload"]}
+        rec["serving"].update({k: sv[k] for k in (
+            "value", "unit", "offered_gbps", "completed", "completed_ok", "p50_ms", "p99_ms", "p999_ms",
+            "latency_by_level_ms", "queue_p50_ms", "queue_p99_ms", "inflate_ok", "outputs_checked", "server")})
     return rec
 
 
@@ -741,6 +771,7 @@ def bench_batch(args) -> dict:
 # content uniform over the four generators, level uniform over {1, 6}, gzip,
 # 32 GiB HBM budget.
 SERVE_RATE = 2000.0
+EXTRA_SERVE_S = 20.0  # the serving run inside the default line
 SERVE_LO = 1 << 10
 SERVE_HI = 8 << 20
 SERVE_MASTER = 0x5E7E0004
@@ -887,6 +918,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch / l6: time only the graph-submitted batch (for profilers)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="batch: skip the level-6 batch and the serving run added to the default line")
     ap.add_argument("--no-plain256", action="store_true",
                     help="batch: skip the extra plain-path line on the 256 MiB four-segment buffer")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -898,6 +931,9 @@ def main() -> int:
         print("bench.py measures one GPU only (--gpus 1). Across GPUs a batch would split into "
               "independent sub-batches, one per GPU, with no exchange between them.", file=sys.stderr)
         return 2
+    if args.workload == "batch" and not args.batch_only and not args.no_extras:
+        specs = l6_specs()
+        _BUFFERS[tuple(specs)] = make_buffers(specs)
     rec = {"l1": bench_l1, "text16": bench_text16, "l6": bench_l6, "batch": bench_batch,
            "serve": bench_serve, "stream": bench_stream}[args.workload](args)
     try:

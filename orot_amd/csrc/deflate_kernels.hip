@@ -1006,13 +1006,26 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         bool all258 = r0 < r1;
         for (uint32_t p = r0; p < r1; p += 4) {
             const uint32_t cw = D[p >> 2];
+            const uint32_t nc = ~cw;
+            if (((nc - 0x01010101u) & ~nc & 0x80808080u) == 0u) {
+                // no code 255 in the word: every step comes from its code
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (p + k < r1) {
+                for (int k = 0; k < 4; ++k) {
                     const uint32_t b = (cw >> (8 * k)) & 0xFFu;
-                    all258 &= b == 254u;
-                    runmax = max(runmax, p + k + code_step(b, T, L, p + k));
+                    const uint32_t st = b == 0u ? 1u : (b == 254u ? 258u : b + 2u);
+                    const bool valid = p + k < r1;
+                    all258 &= !valid || b == 254u;
+                    runmax = max(runmax, valid ? p + k + st : 0u);
                 }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (p + k < r1) {
+                        const uint32_t b = (cw >> (8 * k)) & 0xFFu;
+                        all258 &= b == 254u;
+                        runmax = max(runmax, p + k + code_step(b, T, L, p + k));
+                    }
+            }
         }
         if (all258) atomicOr(&run258[tid >> 5], 1u << (tid & 31));
         uint32_t v = runmax;
