@@ -442,11 +442,14 @@ __device__ __forceinline__ uint32_t l6_decide(const uint32_t* ML, uint32_t n, ui
 // words loaded first, addresses clamped into the chunk).  Items are chunk x
 // segment, grid-stride, counts from the batch header.
 constexpr int L6D_THREADS = 1024;
+// With gbm (the split parse) each item also clears the token-bitmap words of
+// its positions in the chunk's staging words.
 __global__ __launch_bounds__(L6D_THREADS) void l6_decide_kernel(const ChunkDesc* __restrict__ desc,
                                                                 const BatchHdr* __restrict__ hdr, uint32_t chunk,
                                                                 const uint32_t* __restrict__ mlw,
                                                                 uint16_t* __restrict__ codew, const L6Params prm,
-                                                                uint32_t segs) {
+                                                                uint32_t segs, uint32_t* __restrict__ gbm,
+                                                                uint64_t gbm_stride) {
     const uint32_t items = hdr->nchunks * segs;
     const int lazy = prm.lazy_depth;
     for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
@@ -458,6 +461,10 @@ __global__ __launch_bounds__(L6D_THREADS) void l6_decide_kernel(const ChunkDesc*
         const uint64_t cp = l6_stride(chunk);
         const uint32_t* ML = mlw + (uint64_t)c * cp;
         uint16_t* CODE = codew + (uint64_t)c * cp;
+        if (gbm) {  // pa is a multiple of 4096
+            uint32_t* G = gbm + (uint64_t)c * gbm_stride;
+            for (uint32_t w = (pa >> 5) + threadIdx.x; w < ((pb + 31) >> 5); w += L6D_THREADS) G[w] = 0;
+        }
         for (uint32_t p0 = pa + threadIdx.x; p0 < pb; p0 += 8 * L6D_THREADS) {
             uint32_t m0[8], m1[8];
 #pragma unroll
@@ -658,6 +665,304 @@ __global__ __launch_bounds__(L6R_THREADS) void l6_parse_kernel(const uint8_t* __
         const uint32_t k = next_item(&hdr->queue[Q_PARSE], hdr->nchunks, smem);
         if (k == NO_ITEM) break;
         l6_parse_body(order[k], smem, src, desc, chunk, mlw, codew, toks, ntok_out, blkpos, maxblk, prm);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Split parse: for batches of few chunks, each chunk is cut into segments of
+// whole bitmap words, one workgroup each.  A decision's step is at most 3 lazy
+// literals plus 258, so whether q is a parse barrier depends only on the
+// decisions of the L6_REACH positions before q: a segment finds its own
+// barriers from the reach of the positions just before it.  Three launches:
+//   l6_pseg_parse_kernel  barriers and exact parse of the segment's ranges;
+//                         token starts into the chunk's bitmap in global
+//                         memory (its staging words, cleared by the decisions
+//                         kernel).  The parse of the last range runs on into
+//                         the next segment, up to the first barrier at or
+//                         after the segment end (looked for over a bounded
+//                         stretch; without one it runs to the chunk end,
+//                         setting the bits that later segments set as well);
+//   l6_pseg_count_kernel  tokens per segment;
+//   l6_pseg_write_kernel  the segment's token offset (the counts of the
+//                         segments before it) and its tokens, in order.
+// ---------------------------------------------------------------------------
+constexpr uint32_t L6_REACH = 262;
+constexpr uint32_t L6_NEXT_SCAN = 4096;  // positions searched for the next segment's first barrier
+
+__device__ __forceinline__ void pseg_bounds(uint32_t n, uint32_t segs, uint32_t g, uint32_t& pa, uint32_t& pb) {
+    const uint32_t per = (((n + segs - 1) / segs) + 31u) & ~31u;
+    pa = min(n, g * per);
+    pb = min(n, pa + per);
+}
+__device__ __forceinline__ uint32_t pseg_range_len(uint32_t pa, uint32_t pb) {
+    return max(32u, (((pb - pa + L6R_THREADS - 1) / L6R_THREADS) + 31u) & ~31u);
+}
+// the per-segment token counts follow the chunk's bitmap words
+__host__ __device__ constexpr uint64_t pseg_count_word(uint32_t chunk) { return (chunk + 31u) / 32u + 1u; }
+
+// 32 decisions from position p (a multiple of 32: aligned 16-byte loads)
+__device__ __forceinline__ void codes32_at(const uint16_t* CODE, uint32_t p, uint32_t* w) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint4 v = *reinterpret_cast<const uint4*>(CODE + p + 8 * i);
+        w[4 * i] = v.x;
+        w[4 * i + 1] = v.y;
+        w[4 * i + 2] = v.z;
+        w[4 * i + 3] = v.w;
+    }
+}
+__device__ __forceinline__ uint32_t code_in(const uint32_t* w, int k) { return (w[k >> 1] >> (16 * (k & 1))) & 0xFFFFu; }
+
+// tokens of the decisions whose bits are set in bitmap words [w0, w1)
+__device__ __forceinline__ uint32_t pseg_tokens(const uint32_t* G, const uint16_t* CODE, uint32_t w0, uint32_t w1) {
+    uint32_t cnt = 0;
+    for (uint32_t w = w0; w < w1; ++w) {
+        uint32_t x = G[w];
+        if (!x) continue;
+        uint32_t cw[16];
+        codes32_at(CODE, w << 5, cw);
+        while (x) {
+            const int k = __builtin_ctz(x);
+            x &= x - 1;
+            cnt += 1u + (code_in(cw, k) >> 12);
+        }
+    }
+    return cnt;
+}
+
+__global__ __launch_bounds__(L6R_THREADS) void l6_pseg_parse_kernel(const ChunkDesc* __restrict__ desc,
+                                                                    const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                                    const uint16_t* __restrict__ codew,
+                                                                    uint32_t* __restrict__ gbm, uint64_t gbm_stride,
+                                                                    uint32_t segs) {
+    extern __shared__ uint32_t smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t items = hdr->nchunks * segs;
+    uint32_t* bm = smem;                          // the segment's bitmap words
+    uint32_t* fb = smem + l6r_bm_words(chunk);    // first barrier per range
+    uint32_t* misc = fb + L6R_THREADS;
+    for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t c = it / segs, g = it % segs;
+        const uint32_t n = desc[c].len;
+        uint32_t pa, pb;
+        pseg_bounds(n, segs, g, pa, pb);
+        if (pa >= pb) continue;
+        const uint16_t* CODE = codew + (uint64_t)c * l6_stride(chunk);
+        uint32_t* G = gbm + (uint64_t)c * gbm_stride;
+        const uint32_t nwl = (pb - pa + 31) >> 5;
+        const uint32_t S = pseg_range_len(pa, pb);
+        const uint32_t r0 = min(pb, pa + (uint32_t)tid * S), r1 = min(pb, r0 + S);
+        for (uint32_t w = tid; w < nwl; w += L6R_THREADS) bm[w] = 0;
+        if (tid == 0) misc[32] = 0;
+        __syncthreads();
+        // the farthest reach of the positions before the segment
+        if ((uint32_t)tid < L6_REACH && pa > (uint32_t)tid) {
+            const uint32_t p = pa - 1 - (uint32_t)tid;
+            atomicMax(&misc[32], p + (CODE[p] & 0xFFFu));
+        }
+        __syncthreads();
+        const uint32_t carry_in = misc[32];
+        uint32_t runmax = 0;
+        for (uint32_t p0 = r0; p0 < r1; p0 += 32) {
+            uint32_t w[16];
+            codes32_at(CODE, p0, w);
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+                if (p0 + k < r1) runmax = max(runmax, p0 + k + (code_in(w, k) & 0xFFFu));
+        }
+        uint32_t v = runmax;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t t = __shfl_up(v, d, 64);
+            if (lane >= d) v = max(v, t);
+        }
+        if (lane == 63) misc[wv] = v;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = carry_in;
+            for (int i = 0; i < L6R_WAVES; ++i) {
+                const uint32_t t = misc[i];
+                misc[i] = acc;
+                acc = max(acc, t);
+            }
+            misc[33] = acc;  // reach of every position before pb
+        }
+        __syncthreads();
+        const uint32_t prev_lane = __shfl_up(v, 1, 64);
+        uint32_t carry = max(misc[wv], lane ? prev_lane : 0u);
+        uint32_t first = 0xFFFFFFFFu;
+        for (uint32_t p0 = r0; p0 < r1 && first == 0xFFFFFFFFu; p0 += 32) {
+            uint32_t w[16];
+            codes32_at(CODE, p0, w);
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                const uint32_t p = p0 + k;
+                if (p < r1 && first == 0xFFFFFFFFu) {
+                    if (carry <= p) first = p;
+                    else carry = max(carry, p + (code_in(w, k) & 0xFFFu));
+                }
+            }
+        }
+        fb[tid] = r0 < r1 ? first : 0xFFFFFFFFu;
+        // the first barrier at or after pb
+        if (tid == 0) {
+            uint32_t nxt = 0xFFFFFFFFu;
+            uint32_t cr = misc[33];
+            for (uint32_t q = pb; q < min(n, pb + L6_NEXT_SCAN); ++q) {
+                if (cr <= q) {
+                    nxt = q;
+                    break;
+                }
+                cr = max(cr, q + (CODE[q] & 0xFFFu));
+            }
+            misc[34] = nxt;
+        }
+        __syncthreads();
+        const uint32_t nxt = misc[34];
+        // exact parse from this range's barrier to the next barrier of a later range
+        const uint32_t pstart = fb[tid];
+        if (pstart != 0xFFFFFFFFu) {
+            uint32_t pos = pstart;
+            while (pos < n) {
+                if (pos >= r1 && (pos < pb ? fb[(pos - pa) / S] == pos : pos == nxt)) break;
+                if (pos < pb) atomicOr(&bm[(pos - pa) >> 5], 1u << (pos & 31));
+                else atomicOr(&G[pos >> 5], 1u << (pos & 31));
+                pos += CODE[pos] & 0xFFFu;
+            }
+        }
+        __syncthreads();
+        for (uint32_t w = tid; w < nwl; w += L6R_THREADS)
+            if (bm[w]) atomicOr(&G[(pa >> 5) + w], bm[w]);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(L6R_THREADS) void l6_pseg_count_kernel(const ChunkDesc* __restrict__ desc,
+                                                                    const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                                    const uint16_t* __restrict__ codew,
+                                                                    uint32_t* __restrict__ gbm, uint64_t gbm_stride,
+                                                                    uint32_t segs) {
+    __shared__ uint32_t red[L6R_WAVES];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t items = hdr->nchunks * segs;
+    for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t c = it / segs, g = it % segs;
+        const uint32_t n = desc[c].len;
+        uint32_t pa, pb;
+        pseg_bounds(n, segs, g, pa, pb);
+        uint32_t* G = gbm + (uint64_t)c * gbm_stride;
+        uint32_t cnt = 0;
+        if (pa < pb) {
+            const uint16_t* CODE = codew + (uint64_t)c * l6_stride(chunk);
+            const uint32_t S = pseg_range_len(pa, pb);
+            const uint32_t r0 = min(pb, pa + (uint32_t)tid * S), r1 = min(pb, r0 + S);
+            if (r0 < r1) cnt = pseg_tokens(G, CODE, r0 >> 5, (r1 + 31) >> 5);
+        }
+        const uint32_t incl = wave_incl_scan(cnt, lane);
+        if (lane == 63) red[wv] = incl;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (int i = 0; i < L6R_WAVES; ++i) acc += red[i];
+            G[pseg_count_word(chunk) + g] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(L6R_THREADS) void l6_pseg_write_kernel(const uint8_t* __restrict__ src,
+                                                                    const ChunkDesc* __restrict__ desc,
+                                                                    const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                                    const uint32_t* __restrict__ mlw,
+                                                                    const uint16_t* __restrict__ codew,
+                                                                    const uint32_t* __restrict__ gbm,
+                                                                    uint64_t gbm_stride, uint32_t segs,
+                                                                    uint32_t* __restrict__ toks,
+                                                                    uint32_t* __restrict__ ntok_out,
+                                                                    uint32_t* __restrict__ blkpos, uint32_t maxblk) {
+    __shared__ uint32_t misc[L6R_WAVES + 2];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t items = hdr->nchunks * segs;
+    for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t c = it / segs, g = it % segs;
+        const uint32_t n = desc[c].len;
+        uint32_t pa, pb;
+        pseg_bounds(n, segs, g, pa, pb);
+        if (pa >= pb) {
+            if (n == 0 && g == 0 && tid == 0) ntok_out[c] = 0;
+            continue;
+        }
+        const uint8_t* s = src + desc[c].in_off;
+        const uint64_t cp = l6_stride(chunk);
+        const uint32_t* ML = mlw + (uint64_t)c * cp;
+        const uint16_t* CODE = codew + (uint64_t)c * cp;
+        uint32_t* TK = toks + (uint64_t)c * cp;
+        const uint32_t* G = gbm + (uint64_t)c * gbm_stride;
+        const uint32_t S = pseg_range_len(pa, pb);
+        const uint32_t r0 = min(pb, pa + (uint32_t)tid * S), r1 = min(pb, r0 + S);
+        const uint32_t w0 = r0 >> 5, w1 = r0 < r1 ? (r1 + 31) >> 5 : w0;
+        const uint32_t cnt = pseg_tokens(G, CODE, w0, w1);
+        const uint32_t incl = wave_incl_scan(cnt, lane);
+        if (lane == 63) misc[wv] = incl;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t base = 0;
+            for (uint32_t h = 0; h < g; ++h) base += G[pseg_count_word(chunk) + h];
+            uint32_t acc = base;
+            for (int i = 0; i < L6R_WAVES; ++i) {
+                const uint32_t t = misc[i];
+                misc[i] = acc;
+                acc += t;
+            }
+            if (pb == n) ntok_out[c] = acc;
+        }
+        __syncthreads();
+        uint32_t ti = misc[wv] + incl - cnt;
+        for (uint32_t w = w0; w < w1; ++w) {
+            uint32_t x = G[w];
+            if (!x) continue;
+            // the word's codes, and its 32 input bytes plus 8 more (a token's
+            // lazy literals may run past the word), in registers
+            const uint32_t pw = w << 5;
+            uint32_t cw[16];
+            codes32_at(CODE, pw, cw);
+            uint64_t sb[5];
+            if (pw + 40 <= n) {
+#pragma unroll
+                for (int i = 0; i < 5; ++i) sb[i] = ld64u(s + pw + 8 * i);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 5; ++i) {
+                    uint64_t v = 0;
+                    for (uint32_t k = 0; k < 8; ++k)
+                        if (pw + 8 * i + k < n) v |= (uint64_t)s[pw + 8 * i + k] << (8 * k);
+                    sb[i] = v;
+                }
+            }
+            auto byte_at = [&](uint32_t o) -> uint32_t {  // o < 40
+                const uint64_t v = o < 8 ? sb[0] : (o < 16 ? sb[1] : (o < 24 ? sb[2] : (o < 32 ? sb[3] : sb[4])));
+                return (uint32_t)(v >> (8 * (o & 7))) & 0xFFu;
+            };
+            while (x) {
+                const int kk = __builtin_ctz(x);
+                x &= x - 1;
+                const uint32_t o = (uint32_t)kk, p = pw + o;
+                const uint32_t cd = code_in(cw, kk);
+                const uint32_t lits = cd >> 12, step = cd & 0xFFFu;
+                for (uint32_t k = 0; k < lits; ++k) {
+                    if ((ti % BLK_SYMS) == 0 && ti / BLK_SYMS < maxblk) blkpos[(uint64_t)c * maxblk + ti / BLK_SYMS] = p + k;
+                    TK[ti++] = byte_at(o + k);
+                }
+                if ((ti % BLK_SYMS) == 0 && ti / BLK_SYMS < maxblk) blkpos[(uint64_t)c * maxblk + ti / BLK_SYMS] = p + lits;
+                if (step > lits + 1) {
+                    const uint32_t m = ML[p + lits];
+                    TK[ti++] = 0x80000000u | (ml_len(m) << 16) | ml_dist(m);
+                } else {
+                    TK[ti++] = byte_at(o + lits);
+                }
+            }
+        }
         __syncthreads();
     }
 }
@@ -1426,6 +1731,7 @@ hipError_t set_kernel_attributes_l6(size_t lds_max) {
     if (e == hipSuccess) e = l6_allow_lds(&l6_pred_kernel<false>, L6P_LDS);
     if (e == hipSuccess) e = l6_allow_lds(&l6_match_kernel, L6M_LDS);
     if (e == hipSuccess) e = l6_allow_lds(&l6_parse_kernel, L6R_LDS);
+    if (e == hipSuccess) e = l6_allow_lds(&l6_pseg_parse_kernel, L6R_LDS);
     return e;
 }
 
@@ -1455,11 +1761,28 @@ hipError_t launch_l6_front(const Plan& p) {
     const uint32_t dcap = p.cus * 2u;  // two 1024-thread workgroups per CU
     const uint32_t dsegs = l6_segments(p, dcap);
     const unsigned gd = grid_for(p.persistent_grid, p.nchunks * dsegs, dcap);
+    // the parse: one workgroup per chunk, or, for batches of few chunks,
+    // segments of each chunk with the token bitmap in the staging words
+    const uint32_t rcap = p.cus * l6_per_cu(p, l6r_lds(p.chunk), L6R_THREADS);
+    const uint32_t psegs = l6_segments(p, rcap);
+    uint32_t* gbm = psegs > 1 ? p.d_stage : nullptr;
     hipLaunchKernelGGL(l6_decide_kernel, dim3(gd), dim3(L6D_THREADS), 0, p.stream, p.d_desc, p.d_hdr, p.chunk, p.d_tok,
-                       p.d_pd, p.l6, dsegs);
-    const unsigned gr = grid_for(p.persistent_grid, p.nchunks, p.cus * l6_per_cu(p, l6r_lds(p.chunk), L6R_THREADS));
-    hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), l6r_lds(p.chunk), p.stream, p.d_in, p.d_desc, p.d_order,
-                       p.d_hdr, p.chunk, p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos, maxblk, p.l6);
+                       p.d_pd, p.l6, dsegs, gbm, p.stage_words);
+    if (psegs > 1) {
+        const unsigned gp = grid_for(false, p.nchunks * psegs, rcap);
+        hipLaunchKernelGGL(l6_pseg_parse_kernel, dim3(gp), dim3(L6R_THREADS), l6r_lds(p.chunk), p.stream, p.d_desc,
+                           p.d_hdr, p.chunk, p.d_pd, gbm, p.stage_words, psegs);
+        hipLaunchKernelGGL(l6_pseg_count_kernel, dim3(gp), dim3(L6R_THREADS), 0, p.stream, p.d_desc, p.d_hdr, p.chunk,
+                           p.d_pd, gbm, p.stage_words, psegs);
+        hipLaunchKernelGGL(l6_pseg_write_kernel, dim3(gp), dim3(L6R_THREADS), 0, p.stream, p.d_in, p.d_desc, p.d_hdr,
+                           p.chunk, p.d_tok, p.d_pd, gbm, p.stage_words, psegs, p.d_aux, p.d_ntok, p.d_blkpos,
+                           maxblk);
+    } else {
+        const unsigned gr = grid_for(p.persistent_grid, p.nchunks, rcap);
+        hipLaunchKernelGGL(l6_parse_kernel, dim3(gr), dim3(L6R_THREADS), l6r_lds(p.chunk), p.stream, p.d_in,
+                           p.d_desc, p.d_order, p.d_hdr, p.chunk, p.d_tok, p.d_pd, p.d_aux, p.d_ntok, p.d_blkpos,
+                           maxblk, p.l6);
+    }
     const unsigned gb = grid_for(p.persistent_grid, p.nchunks * maxblk, p.cus * 8u);
     hipLaunchKernelGGL(l6_block_kernel, dim3(gb), dim3(L6B_THREADS), 0, p.stream, p.d_aux, p.d_ntok, p.d_blkpos,
                        p.d_hdr, p.chunk, maxblk, reinterpret_cast<BlockRec*>(p.d_blocks));
