@@ -24,7 +24,8 @@
 //                     incoming offset mod 8), checksum combine, framing; a
 //                     stream piece starts at its stream's bit offset and has
 //                     no framing.
-//   assemble_kernel   one workgroup per chunk: shifts the chunk's bits to its
+//   assemble_kernel   one workgroup per chunk (or per slice of a chunk's
+//                     output when a batch has few chunks): shifts the bits to its
 //                     offset in the output (stored chunks copy the input).
 #include "deflate_gpu.h"
 #include "device_common.h"
@@ -1521,7 +1522,9 @@ __device__ __forceinline__ ChunkView make_view(const uint8_t* src, const ChunkDe
 }
 
 // Each chunk's bits, shifted to its offset in its buffer's output (grid-stride
-// over chunks; the byte a chunk shares with the next is written by the first).
+// over chunk x slice: a batch of few chunks cuts each chunk's aligned output
+// words into `slices` runs; the byte a chunk shares with the next is written
+// by the first).
 __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict__ src,
                                                        const ChunkDesc* __restrict__ desc,
                                                        const BufDesc* __restrict__ bufs,
@@ -1529,9 +1532,10 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
                                                        const uint32_t* __restrict__ stage, uint64_t stage_words,
                                                        const ChunkInfo* __restrict__ info,
                                                        const Result* __restrict__ res, uint8_t* __restrict__ out_arena,
-                                                       uint32_t hdr_bytes) {
-    const uint32_t nchunks = hdr->nchunks;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+                                                       uint32_t hdr_bytes, uint32_t slices) {
+    const uint32_t items = hdr->nchunks * slices;
+    for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const uint32_t c = it / slices, sl = it % slices;
         const uint32_t bi = desc[c].buf & CHUNK_BUF;
         if (res[bi].total_bytes == 0) continue;
         const ChunkView cv = make_view(src, desc, stage, stage_words, info, c);
@@ -1546,7 +1550,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
         // a stream piece's first chunk starts inside byte 0, below the held
         // bits of the stream (every chunk is at least 10 bits, so byte 0
         // holds no bits of a second chunk)
-        if ((piece & PIECE) && c == bufs[bi].chunk0 && (o & 7) && threadIdx.x == 0)
+        if ((piece & PIECE) && c == bufs[bi].chunk0 && (o & 7) && threadIdx.x == 0 && sl == 0)
             dst[o >> 3] = (uint8_t)(((piece >> 8) & ((1u << (o & 7)) - 1u)) |
                                     (chunk_bits(cv, 0, 8 - (uint32_t)(o & 7)) << (o & 7)));
         auto byte_at = [&](uint64_t b) -> uint32_t {
@@ -1562,12 +1566,16 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
         const uint64_t wb0 = min(b1, b0 + ((4 - (a0 & 3)) & 3));   // first byte of the aligned run
         const uint64_t nwords = (b1 - wb0) >> 2;
         const uint64_t wb1 = wb0 + 4 * nwords;                       // first byte after it
-        for (uint64_t b = b0 + threadIdx.x; b < wb0; b += 256) dst[b] = (uint8_t)byte_at(b);
-        for (uint64_t b = wb1 + threadIdx.x; b < b1; b += 256) dst[b] = (uint8_t)byte_at(b);
+        if (sl == 0) {
+            for (uint64_t b = b0 + threadIdx.x; b < wb0; b += 256) dst[b] = (uint8_t)byte_at(b);
+            for (uint64_t b = wb1 + threadIdx.x; b < b1; b += 256) dst[b] = (uint8_t)byte_at(b);
+        }
         // a stored chunk's bytes are byte-aligned after its first header
         // bits and pad: output byte b is byte b - jb0 of that part
         const uint64_t jb0 = (o + 3 + ((8 - ((o + 3) & 7)) & 7)) >> 3;
-        for (uint64_t w = threadIdx.x; w < nwords; w += 256) {
+        const uint64_t per = (nwords + slices - 1) / slices;
+        const uint64_t wlo = min(nwords, per * sl), whi = min(nwords, wlo + per);
+        for (uint64_t w = wlo + threadIdx.x; w < whi; w += 256) {
             const uint64_t b = wb0 + 4 * w;
             uint32_t v;
             if (!cv.stored && 8 * (b + 4) <= e) {
@@ -1766,8 +1774,11 @@ hipError_t launch_parallel_compress(const Plan& p) {
     }
     record(p.ev[3], s);
     const uint32_t hdr = p.format == 1 ? 2u : (p.format == 2 ? 10u : 0u);
-    hipLaunchKernelGGL(assemble_kernel, dim3(grid_for(p.persistent_grid, p.nchunks, light)), dim3(256), 0, s, p.d_in,
-                       p.d_desc, p.d_bufs, p.d_hdr, p.d_stage, p.stage_words, p.d_info, p.d_result, p.d_out, hdr);
+    // a batch of few chunks splits each chunk's output over several workgroups
+    const uint32_t slices = (p.persistent_grid || p.nchunks == 0) ? 1u : max(1u, min(32u, light / p.nchunks));
+    hipLaunchKernelGGL(assemble_kernel, dim3(grid_for(p.persistent_grid, p.nchunks * slices, light)), dim3(256), 0, s,
+                       p.d_in, p.d_desc, p.d_bufs, p.d_hdr, p.d_stage, p.stage_words, p.d_info, p.d_result, p.d_out,
+                       hdr, slices);
     record(p.ev[4], s);
     return hipGetLastError();
 }
