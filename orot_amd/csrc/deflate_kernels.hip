@@ -597,11 +597,15 @@ constexpr uint32_t LC_MAX_CHUNK = 128u * 1024u;
 constexpr uint32_t LC_BM_WORDS = LC_MAX_CHUNK / 32;
 constexpr uint32_t LC_REGION0 = LC_MAX_CHUNK;  // bytes: the chunk, then length codes
 
-__device__ __forceinline__ uint32_t len_code8(uint32_t m1) {
-    if (m1 == 0) return 0;
+// u8 code of a position: its parse step, and for a literal the length of its
+// fixed-Huffman code.  0: literal below 144 (8 bits); 1: literal 144..255 (9
+// bits); 2..253: match of length code + 1 (3..254); 254: 258; 255: 255..257,
+// with length - 255 in the position's byte of the side array.
+__device__ __forceinline__ uint32_t len_code8(uint32_t m1, uint32_t lit) {
+    if (m1 == 0) return lit >= 144u ? 1u : 0u;
     if (m1 == 258) return 254;
-    if (m1 >= 256) return 255;
-    return m1 - 2;
+    if (m1 >= 255) return 255;
+    return m1 - 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -623,23 +627,26 @@ __device__ __forceinline__ uint32_t len_code8(uint32_t m1) {
 //       C recovered and first byte compared at C, lz77.cpp:178-184) and
 //       greedy match length
 //       (lz77.cpp:163-191), the verified extent carried along a diagonal,
-//       all compares in LDS; T[p] = literal | length << 8 | distance << 17.
-//   K   the LDS bytes are replaced by u8 length codes (0 literal,
-//       1..253 = len - 2, 254 = 258, 255 = 256/257 read back from T), which
-//       A2 left in each group's slot of the candidate codes.
+//       all compares in LDS.  Output per position: a u8 code (len_code8),
+//       to HBM in LbLayout order; the candidate codes stay as they are, since
+//       a match's distance is its candidate code's low 15 bits.
+//   K   the LDS bytes are replaced by the u8 codes.
 //   P   parse barriers: q is a barrier when no token starting before q ends
 //       after q (max over p < q of p + step(p) == q); every greedy parse
 //       visits every barrier.  Lane j parses exactly from the first barrier
 //       of its 128-position range to the first barrier of a later range
 //       (lz77.cpp:378-406), counting fixed-Huffman bits.
-//   C   block-wide scan, stored / fixed choice (deflate_block.cpp:414-421),
-//       emission into the chunk's staging bitstream.
+//   C   per-range fixed-Huffman bit counts from the codes in LDS and the
+//       distances, block-wide scan, stored / fixed choice
+//       (deflate_block.cpp:414-421), emission into the chunk's staging
+//       bitstream (literals read from the input, codes from HBM).
 // ---------------------------------------------------------------------------
 static_assert(LC_MAX_CHUNK / LC_THREADS <= 128, "ranges of at most 136 positions span at most five bitmap words");
 constexpr uint32_t LB_BM_OFF = LC_REGION0 / 4;               // words: the token bitmap
 constexpr uint32_t LB_FB_OFF = LB_BM_OFF + LC_BM_WORDS;       // first barrier per range
 constexpr uint32_t LB_MISC_OFF = LB_FB_OFF + LC_THREADS;
-constexpr uint32_t LB_LDS_BYTES = (LB_MISC_OFF + 64u) * 4u;
+constexpr uint32_t LB_LTAB_OFF = LB_MISC_OFF + 64u;  // fixed-Huffman length symbols by u8 code
+constexpr uint32_t LB_LDS_BYTES = (LB_LTAB_OFF + 256u) * 4u;
 
 constexpr uint16_t LA_NONE = 0xFFFFu;
 
@@ -841,15 +848,45 @@ __device__ __forceinline__ uint64_t lds_u64(const uint32_t* L, uint32_t off) {
 __device__ __forceinline__ uint32_t lds_u8(const uint32_t* L, uint32_t off) {
     return (L[off >> 2] >> (8 * (off & 3u))) & 0xFFu;
 }
-__device__ __forceinline__ uint32_t code_step(uint32_t b, const uint32_t* T, const LbLayout& L, uint32_t pos) {
-    if (b == 0) return 1;
-    if (b < 254) return b + 2;
-    if (b == 254) return 258;
-    return tok_len(T[L.index(pos)]);
+// Fixed-Huffman length symbol of u8 code b >= 2 (len_code8), LSB-first: the
+// code bits, then the extra bits, in the low 16 bits; the bit count in bits
+// 16..23.  Code 255 is length 255 (symbol 284, extra 28); lengths 256 and 257
+// add 1 or 2 to the extra value, which keeps the bit count.
+__device__ __forceinline__ uint32_t len_entry(uint32_t b) {
+    if (b < 2u) return 0u;
+    const uint32_t L = b == 254u ? 258u : (b == 255u ? 255u : b + 1u);
+    const LenSym ls = len_sym(L);
+    const uint32_t sym = 257u + ls.code;
+    const uint32_t cl = sym < 280u ? 7u : 8u;
+    const uint32_t code = sym < 280u ? sym - 256u : 0xC0u + (sym - 280u);
+    const uint32_t rc = __builtin_bitreverse32(code) >> (32u - cl);
+    return (rc | (ls.extra << cl)) | ((cl + ls.ebits) << 16);
 }
 
-// Fixed-Huffman bit count of the tokens whose start bit is set in
-// bm over [a, b); T read 8 tokens at a time.
+// Fixed-Huffman distance symbol (5-bit code, then the extra bits) of d in
+// 1..32768, without branches.
+__device__ __forceinline__ void fixed_dist(uint32_t d, uint32_t& v, uint32_t& len) {
+    const uint32_t x = d - 1u;
+    const bool small = d <= 4u;
+    const uint32_t e = small ? 0u : 30u - (uint32_t)__builtin_clz(x);
+    const uint32_t code = small ? x : 2u * e + 2u + ((x >> e) & 1u);
+    v = (__builtin_bitreverse32(code) >> 27) | ((x & ((1u << e) - 1u)) << 5);
+    len = 5u + e;
+}
+
+// Bit j set where byte j of w is zero.
+__device__ __forceinline__ uint32_t zero_bytes8(uint64_t w) {
+    const uint64_t lo7 = 0x7F7F7F7F7F7F7F7Full;
+    const uint64_t y = ~(((w & lo7) + lo7) | w | lo7);
+    return (uint32_t)(((y >> 7) * 0x0102040810204080ull) >> 56);
+}
+
+// Parse step of the position `pos` with u8 code b (len_code8); XS is the side
+// array of lengths 255..257.
+__device__ __forceinline__ uint32_t code_step(uint32_t b, const uint8_t* XS, const LbLayout& L, uint32_t pos) {
+    if (b == 255u) return 255u + XS[L.index(pos)];
+    return b <= 1u ? 1u : (b == 254u ? 258u : b + 1u);
+}
 
 template <int HB>
 __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
@@ -858,10 +895,11 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     ChunkInfo* __restrict__ info) {
     const uint32_t n = desc[c].len;
     const uint8_t* s = src + desc[c].in_off;
-    // candidate codes from l1_candidate_kernel (then, per group, the group's
-    // length codes) and token words, LbLayout order
-    uint16_t* Q = cand + (uint64_t)c * lb_tok_stride(chunk);
-    uint32_t* T = tokw + (uint64_t)c * lb_tok_stride(chunk);
+    // candidate codes from l1_candidate_kernel, the u8 codes written by A2
+    // and the side array of lengths 255..257, all in LbLayout order
+    const uint16_t* Q = cand + (uint64_t)c * lb_tok_stride(chunk);
+    uint8_t* LCB = reinterpret_cast<uint8_t*>(tokw + (uint64_t)c * lb_tok_stride(chunk));
+    uint8_t* XS = LCB + lb_tok_stride(chunk);
     const LbLayout L(n);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     uint32_t* D = smem;                         // chunk bytes, then length codes
@@ -897,8 +935,9 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     const uint32_t nh = n >= 4 ? n - 3 : 0;
     const uint32_t S = L.S;
     const uint32_t r0 = min(n, (uint32_t)tid * S), r1 = min(n, r0 + S);
-    // this range's group holding position q (r0 <= q < r1, q - r0 a multiple of 8)
-    auto tg = [&](uint32_t q) { return T + L.group((uint32_t)tid, (q - r0) >> 3); };
+    // layout index of the group of this range holding position q (r0 <= q < r1,
+    // q - r0 a multiple of 8)
+    auto gi = [&](uint32_t q) { return L.group((uint32_t)tid, (q - r0) >> 3); };
     {
         const uint32_t lim = min(n, r1 + 257u);
         uint32_t cd = 0, ce = 0;
@@ -919,7 +958,9 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             const uint32_t lw[4] = {lv.x, lv.y, lv.z, lv.w};
             if (p0 + 8 < r1) nx = load_loc(p0 + 8);
             const uint64_t w0 = lds_u64(D, p0), w1 = lds_u64(D, p0 + 8);
-            uint32_t outw[8];
+            uint32_t cw[2] = {0, 0};  // the group's u8 codes
+            uint32_t xl = 0;          // 2 bits per position: length - 255 where it is 255..257
+            bool big = false;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const uint32_t p = p0 + k;
@@ -968,32 +1009,29 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                     L = min(L, cap);
                     m1 = L >= 3 ? L : 0u;
                 }
-                outw[k] = lit | (m1 << 8) | ((m1 ? d : 0u) << 17);
+                // a match's distance is its candidate code's low 15 bits
+                // (d == e & 0x7FFF whenever m1 != 0), which C reads back
+                big |= m1 >= 255u;
+                xl |= (m1 >= 255u ? m1 - 255u : 0u) << (2 * k);
+                cw[k >> 2] |= len_code8(m1, lit) << (8 * (k & 3));
             }
-            uint32_t* g = tg(p0);
-            if (p0 + 8 <= r1) {
-                *reinterpret_cast<uint4*>(g) = make_uint4(outw[0], outw[1], outw[2], outw[3]);
-                *reinterpret_cast<uint4*>(g + 4) = make_uint4(outw[4], outw[5], outw[6], outw[7]);
-            } else {
+            if (big) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
-                    if (p0 + k < r1) g[k] = outw[k];
+                    if (((cw[k >> 2] >> (8 * (k & 3))) & 0xFFu) == 255u) XS[L.index(p0 + k)] = (uint8_t)((xl >> (2 * k)) & 3u);
             }
-            // the group's u8 length codes over its (already read) candidate
-            // codes, for K
-            uint32_t cw[2] = {0, 0};
-#pragma unroll
-            for (int k = 0; k < 8; ++k) cw[k >> 2] |= len_code8(tok_len(outw[k])) << (8 * (k & 3));
-            *reinterpret_cast<uint2*>(Q + L.group((uint32_t)tid, (p0 - r0) >> 3)) = make_uint2(cw[0], cw[1]);
+            // codes of a last group's positions past r1 (literals) are never
+            // read: no token starts there
+            *reinterpret_cast<uint2*>(LCB + gi(p0)) = make_uint2(cw[0], cw[1]);
         }
     }
     __syncthreads();
 
-    // ---- K: the length codes A2 left per group, to LDS over the bytes, 8 at
-    // a time in range order (codes of a last group's positions past n land
+    // ---- K: the u8 codes A2 wrote per group, to LDS over the bytes, 8 at a
+    // time in range order (codes of a last group's positions past n land
     // below LC_REGION0 and are never read); bitmap cleared.
     for (uint32_t p0 = r0; p0 < r1; p0 += 8)
-        *reinterpret_cast<uint2*>(code + p0) = *reinterpret_cast<const uint2*>(Q + L.group((uint32_t)tid, (p0 - r0) >> 3));
+        *reinterpret_cast<uint2*>(code + p0) = *reinterpret_cast<const uint2*>(LCB + gi(p0));
     for (uint32_t w = tid; w < ((n + 31) >> 5); w += LC_THREADS) bm[w] = 0;
     uint32_t* run258 = misc + 32;  // bit t: every position of range t steps 258
     if (tid < (int)(LC_THREADS / 32)) run258[tid] = 0;
@@ -1013,7 +1051,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint32_t b = (cw >> (8 * k)) & 0xFFu;
-                    const uint32_t st = b == 0u ? 1u : (b == 254u ? 258u : b + 2u);
+                    const uint32_t st = b <= 1u ? 1u : (b == 254u ? 258u : b + 1u);
                     const bool valid = p + k < r1;
                     all258 &= !valid || b == 254u;
                     runmax = max(runmax, valid ? p + k + st : 0u);
@@ -1024,7 +1062,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                     if (p + k < r1) {
                         const uint32_t b = (cw >> (8 * k)) & 0xFFu;
                         all258 &= b == 254u;
-                        runmax = max(runmax, p + k + code_step(b, T, L, p + k));
+                        runmax = max(runmax, p + k + code_step(b, XS, L, p + k));
                     }
             }
         }
@@ -1060,7 +1098,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             for (int k = 0; k < 4; ++k) {
                 if (p + k < r1 && first == 0xFFFFFFFFu) {
                     if (carry <= p + k) first = p + k;
-                    else carry = max(carry, p + k + code_step((cw >> (8 * k)) & 0xFFu, T, L, p + k));
+                    else carry = max(carry, p + k + code_step((cw >> (8 * k)) & 0xFFu, XS, L, p + k));
                 }
             }
         }
@@ -1081,7 +1119,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 if (pos >= r1 && fb[L.range_of(pos)] == pos) break;
                 atomicOr(&bm[pos >> 5], 1u << (pos & 31));
                 const uint32_t b = code[pos];
-                pos += code_step(b, T, L, pos);
+                pos += code_step(b, XS, L, pos);
                 if (b != 254u) continue;
                 for (;;) {
                     // all loads first (addresses clamped into the chunk), then
@@ -1121,9 +1159,9 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     const uint32_t w0 = r0 >> 5, nwd = r0 < r1 ? ((r1 + 31) >> 5) - w0 : 0u;
     const uint32_t bw0 = nwd > 0 ? bm[w0] : 0u, bw1 = nwd > 1 ? bm[w0 + 1] : 0u, bw2 = nwd > 2 ? bm[w0 + 2] : 0u,
                    bw3 = nwd > 3 ? bm[w0 + 3] : 0u, bw4 = nwd > 4 ? bm[w0 + 4] : 0u;
-    // token words are streamed in range order, 8 positions (two 16-byte
-    // loads) at a time, skipping groups without a token start; the next
-    // group's loads are issued before the current group is used
+    // groups of 8 positions are streamed in range order, skipping groups
+    // without a token start; the next groups' loads are issued before the
+    // current group is used
     auto group_mask = [&](uint32_t p0) -> uint32_t {
         const uint32_t k = (p0 >> 5) - w0;
         const uint32_t wv32 = k == 0 ? bw0 : (k == 1 ? bw1 : (k == 2 ? bw2 : (k == 3 ? bw3 : bw4)));
@@ -1133,50 +1171,84 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         while (p0 < r1 && group_mask(p0) == 0) p0 += 8;
         return p0;
     };
-    // The groups with token starts, in order, with the token words of two
-    // more in flight.  Every lane loads whole 8-word slots (a group past n
-    // reads slot words whose positions have no start bit) and a lane with
-    // no such group loads slot 0, so the loads carry no branch and the
-    // waits for them are exact.  Unrolled by three over fixed buffers.
-    auto stream = [&](auto&& body) {
-        auto load8 = [&](uint32_t g, uint32_t* v) {
-            const uint32_t* a = g < r1 ? tg(g) : T;
-            const uint4 x = *reinterpret_cast<const uint4*>(a), y = *reinterpret_cast<const uint4*>(a + 4);
-            v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-            v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    // The groups with token starts, in order, with the loads of two more in
+    // flight: the group's eight candidate codes (v[0..3], the distances) and,
+    // for the emission pass, its eight u8 codes (v[4..5]) and input bytes
+    // (v[6..7], the literals).  Every lane loads whole slots (a group past n
+    // reads slot entries whose positions have no start bit) and a lane with
+    // no such group loads group 0, so the loads carry no branch on the
+    // group and the waits for them are exact.  Unrolled by three over fixed
+    // buffers.
+    auto stream = [&](auto emit, auto&& body) {
+        constexpr bool EMIT = decltype(emit)::value;
+        auto load = [&](uint32_t g, uint32_t* v) {
+            const bool in = g < r1;
+            const uint32_t x = in ? gi(g) : 0u;
+            const uint4 q = *reinterpret_cast<const uint4*>(Q + x);
+            v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+            if (EMIT) {
+                const uint2 cb = *reinterpret_cast<const uint2*>(LCB + x);
+                v[4] = cb.x; v[5] = cb.y;
+                const uint32_t pg = in ? g : 0u;
+                uint64_t lt = 0;
+                if (pg + 8 <= n) {
+                    lt = ld64u(s + pg);
+                } else {
+                    for (uint32_t k = 0; k < 8; ++k)
+                        if (pg + k < n) lt |= (uint64_t)s[pg + k] << (8 * k);
+                }
+                v[6] = (uint32_t)lt;
+                v[7] = (uint32_t)(lt >> 32);
+            }
         };
         uint32_t A[8], B[8], C[8];
         uint32_t ga = nwd ? next_group(r0) : r1;
         uint32_t gb = next_group(ga + 8);
-        load8(ga, A);
-        load8(gb, B);
+        load(ga, A);
+        load(gb, B);
         for (;;) {
             if (ga >= r1) break;
             uint32_t gc = next_group(gb + 8);
-            load8(gc, C);
+            load(gc, C);
             body(ga, A);
             ga = gb;
             gb = gc;
             if (ga >= r1) break;
             gc = next_group(gb + 8);
-            load8(gc, A);
+            load(gc, A);
             body(ga, B);
             ga = gb;
             gb = gc;
             if (ga >= r1) break;
             gc = next_group(gb + 8);
-            load8(gc, B);
+            load(gc, B);
             body(ga, C);
             ga = gb;
             gb = gc;
         }
     };
+    // distance of position j (0..7) of a group from its candidate code
+    auto dist_of = [](const uint32_t* v, uint32_t j) {
+        const uint64_t q = j < 4u ? ((uint64_t)v[1] << 32 | v[0]) : ((uint64_t)v[3] << 32 | v[2]);
+        return (uint32_t)(q >> (16u * (j & 3u))) & 0x7FFFu;
+    };
+    const uint32_t* ltab = smem + LB_LTAB_OFF;
+    // Literals are counted by popcounts (8 bits, one more for codes 144..255);
+    // each lane then visits only its own match starts.
     uint32_t mybits = 0;
-    stream([&](uint32_t g, const uint32_t* tv) {
+    stream(std::false_type{}, [&](uint32_t g, const uint32_t* v) {
         const uint32_t m = group_mask(g);
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if ((m >> j) & 1u) mybits += tok_bits(tv[j]);
+        const uint2 c2 = *reinterpret_cast<const uint2*>(code + g);  // the group's u8 codes in LDS
+        const uint64_t cb = (uint64_t)c2.y << 32 | c2.x;
+        const uint32_t lm = zero_bytes8(cb & ~0x0101010101010101ull);  // codes 0 and 1
+        const uint32_t hm = zero_bytes8(cb ^ 0x0101010101010101ull);   // code 1
+        mybits += 8u * (uint32_t)__popc(m & lm) + (uint32_t)__popc(m & hm);
+        for (uint32_t mm = m & ~lm; mm; mm &= mm - 1u) {
+            const uint32_t j = (uint32_t)__builtin_ctz(mm);
+            const uint32_t b = (uint32_t)(cb >> (8u * j)) & 0xFFu;
+            const uint32_t d = dist_of(v, j);
+            mybits += (ltab[b] >> 16) + 5u + (d <= 4u ? 0u : 30u - (uint32_t)__builtin_clz(d - 1u));
+        }
     });
     const uint32_t incl = wave_incl_scan(mybits, lane);
     if (lane == 63) misc[wv] = incl;
@@ -1213,11 +1285,7 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         uint32_t nacc = bitpos & 31u;
         bool partial = nacc != 0;
         uint64_t acc = 0;
-        auto put = [&](uint32_t tkn) {
-            uint32_t v, len;
-            const uint32_t l = tok_len(tkn);
-            if (l) fixed_match(l, tok_dist(tkn), v, len);
-            else fixed_lit(tkn & 0xFFu, v, len);
+        auto put = [&](uint32_t v, uint32_t len) {
             acc |= (uint64_t)v << nacc;
             nacc += len;
             if (nacc >= 32) {
@@ -1232,10 +1300,36 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 nacc -= 32;
             }
         };
-        stream([&](uint32_t g, const uint32_t* tv) {
+        stream(std::true_type{}, [&](uint32_t g, const uint32_t* v) {
             const uint32_t m = group_mask(g);
-            for (int j = 0; j < 8; ++j)
-                if ((m >> j) & 1u) put(tv[j]);
+            const uint64_t cb = (uint64_t)v[5] << 32 | v[4];
+            const uint64_t lt = (uint64_t)v[7] << 32 | v[6];
+            // extra values of lengths 256 and 257 (code 255): 2 bits per position
+            uint32_t xadd = 0;
+            const uint32_t xm = m & zero_bytes8(~cb);
+            for (uint32_t k = xm; k; k &= k - 1u) {
+                const uint32_t j = (uint32_t)__builtin_ctz(k);
+                xadd |= (uint32_t)XS[L.index(g + j)] << (2u * j);
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if ((m >> j) & 1u) {
+                    const uint32_t b = (uint32_t)(cb >> (8u * j)) & 0xFFu;
+                    uint32_t val, len;
+                    if (b <= 1u) {
+                        const uint32_t byte = (uint32_t)(lt >> (8u * j)) & 0xFFu;
+                        len = 8u + b;  // code 1: literal 144..255
+                        val = __builtin_bitreverse32(byte + (byte < 144u ? 0x30u : 0x100u)) >> (32u - len);
+                    } else {
+                        const uint32_t ent = ltab[b] + (((xadd >> (2u * j)) & 3u) << 8);
+                        uint32_t dv, dl;
+                        fixed_dist(dist_of(v, j), dv, dl);
+                        const uint32_t ml = ent >> 16;
+                        val = (ent & 0xFFFFu) | (dv << ml);
+                        len = ml + dl;
+                    }
+                    put(val, len);
+                }
         });
         if (nacc) atomicOr(&out[wi], (uint32_t)acc);
     }
@@ -1250,6 +1344,7 @@ __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* _
     uint32_t chunk, uint16_t* __restrict__ cand, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage,
     uint64_t stage_words, ChunkInfo* __restrict__ info) {
     extern __shared__ uint32_t smem[];
+    for (uint32_t i = threadIdx.x; i < 256u; i += LC_THREADS) smem[LB_LTAB_OFF + i] = len_entry(i);
     for (;;) {
         const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
         if (k == NO_ITEM) break;
