@@ -602,10 +602,9 @@ constexpr uint32_t LC_REGION0 = LC_MAX_CHUNK;  // bytes: the chunk, then length 
 // bits); 2..253: match of length code + 1 (3..254); 254: 258; 255: 255..257,
 // with length - 255 in the position's byte of the side array.
 __device__ __forceinline__ uint32_t len_code8(uint32_t m1, uint32_t lit) {
-    if (m1 == 0) return lit >= 144u ? 1u : 0u;
-    if (m1 == 258) return 254;
-    if (m1 >= 255) return 255;
-    return m1 - 1;
+    uint32_t c = m1 >= 255u ? 255u : m1 - 1u;
+    c = m1 == 258u ? 254u : c;
+    return m1 == 0u ? (uint32_t)(lit >= 144u) : c;
 }
 
 // ---------------------------------------------------------------------------
@@ -836,14 +835,17 @@ __global__ __launch_bounds__(64 * W) void l1_candidate_kernel(const uint8_t* __r
     }
 }
 
+// Unaligned LDS reads: gfx950 serves them as one ds_read_b32 / ds_read_b64
+// at any byte address.
 __device__ __forceinline__ uint32_t lds_u32(const uint32_t* L, uint32_t off) {
-    const uint32_t i = off >> 2, k = off & 3u;
-    return __builtin_amdgcn_alignbyte(L[i + 1], L[i], k);
+    uint32_t v;
+    __builtin_memcpy(&v, reinterpret_cast<const uint8_t*>(L) + off, 4);
+    return v;
 }
 __device__ __forceinline__ uint64_t lds_u64(const uint32_t* L, uint32_t off) {
-    const uint32_t i = off >> 2, k = off & 3u;
-    const uint32_t w0 = L[i], w1 = L[i + 1], w2 = L[i + 2];
-    return (uint64_t)__builtin_amdgcn_alignbyte(w1, w0, k) | ((uint64_t)__builtin_amdgcn_alignbyte(w2, w1, k) << 32);
+    uint64_t v;
+    __builtin_memcpy(&v, reinterpret_cast<const uint8_t*>(L) + off, 8);
+    return v;
 }
 __device__ __forceinline__ uint32_t lds_u8(const uint32_t* L, uint32_t off) {
     return (L[off >> 2] >> (8 * (off & 3u))) & 0xFFu;
