@@ -683,10 +683,10 @@ constexpr uint32_t lb_tok_stride(uint32_t chunk) { return ((chunk + 7u) & ~7u) +
 // For the last, C is the nearest of p - dd - 32768 k (k >= 1) whose hash is
 // p's: no same-hash position lies between C and p.
 __device__ __forceinline__ uint32_t cand_code(uint32_t p, uint32_t C1) {
-    if (C1 <= 1) return 0;
-    const uint32_t dist = p - (C1 - 1u), dd = dist & 0x7FFFu;
-    if (dist == dd) return dd;
-    return dd ? (0x8000u | dd) : 0u;
+    const uint32_t dist = p + 1u - C1, dd = dist & 0x7FFFu;
+    const uint32_t far = dd ? (0x8000u | dd) : 0u;
+    const uint32_t code = dist == dd ? dd : far;
+    return C1 <= 1u ? 0u : code;
 }
 
 // LDS of the candidate kernel with HB-bit hashes and W waves: a table per
@@ -796,7 +796,8 @@ __global__ __launch_bounds__(64 * W) void l1_candidate_kernel(const uint8_t* __r
                 const uint32_t p = p0 + 64u * j + (uint32_t)lane;
                 const bool valid = p < b;
                 if (valid && wv >= 1 && res[j] == 0) first[(wv - 1) * LA_TAB + h[j]] = (uint16_t)(p - a);  // one per hash
-                Q[valid ? L.index(p) : spare] = (uint16_t)cand_code(p, res[j]);
+                const uint32_t qi = L.index(p);
+                Q[valid ? qi : spare] = (uint16_t)cand_code(p, res[j]);
             }
         };
         uint32_t w0[8], w1[8], w2[8], w3[8];
