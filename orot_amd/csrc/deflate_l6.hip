@@ -40,6 +40,7 @@ constexpr uint32_t L6_TAB = 1u << L6_HB;
 constexpr int L6P_THREADS = 512;
 constexpr int L6P_W = 8;
 constexpr uint32_t L6P_LDS = L6P_W * L6_TAB * 4u;  // 128 KiB
+constexpr uint32_t L6P_UNRESOLVED = 0xFFFFu;  // chain step to take from an earlier sub-range (steps are <= 32768)
 
 constexpr int L6M_THREADS = 1024;
 constexpr uint32_t L6M_TILE = 4096;
@@ -99,7 +100,7 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t seg, uint32_t 
     const uint32_t n = desc[c].len + H;
     const uint8_t* s = src + desc[c].in_off - H;
     const uint64_t cp = l6_stride(chunk);
-    uint32_t* T = tokw + (uint64_t)c * cp;
+    (void)tokw;
     uint16_t* PD = pd + (uint64_t)c * cp;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const uint32_t nh = n >= 4 ? n - 3 : 0;
@@ -114,8 +115,9 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t seg, uint32_t 
         for (uint32_t i = lane; i < L6_TAB; i += 64) tab[i] = 0;
         const uint32_t a = lo + (uint32_t)wv * SR, b = min(a + SR, he);
         // Window loads of three more steps stay in flight, and every lane
-        // stores (lanes past the sub-range on position n - 1, which the
-        // resolution below never reads from T), so the waits are exact.
+        // stores (lanes outside the segment's part of the sub-range on
+        // position n - 1, whose chain step nothing reads: walks start at
+        // n - 4 or before and go back), so the waits are exact.
         auto load8 = [&](uint32_t q0, uint32_t* w) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) w[j] = ld32u(s + min(q0 + 64u * j + (uint32_t)lane, b - 1));
@@ -151,11 +153,16 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t seg, uint32_t 
                     }
                 }
             }
-            // history positions (below pa) belong to another segment's T
+            // the chain step, 0 outside 32 KiB; a position with no earlier
+            // occurrence in a sub-range after the first is marked for the
+            // resolution below.  History positions (below pa) belong to
+            // another segment.
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const uint32_t p = p0 + 64u * j + (uint32_t)lane;
-                T[p < b && p >= pa ? p : n - 1] = res[j];
+                const uint32_t dist = p + 1u - res[j];
+                const uint32_t v = res[j] ? (dist <= 32768u ? dist : 0u) : (wv > 0 ? L6P_UNRESOLVED : 0u);
+                PD[p < b && p >= pa ? p : n - 1] = (uint16_t)v;
             }
         };
         uint32_t w0[8], w1[8], w2[8], w3[8];
@@ -179,25 +186,45 @@ __device__ __forceinline__ void l6_pred_body(uint32_t c, uint32_t seg, uint32_t 
         }
     }
     __syncthreads();
-    // first occurrences of a sub-range take the latest earlier sub-range's entry
-    for (uint32_t p = pa + tid; p < pb; p += L6P_THREADS) {
-        uint32_t step = 0;
-        if (p < nh) {
-            uint32_t C1 = T[p];
-            const uint32_t w = (p - lo) / SR;
-            if (C1 == 0 && w > 0) {
+    // First occurrences of a sub-range take the latest earlier sub-range's
+    // entry; positions past the last hashed one get step 0.  Eight
+    // positions per thread and pass, read with one 16-byte load (pa and p0
+    // are multiples of 8), written back where they lie in the segment.
+    for (uint32_t p0 = pa + 8u * (uint32_t)tid; p0 < pb; p0 += 8u * L6P_THREADS) {
+        const uint4 v4 = *reinterpret_cast<const uint4*>(PD + p0);
+        uint32_t w[4] = {v4.x, v4.y, v4.z, v4.w};
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t p = p0 + (uint32_t)k;
+            const uint32_t cur = (w[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+            any |= p >= nh || cur == L6P_UNRESOLVED;
+        }
+        if (!any) continue;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t p = p0 + (uint32_t)k;
+            uint32_t step = (w[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+            if (p >= nh) {
+                step = 0;
+            } else if (step == L6P_UNRESOLVED) {
+                step = 0;
                 const uint32_t h = hash4<L6_HB>(ld32u(s + p));
-                for (int w2 = (int)w - 1; w2 >= 0; --w2) {
-                    const uint32_t u = smem[(uint32_t)w2 * L6_TAB + h];
-                    if (u) {
-                        C1 = u;
+                for (int w2 = (int)((p - lo) / SR) - 1; w2 >= 0; --w2) {
+                    const uint32_t C1 = smem[(uint32_t)w2 * L6_TAB + h];
+                    if (C1) {
+                        if (p - (C1 - 1u) <= 32768u) step = p - (C1 - 1u);
                         break;
                     }
                 }
             }
-            if (C1 && p - (C1 - 1u) <= 32768u) step = p - (C1 - 1u);
+            w[k >> 1] = (w[k >> 1] & ~(0xFFFFu << (16 * (k & 1)))) | (step << (16 * (k & 1)));
         }
-        PD[p] = (uint16_t)step;
+        if (p0 + 8u <= pb) {
+            *reinterpret_cast<uint4*>(PD + p0) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+            for (uint32_t k = 0; p0 + k < pb; ++k) PD[p0 + k] = (uint16_t)((w[k >> 1] >> (16 * (k & 1))) & 0xFFFFu);
+        }
     }
 }
 
