@@ -357,11 +357,11 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t seg, uint32_t
                     const bool ok = a.live && a.k < max_chain && st[i] != 0 && a.x - q[i] <= 32768u;
                     a.live = ok;
                     any |= ok;
-                    if (ok) {
-                        ++a.k;
-                        a.cur = q[i];
-                        a.ri = a.ri >= st[i] ? a.ri - st[i] : a.ri + L6M_PRING - st[i];  // st <= 32768 < L6M_PRING
-                    }
+                    // a dead walk's cursor is never read again, so the step
+                    // is taken without a branch
+                    ++a.k;
+                    a.cur = q[i];
+                    a.ri = a.ri >= st[i] ? a.ri - st[i] : a.ri + L6M_PRING - st[i];  // st <= 32768 < L6M_PRING
                     // addresses masked to 0 for dead walks, kept opaque so the
                     // select is not turned back into branches around the reads
                     uint32_t a0 = q[i] & (0u - (uint32_t)ok);
