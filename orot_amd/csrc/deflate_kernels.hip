@@ -1589,6 +1589,32 @@ __device__ __forceinline__ bool stored_word(const ChunkView& cv, uint32_t j, uin
     return true;
 }
 
+// Stored chunk: the sixteen output bytes j..j+15 of the byte-aligned part
+// when they are all data bytes of one piece (as stored_word).
+__device__ __forceinline__ bool stored_unit(const ChunkView& cv, uint32_t j, uint32_t* v) {
+    const uint32_t L = cv.len;
+    const uint32_t L0 = min(L, 65535u);
+    uint32_t q, Lk, dbase;
+    if (j < 4u + L0) {
+        q = j + 1u;
+        Lk = L0;
+        dbase = 0;
+    } else {
+        const uint32_t r = j - 4u - L0;
+        const uint32_t k = 1u + r / 65540u;
+        q = r - (k - 1u) * 65540u;
+        Lk = min(L - k * 65535u, 65535u);
+        dbase = k * 65535u;
+    }
+    if (q < 5u || q + 16u > 5u + Lk) return false;
+    const uint64_t x = ld64u(cv.raw + dbase + q - 5u), y = ld64u(cv.raw + dbase + q + 3u);
+    v[0] = (uint32_t)x;
+    v[1] = (uint32_t)(x >> 32);
+    v[2] = (uint32_t)y;
+    v[3] = (uint32_t)(y >> 32);
+    return true;
+}
+
 // k (<= 8) bits of the chunk's bitstream starting at relative bit r.
 __device__ __forceinline__ uint32_t chunk_bits(const ChunkView& cv, uint64_t r, uint32_t k) {
     const uint32_t mask = (1u << k) - 1u;
@@ -1658,33 +1684,44 @@ __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict
             if (k < 8 && has_next) v |= chunk_bits(nv, 0, 8 - k) << k;
             return v;
         };
-        // bytes before the first and after the last 4-byte-aligned output word
-        // one at a time; the aligned words in between with one store each
+        // bytes before the first and after the last 16-byte-aligned output
+        // unit one at a time; the aligned units in between with one 16-byte
+        // store each
         const uint64_t a0 = (uint64_t)(uintptr_t)(dst + b0);
-        const uint64_t wb0 = min(b1, b0 + ((4 - (a0 & 3)) & 3));   // first byte of the aligned run
-        const uint64_t nwords = (b1 - wb0) >> 2;
-        const uint64_t wb1 = wb0 + 4 * nwords;                       // first byte after it
+        const uint64_t ub0 = min(b1, b0 + ((16 - (a0 & 15)) & 15));  // first byte of the aligned run
+        const uint64_t nunits = (b1 - ub0) >> 4;
+        const uint64_t ub1 = ub0 + 16 * nunits;                         // first byte after it
         if (sl == 0) {
-            for (uint64_t b = b0 + threadIdx.x; b < wb0; b += 256) dst[b] = (uint8_t)byte_at(b);
-            for (uint64_t b = wb1 + threadIdx.x; b < b1; b += 256) dst[b] = (uint8_t)byte_at(b);
+            for (uint64_t b = b0 + threadIdx.x; b < ub0; b += 256) dst[b] = (uint8_t)byte_at(b);
+            for (uint64_t b = ub1 + threadIdx.x; b < b1; b += 256) dst[b] = (uint8_t)byte_at(b);
         }
         // a stored chunk's bytes are byte-aligned after its first header
         // bits and pad: output byte b is byte b - jb0 of that part
         const uint64_t jb0 = (o + 3 + ((8 - ((o + 3) & 7)) & 7)) >> 3;
-        const uint64_t per = (nwords + slices - 1) / slices;
-        const uint64_t wlo = min(nwords, per * sl), whi = min(nwords, wlo + per);
-        for (uint64_t w = wlo + threadIdx.x; w < whi; w += 256) {
-            const uint64_t b = wb0 + 4 * w;
-            uint32_t v;
-            if (!cv.stored && 8 * (b + 4) <= e) {
+        const uint64_t per = (nunits + slices - 1) / slices;
+        const uint64_t ulo = min(nunits, per * sl), uhi = min(nunits, ulo + per);
+        for (uint64_t u = ulo + threadIdx.x; u < uhi; u += 256) {
+            const uint64_t b = ub0 + 16 * u;
+            uint32_t v[4];
+            if (!cv.stored && 8 * (b + 16) <= e) {
+                // five staging words hold the unit's 128 bits
                 const uint64_t r = 8 * b - o;
-                const uint64_t wi = r >> 5;
-                const uint64_t x = (uint64_t)cv.stg[wi] | ((uint64_t)cv.stg[wi + 1] << 32);
-                v = (uint32_t)(x >> (r & 31));
-            } else if (!(cv.stored && 8 * (b + 4) <= e && b >= jb0 && stored_word(cv, (uint32_t)(b - jb0), v))) {
-                v = byte_at(b) | (byte_at(b + 1) << 8) | (byte_at(b + 2) << 16) | (byte_at(b + 3) << 24);
+                const uint32_t* w = cv.stg + (r >> 5);
+                const uint32_t sh = (uint32_t)(r & 31);
+                const uint32_t s0 = w[0], s1 = w[1], s2 = w[2], s3 = w[3], s4 = w[4];
+                v[0] = __builtin_amdgcn_alignbit(s1, s0, sh);
+                v[1] = __builtin_amdgcn_alignbit(s2, s1, sh);
+                v[2] = __builtin_amdgcn_alignbit(s3, s2, sh);
+                v[3] = __builtin_amdgcn_alignbit(s4, s3, sh);
+            } else if (!(cv.stored && 8 * (b + 16) <= e && b >= jb0 && stored_unit(cv, (uint32_t)(b - jb0), v))) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint64_t bk = b + 4 * k;
+                    if (!(cv.stored && 8 * (bk + 4) <= e && bk >= jb0 && stored_word(cv, (uint32_t)(bk - jb0), v[k])))
+                        v[k] = byte_at(bk) | (byte_at(bk + 1) << 8) | (byte_at(bk + 2) << 16) | (byte_at(bk + 3) << 24);
+                }
             }
-            *reinterpret_cast<uint32_t*>(dst + b) = v;
+            *reinterpret_cast<uint4*>(dst + b) = make_uint4(v[0], v[1], v[2], v[3]);
         }
     }
 }
