@@ -630,6 +630,25 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
     }
     __syncthreads();
     uint32_t ti = misc[wv] + incl - cnt;
+    // Tokens go out four at a time as one 16-byte store once ti is a multiple
+    // of 4; the tokens of a group this range shares with its neighbours are
+    // stored one by one.
+    uint32_t tb0 = 0, tb1 = 0, tb2 = 0, tb3 = 0;
+    bool grouped = (ti & 3u) == 0;
+    auto put = [&](uint32_t t) {
+        if (!grouped) {
+            TK[ti++] = t;
+            grouped = (ti & 3u) == 0;
+            return;
+        }
+        const uint32_t k = ti & 3u;
+        tb0 = k == 0 ? t : tb0;
+        tb1 = k == 1 ? t : tb1;
+        tb2 = k == 2 ? t : tb2;
+        tb3 = t;
+        if (k == 3) *reinterpret_cast<uint4*>(TK + ti - 3) = make_uint4(tb0, tb1, tb2, tb3);
+        ++ti;
+    };
     for (uint32_t w = w0; w < w1; ++w) {
         uint32_t x = bm[w];
         if (!x) continue;
@@ -664,16 +683,23 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
             const uint32_t lits = cd >> 12, step = cd & 0xFFFu;
             for (uint32_t k = 0; k < lits; ++k) {
                 if ((ti % BLK_SYMS) == 0 && ti / BLK_SYMS < maxblk) blkpos[(uint64_t)c * maxblk + ti / BLK_SYMS] = p + k;
-                TK[ti++] = byte_at(o + k);
+                put(byte_at(o + k));
             }
             if ((ti % BLK_SYMS) == 0 && ti / BLK_SYMS < maxblk) blkpos[(uint64_t)c * maxblk + ti / BLK_SYMS] = p + lits;
             if (step > lits + 1) {
                 const uint32_t m = ML[p + lits];
-                TK[ti++] = 0x80000000u | (ml_len(m) << 16) | ml_dist(m);
+                put(0x80000000u | (ml_len(m) << 16) | ml_dist(m));
             } else {
-                TK[ti++] = byte_at(o + lits);
+                put(byte_at(o + lits));
             }
         }
+    }
+    // the tokens of the last, partly filled group
+    if (grouped) {
+        const uint32_t k = ti & 3u;
+        if (k > 0) TK[ti - k] = tb0;
+        if (k > 1) TK[ti - k + 1] = tb1;
+        if (k > 2) TK[ti - k + 2] = tb2;
     }
 }
 
