@@ -354,7 +354,8 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t seg, uint32_t
                 for (int i = 0; i < L6M_WALKS; ++i) {
                     Walk& a = m[i];
                     q[i] = a.cur - st[i];
-                    const bool ok = a.live && a.k < max_chain && st[i] != 0 && a.x - q[i] <= 32768u;
+                    // bitwise: no exec-mask region per walk
+                    const bool ok = a.live & (a.k < max_chain) & (st[i] != 0) & (a.x - q[i] <= 32768u);
                     a.live = ok;
                     any |= ok;
                     // a dead walk's cursor is never read again, so the step
@@ -374,10 +375,9 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t seg, uint32_t
 #pragma unroll
                 for (int i = 0; i < L6M_WALKS; ++i) {
                     Walk& a = m[i];
-                    if (!a.live || c0[i] != a.w) continue;
                     // only a match that also covers byte `best` can be longer
                     // than best (best < cap here, so these bytes are in the chunk)
-                    if (a.best >= 4 && c1[i] != a.xbw) continue;
+                    if (!(a.live & (c0[i] == a.w) & ((a.best < 4) | (c1[i] == a.xbw)))) continue;
                     const uint32_t qq = q[i];
                     uint32_t l;
                     uint64_t df = ring_u64(DR, qq) ^ a.xa;
@@ -386,26 +386,25 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t seg, uint32_t
                     } else if ((df = ring_u64(DR, qq + 8) ^ a.xb) != 0) {
                         l = 8 + (uint32_t)(__builtin_ctzll(df) >> 3);
                     } else {
+                        // 8-byte steps up to cap: bytes past the chunk's end
+                        // in the ring only lengthen a compare that the cap
+                        // below cuts back
                         l = 16;
                         while (l < a.cap) {
-                            if (l + 8 <= a.cap) {
-                                const uint64_t d2 = ring_u64(DR, a.x + l) ^ ring_u64(DR, qq + l);
-                                if (d2) {
-                                    l += (uint32_t)(__builtin_ctzll(d2) >> 3);
-                                    break;
-                                }
-                                l += 8;
-                            } else {
-                                if (ring_u8(DR, a.x + l) != ring_u8(DR, qq + l)) break;
-                                ++l;
+                            const uint64_t d2 = ring_u64(DR, a.x + l) ^ ring_u64(DR, qq + l);
+                            if (d2) {
+                                l += (uint32_t)(__builtin_ctzll(d2) >> 3);
+                                break;
                             }
+                            l += 8;
                         }
                     }
                     l = min(l, a.cap);
                     if (l > a.best) {
+                        // l >= 4: the first four bytes passed
                         a.best = l;
                         a.bd = a.x - qq;
-                        if (l >= 4) a.xbw = ring_u32(DR, a.x + l - 3);
+                        a.xbw = ring_u32(DR, a.x + l - 3);
                         // nothing later can be longer than nice (stop rule) or cap
                         if ((int)l >= nice || l >= a.cap) a.live = false;
                     }
