@@ -380,9 +380,11 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t seg, uint32_t
                     if (!(a.live & (c0[i] == a.w) & ((a.best < 4) | (c1[i] == a.xbw)))) continue;
                     const uint32_t qq = q[i];
                     uint32_t l;
-                    uint64_t df = ring_u64(DR, qq) ^ a.xa;
-                    if (df) {
-                        l = (uint32_t)(__builtin_ctzll(df) >> 3);
+                    uint64_t df;
+                    // bytes 0..3 are equal (c0): compare 4..7, then 8..15
+                    const uint32_t d4 = ring_u32(DR, qq + 4) ^ (uint32_t)(a.xa >> 32);
+                    if (d4) {
+                        l = 4 + (uint32_t)(__builtin_ctz(d4) >> 3);
                     } else if ((df = ring_u64(DR, qq + 8) ^ a.xb) != 0) {
                         l = 8 + (uint32_t)(__builtin_ctzll(df) >> 3);
                     } else {
