@@ -941,6 +941,10 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     // layout index of the group of this range holding position q (r0 <= q < r1,
     // q - r0 a multiple of 8)
     auto gi = [&](uint32_t q) { return L.group((uint32_t)tid, (q - r0) >> 3); };
+    // the range's farthest reach p + step(p), and whether every step is 258,
+    // taken from A2's match lengths for P1
+    uint32_t runmax = 0;
+    bool all258 = r0 < r1;
     {
         const uint32_t lim = min(n, r1 + 257u);
         uint32_t cd = 0, ce = 0;
@@ -1016,6 +1020,9 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 // (d == e & 0x7FFF whenever m1 != 0), which C reads back
                 big |= m1 >= 255u;
                 xl |= (m1 >= 255u ? m1 - 255u : 0u) << (2 * k);
+                const bool in = p < r1;
+                runmax = max(runmax, in ? p + (m1 ? m1 : 1u) : 0u);
+                all258 &= !in || m1 == 258u;
                 cw[k >> 2] |= len_code8(m1, lit) << (8 * (k & 3));
             }
             if (big) {
@@ -1044,31 +1051,6 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     // q is a barrier when max over p < q of reach(p) == q (position 0 is one):
     // every greedy parse visits it.
     {
-        uint32_t runmax = 0;
-        bool all258 = r0 < r1;
-        for (uint32_t p = r0; p < r1; p += 4) {
-            const uint32_t cw = D[p >> 2];
-            const uint32_t nc = ~cw;
-            if (((nc - 0x01010101u) & ~nc & 0x80808080u) == 0u) {
-                // no code 255 in the word: every step comes from its code
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const uint32_t b = (cw >> (8 * k)) & 0xFFu;
-                    const uint32_t st = b <= 1u ? 1u : (b == 254u ? 258u : b + 1u);
-                    const bool valid = p + k < r1;
-                    all258 &= !valid || b == 254u;
-                    runmax = max(runmax, valid ? p + k + st : 0u);
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (p + k < r1) {
-                        const uint32_t b = (cw >> (8 * k)) & 0xFFu;
-                        all258 &= b == 254u;
-                        runmax = max(runmax, p + k + code_step(b, XS, L, p + k));
-                    }
-            }
-        }
         if (all258) atomicOr(&run258[tid >> 5], 1u << (tid & 31));
         uint32_t v = runmax;
 #pragma unroll
