@@ -644,7 +644,7 @@ static_assert(LC_MAX_CHUNK / LC_THREADS <= 128, "ranges of at most 136 positions
 constexpr uint32_t LB_BM_OFF = LC_REGION0 / 4;               // words: the token bitmap
 constexpr uint32_t LB_FB_OFF = LB_BM_OFF + LC_BM_WORDS;       // first barrier per range
 constexpr uint32_t LB_MISC_OFF = LB_FB_OFF + LC_THREADS;
-constexpr uint32_t LB_LTAB_OFF = LB_MISC_OFF + 64u;  // fixed-Huffman length symbols by u8 code
+constexpr uint32_t LB_LTAB_OFF = LB_MISC_OFF + 96u;  // fixed-Huffman length symbols by u8 code
 constexpr uint32_t LB_LDS_BYTES = (LB_LTAB_OFF + 256u) * 4u;
 
 constexpr uint16_t LA_NONE = 0xFFFFu;
@@ -1044,7 +1044,11 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         *reinterpret_cast<uint2*>(code + p0) = *reinterpret_cast<const uint2*>(LCB + gi(p0));
     for (uint32_t w = tid; w < ((n + 31) >> 5); w += LC_THREADS) bm[w] = 0;
     uint32_t* run258 = misc + 32;  // bit t: every position of range t steps 258
-    if (tid < (int)(LC_THREADS / 32)) run258[tid] = 0;
+    uint32_t* b0 = misc + 64;      // bit t: range t's first parse barrier is its first position
+    if (tid < (int)(LC_THREADS / 32)) {
+        run258[tid] = 0;
+        b0[tid] = 0;
+    }
     __syncthreads();
 
     // ---- P1: first parse barrier of each range.  reach(p) = p + step(p);
@@ -1088,14 +1092,17 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             }
         }
         fb[tid] = r0 < r1 ? first : 0xFFFFFFFFu;
+        if (r0 < r1 && first == r0) atomicOr(&b0[tid >> 5], 1u << (tid & 31));
     }
     __syncthreads();
 
     // ---- P2: exact greedy parse (lz77.cpp:378-406) from this range's
     // barrier to the first barrier of a later range; token starts -> bm.
     // Through ranges whose every step is 258 the token starts are pos + 258 k:
-    // they are visited eight at a time without reading their codes, with the
-    // same barrier test and bit per token.
+    // the run ends at the first start past n, outside such a range, or on the
+    // first barrier of a later range, which in such a range can only be its
+    // first position.  Both limits come from the run258 and b0 bitmasks, so
+    // the walk only sets the run's bits.
     {
         const uint32_t pstart = fb[tid];
         if (pstart != 0xFFFFFFFFu) {
@@ -1105,30 +1112,43 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 atomicOr(&bm[pos >> 5], 1u << (pos & 31));
                 const uint32_t b = code[pos];
                 pos += code_step(b, XS, L, pos);
-                if (b != 254u) continue;
+                if (b != 254u || pos >= n) continue;
+                const uint32_t t = L.range_of(pos);
+                if (!((run258[t >> 5] >> (t & 31)) & 1u)) continue;
+                // E: the first range at or after t whose steps are not all 258
+                uint32_t E = t;
                 for (;;) {
-                    // all loads first (addresses clamped into the chunk), then
-                    // one mask: the run ends at the first start past n, outside
-                    // a 258-step range, or on a barrier of a later range
-                    uint32_t q[8], f[8], fl[8];
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        q[j] = pos + 258u * (uint32_t)j;
-                        const uint32_t t = L.range_of(min(q[j], n - 1));
-                        f[j] = fb[t];
-                        fl[j] = run258[t >> 5] >> (t & 31);
+                    const uint32_t zw = ~run258[E >> 5] >> (E & 31);
+                    if (zw) {
+                        E += (uint32_t)__builtin_ctz(zw);
+                        break;
                     }
-                    uint32_t ok = 0;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        ok |= ((uint32_t)(q[j] < n) & fl[j] & (uint32_t)(q[j] < r1 || f[j] != q[j]) & 1u) << j;
-                    const uint32_t stop = (uint32_t)__builtin_ctz(~ok);
-#pragma unroll
-                    for (int j = 0; j < 8; ++j)
-                        if ((uint32_t)j < stop) atomicOr(&bm[q[j] >> 5], 1u << (q[j] & 31));
-                    pos += 258u * stop;
-                    if (stop < 8) break;
+                    E = ((E >> 5) + 1u) << 5;
+                    if (E >= LC_THREADS) break;
                 }
+                uint32_t lim = min(n, E * S);
+                // the first barrier on the run's stride, of a range after this one
+                for (uint32_t u = t; u < E;) {
+                    const uint32_t bw = b0[u >> 5] >> (u & 31);
+                    if (!bw) {
+                        u = ((u >> 5) + 1u) << 5;
+                        continue;
+                    }
+                    u += (uint32_t)__builtin_ctz(bw);
+                    if (u >= E) break;
+                    const uint32_t rs = u * S;
+                    if (rs >= pos && rs >= r1 && (rs - pos) % 258u == 0u) {
+                        lim = min(lim, rs);
+                        break;
+                    }
+                    ++u;
+                }
+                const uint32_t cnt = (lim - pos + 257u) / 258u;  // pos < lim, or pos == lim (then 0)
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    const uint32_t q = pos + 258u * k;
+                    atomicOr(&bm[q >> 5], 1u << (q & 31));
+                }
+                pos += 258u * cnt;
             }
         }
     }
