@@ -644,7 +644,8 @@ static_assert(LC_MAX_CHUNK / LC_THREADS <= 128, "ranges of at most 136 positions
 constexpr uint32_t LB_BM_OFF = LC_REGION0 / 4;               // words: the token bitmap
 constexpr uint32_t LB_FB_OFF = LB_BM_OFF + LC_BM_WORDS;       // first barrier per range
 constexpr uint32_t LB_MISC_OFF = LB_FB_OFF + LC_THREADS;
-constexpr uint32_t LB_LTAB_OFF = LB_MISC_OFF + 96u;  // fixed-Huffman length symbols by u8 code
+constexpr uint32_t LB_RUNS = 32;                     // long 258-step runs of a chunk (<= 31 fit 128 KiB)
+constexpr uint32_t LB_LTAB_OFF = LB_MISC_OFF + 96u + 2u * LB_RUNS;  // fixed-Huffman length symbols by u8 code
 constexpr uint32_t LB_LDS_BYTES = (LB_LTAB_OFF + 256u) * 4u;
 
 constexpr uint16_t LA_NONE = 0xFFFFu;
@@ -1045,10 +1046,12 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     for (uint32_t w = tid; w < ((n + 31) >> 5); w += LC_THREADS) bm[w] = 0;
     uint32_t* run258 = misc + 32;  // bit t: every position of range t steps 258
     uint32_t* b0 = misc + 64;      // bit t: range t's first parse barrier is its first position
+    uint32_t* runs = misc + 96;    // (first start, starts) of the long 258-step runs; count in misc[24]
     if (tid < (int)(LC_THREADS / 32)) {
         run258[tid] = 0;
         b0[tid] = 0;
     }
+    if (tid == 0) misc[24] = 0;
     __syncthreads();
 
     // ---- P1: first parse barrier of each range.  reach(p) = p + step(p);
@@ -1144,11 +1147,31 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                     ++u;
                 }
                 const uint32_t cnt = (lim - pos + 257u) / 258u;  // pos < lim, or pos == lim (then 0)
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    const uint32_t q = pos + 258u * k;
-                    atomicOr(&bm[q >> 5], 1u << (q & 31));
+                if (cnt >= 16u) {
+                    // a long run (>= 4128 positions; the walks' runs are
+                    // disjoint, so at most 31 in a chunk): its bits are set
+                    // by the whole workgroup below
+                    const uint32_t slot = atomicAdd(&misc[24], 1u);
+                    runs[2 * slot] = pos;
+                    runs[2 * slot + 1] = cnt;
+                } else {
+                    for (uint32_t k = 0; k < cnt; ++k) {
+                        const uint32_t q = pos + 258u * k;
+                        atomicOr(&bm[q >> 5], 1u << (q & 31));
+                    }
                 }
                 pos += 258u * cnt;
+            }
+        }
+    }
+    __syncthreads();
+    {
+        const uint32_t nr = misc[24];
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint32_t p0 = runs[2 * r], cnt = runs[2 * r + 1];
+            for (uint32_t k = (uint32_t)tid; k < cnt; k += LC_THREADS) {
+                const uint32_t q = p0 + 258u * k;
+                atomicOr(&bm[q >> 5], 1u << (q & 31));
             }
         }
     }
