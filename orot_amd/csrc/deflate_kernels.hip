@@ -1112,6 +1112,16 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
             uint32_t pos = pstart;
             while (pos < n) {
                 if (pos >= r1 && fb[L.range_of(pos)] == pos) break;
+                // eight literals inside this range (where no barrier test is
+                // needed) with one 8-byte read of their codes
+                if ((pos & 7u) == 0 && pos + 8 <= r1) {
+                    const uint64_t c8 = *reinterpret_cast<const uint64_t*>(code + pos);
+                    if ((c8 & 0xFEFEFEFEFEFEFEFEull) == 0) {
+                        atomicOr(&bm[pos >> 5], 0xFFu << (pos & 31));
+                        pos += 8;
+                        continue;
+                    }
+                }
                 atomicOr(&bm[pos >> 5], 1u << (pos & 31));
                 const uint32_t b = code[pos];
                 pos += code_step(b, XS, L, pos);
