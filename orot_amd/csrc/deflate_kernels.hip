@@ -666,12 +666,14 @@ struct LbLayout {
           R((n + S - 1) / S),
           m(0xFFFFFFFFu / S + 1u) {}
     __device__ uint32_t range_of(uint32_t p) const { return __umulhi(p, m); }
+    // t < 1024, S <= 136, g < 17 and R <= 1024: the products fit the
+    // full-rate 24-bit multiplier
     __device__ uint32_t index(uint32_t p) const {
-        const uint32_t t = range_of(p), q = p - t * S;
-        return (((q >> 3) * R + t) << 3) | (q & 7u);
+        const uint32_t t = range_of(p), q = p - __umul24(t, S);
+        return ((__umul24(q >> 3, R) + t) << 3) | (q & 7u);
     }
     // first word of group g of range t
-    __device__ uint32_t group(uint32_t t, uint32_t g) const { return (g * R + t) << 3; }
+    __device__ uint32_t group(uint32_t t, uint32_t g) const { return (__umul24(g, R) + t) << 3; }
 };
 constexpr uint32_t lb_tok_stride(uint32_t chunk) { return ((chunk + 7u) & ~7u) + 256u; }
 
