@@ -1304,17 +1304,38 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
     // is encoded by the thread holding its first position
     const uint32_t total = hlit + hdist;
     auto seq = [&](uint32_t k) -> uint32_t { return k < hlit ? R.ll[k] : R.dl[k - hlit]; };
+    // A run's length is the distance to the next run start: start flags by
+    // ballot within a wave, each wave's first start in LDS for the others.
     uint32_t mrun[2] = {0, 0}, vrun[2] = {0, 0}, nitems = 0;
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t k = 2u * (uint32_t)tid + (uint32_t)h;
-        if (k >= total) break;
-        const uint32_t v = seq(k);
-        if (k > 0 && seq(k - 1) == v) continue;
-        uint32_t m = 1;
-        while (k + m < total && seq(k + m) == v) ++m;
-        mrun[h] = m;
-        vrun[h] = v;
-        nitems += rle_run_items(v, m, [](uint32_t, uint32_t) {});
+    {
+        const uint32_t wv = (uint32_t)tid >> 6;
+        uint32_t* fs = red[1];  // first run start of each wave (red[1] is free after the sizes)
+        const uint32_t k0 = 2u * (uint32_t)tid, k1 = k0 + 1u;
+        const uint32_t v0 = k0 < total ? seq(k0) : 0u, v1 = k1 < total ? seq(k1) : 0u;
+        const bool s0 = k0 < total && (k0 == 0 || seq(k0 - 1) != v0);
+        const bool s1 = k1 < total && v0 != v1;
+        const uint64_t B0 = ballot(s0), B = B0 | ballot(s1);
+        auto start_of = [&](uint32_t l) { return 2u * (64u * wv + l) + (((B0 >> l) & 1ull) ? 0u : 1u); };
+        if (lane == 0) fs[wv] = B ? start_of((uint32_t)__builtin_ctzll(B)) : 0xFFFFFFFFu;
+        __syncthreads();
+        // the first run start in a later lane of this wave, or in a later wave
+        auto next_start = [&]() -> uint32_t {
+            const uint64_t M = lane == 63 ? 0ull : (B >> (lane + 1)) << (lane + 1);
+            if (M) return start_of((uint32_t)__builtin_ctzll(M));
+            for (uint32_t w = wv + 1; w < L6B_THREADS / 64; ++w)
+                if (fs[w] != 0xFFFFFFFFu) return fs[w];
+            return total;
+        };
+        if (s0) {
+            mrun[0] = s1 ? 1u : next_start() - k0;
+            vrun[0] = v0;
+            nitems += rle_run_items(v0, mrun[0], [](uint32_t, uint32_t) {});
+        }
+        if (s1) {
+            mrun[1] = next_start() - k1;
+            vrun[1] = v1;
+            nitems += rle_run_items(v1, mrun[1], [](uint32_t, uint32_t) {});
+        }
     }
     const uint32_t incl = wave_incl_scan(nitems, lane);
     if (lane == 63) red[0][tid >> 6] = incl;
