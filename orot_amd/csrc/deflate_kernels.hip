@@ -1323,22 +1323,18 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         const uint32_t bitpos = 3u + excl;
         uint32_t wi = bitpos >> 5;
         uint32_t nacc = bitpos & 31u;
-        bool partial = nacc != 0;
         uint64_t acc = 0;
+        // every output word starts at zero, so each completed word is ORed
+        // in (a word shared with a neighbouring range takes both halves);
+        // a token that completes no word ORs 0
         auto put = [&](uint32_t v, uint32_t len) {
             acc |= (uint64_t)v << nacc;
             nacc += len;
-            if (nacc >= 32) {
-                if (partial) {
-                    atomicOr(&out[wi], (uint32_t)acc);
-                    partial = false;
-                } else {
-                    out[wi] = (uint32_t)acc;
-                }
-                ++wi;
-                acc >>= 32;
-                nacc -= 32;
-            }
+            const bool fl = nacc >= 32;
+            atomicOr(&out[wi], fl ? (uint32_t)acc : 0u);
+            wi += fl ? 1u : 0u;
+            acc = fl ? (acc >> 32) : acc;
+            nacc -= fl ? 32u : 0u;
         };
         stream(std::true_type{}, [&](uint32_t g, const uint32_t* v) {
             const uint32_t m = group_mask(g);
