@@ -271,7 +271,8 @@ __device__ int32_t inflate_raw(InfTables& T, const uint8_t* src, uint64_t n, uin
                         rep = 11 + br.bits(7);
                     }
                     if (k + rep > total) return INF_DATA_ERROR;
-                    if ((uint32_t)lane < rep) T.lens[k + lane] = (uint8_t)val;
+                    // rep reaches 138 (symbol 18): more entries than lanes
+                    for (uint32_t i = (uint32_t)lane; i < rep; i += 64) T.lens[k + i] = (uint8_t)val;
                     k += rep;
                 }
                 wave_sync();

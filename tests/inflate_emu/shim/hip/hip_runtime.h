@@ -1,9 +1,14 @@
 // Host stand-in for the HIP runtime header, used only by tests/inflate_emu:
 // device qualifiers vanish, and the wave intrinsics the inflate kernel uses
 // become exchanges between the 64 host threads that play one wave's lanes.
+// A workgroup barrier ends the program: the inflate kernel's waves run
+// independently and must never need one.
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <barrier>
+#include <cstdio>
+#include <cstdlib>
 #include <cstddef>
 #include <cstdint>
 #define __device__
@@ -24,9 +29,9 @@ extern EmuIdx blockIdx, blockDim, gridDim;
 extern thread_local int emu_lane;
 uint64_t emu_ballot(bool b);
 long long emu_shfl_ll(long long v, int src);
-void emu_sync();
+void emu_wave_fence();
 #define __builtin_amdgcn_ballot_w64(b) emu_ballot(b)
-#define __builtin_amdgcn_fence(a, b) emu_sync()
+#define __builtin_amdgcn_fence(a, b) emu_wave_fence()
 #define hipFuncSetAttribute(...) 0
 #define hipMemsetAsync(...) 0
 #define hipLaunchKernelGGL(...) ((void)0)
@@ -40,7 +45,10 @@ inline unsigned __shfl_up(unsigned v, int d, int = 64) {
 }
 inline int __popcll(unsigned long long v) { return __builtin_popcountll(v); }
 inline int __popc(unsigned v) { return __builtin_popcount(v); }
-inline void __syncthreads() { emu_sync(); }
+inline void __syncthreads() {
+    fprintf(stderr, "emu: workgroup barrier reached\n");
+    abort();
+}
 template <class T> T atomicAdd(T* p, T v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 using std::max;
 using std::min;

@@ -1,10 +1,13 @@
 """CPU check of the inflate kernel's source (orot_amd/csrc/inflate.hip)
 against zlib, the library routine that pins RFC 1950-1952 decoding.  The
-unchanged device code is compiled for the host with tests/inflate_emu (64
-host threads are the lanes of one wave; ballot, shuffle and the wave fence
-are barrier exchanges) and runs the kernel entry point on zlib-made streams
-of every level, strategy and framing, on this library's own streams, and on
-truncated, corrupt and oversized cases, against zlib's output and orot's
+unchanged device code is compiled for the host with tests/inflate_emu, at
+-O3 with AddressSanitizer and UndefinedBehaviorSanitizer, and run in its
+launched shape: workgroups of four waves sharing the stream queue, each wave
+64 host threads whose ballots, shuffles and wave fences meet at a barrier of
+the wave (the lockstep of a wave's lanes), with the LDS and the output
+buffers filled with a non-zero pattern first.  It decodes zlib-made streams
+of every level, strategy and framing, this library's own streams, batches,
+and truncated, corrupt and oversized cases, against zlib's output and orot's
 result codes (src/formats/gzip_wrapper.cpp:76-138, zlib_wrapper.cpp).  This
 checks the algorithm as written, not the GPU code generation."""
 import os
@@ -28,18 +31,31 @@ def emu(tmp_path_factory):
         pytest.skip("clang++ of the ROCm toolchain is not installed")
     d = tmp_path_factory.mktemp("inflate_emu")
     exe = str(d / "emu")
-    subprocess.run([CLANG, "-std=c++20", "-O1", "-I", os.path.join(HERE, "inflate_emu", "shim"),
+    subprocess.run([CLANG, "-std=c++20", "-O3", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-fno-omit-frame-pointer", "-I", os.path.join(HERE, "inflate_emu", "shim"),
                     "-I", os.path.join(HERE, "..", "include"), "-o", exe,
                     os.path.join(HERE, "inflate_emu", "emu.cpp"), "-lpthread"], check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+
+    def run_batch(jobs, fmt: int):
+        """jobs: (stream, capacity) pairs, one launch; [(status, output)]."""
+        args = [exe, str(fmt), str(len(jobs))]
+        for i, (stream, cap) in enumerate(jobs):
+            src = d / f"in{i}.bin"
+            src.write_bytes(stream)
+            args += [str(src), str(cap), str(d / f"out{i}.bin")]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=300, env=env)
+        assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-4000:])
+        got = []
+        for i in range(len(jobs)):
+            st = int(r.stdout.split(f"item {i}: status")[1].split()[0])
+            got.append((st, (d / f"out{i}.bin").read_bytes() if st == OK else b""))
+        return got
 
     def run(stream: bytes, fmt: int, cap: int):
-        src, dst = d / "in.bin", d / "out.bin"
-        src.write_bytes(stream)
-        r = subprocess.run([exe, str(src), str(fmt), str(cap), str(dst)], capture_output=True, text=True,
-                           timeout=120, check=True)
-        st = int(r.stdout.split("status")[1].split()[0])
-        return st, dst.read_bytes() if st == OK else b""
+        return run_batch([(stream, cap)], fmt)[0]
 
+    run.batch = run_batch
     yield run
     shutil.rmtree(d, ignore_errors=True)
 
@@ -103,3 +119,30 @@ def test_errors(emu):
     co = zlib.compressobj(6, zlib.DEFLATED, -15, 9, zlib.Z_DEFAULT_STRATEGY, zdict=data[:4000])
     s = co.compress(data[4000:9000]) + co.flush()
     assert emu(s, 0, 5000)[0] == DATA_ERROR  # distance before the start
+
+
+def test_batch_launch_shape(emu):
+    """Eleven streams in one launch: three workgroups' worth, taken from the
+    one queue by four waves at once, longest first; every stream's result in
+    its own slot."""
+    jobs, want = [], []
+    kinds = list(corpus.GENERATORS) * 3
+    for i, kind in enumerate(kinds[:11]):
+        data = corpus.generate(kind, 5_000 + 7_000 * i, 40 + i).tobytes()
+        jobs.append((zstream(data, (0, 1, 6, 9)[i % 4], 1), len(data)))
+        want.append(data)
+    got = emu.batch(jobs, 1)
+    for i, (st, out) in enumerate(got):
+        assert st == OK and out == want[i], i
+
+
+def test_long_zero_length_runs(emu):
+    """Dynamic headers whose code-length runs (symbol 18) are longer than the
+    64 lanes that write them, over LDS that starts non-zero: a block whose
+    literals are 16 byte values leaves runs of 138 zero lengths."""
+    rng = corpus.generate("random", 30_000, 77)
+    data = bytes(b & 0x0F | 0x40 for b in rng.tobytes())
+    for fmt in (0, 2):
+        s = zstream(data, 9, fmt)
+        st, got = emu(s, fmt, len(data))
+        assert st == OK and got == data, fmt
