@@ -428,7 +428,9 @@ __global__ __launch_bounds__(L6M_THREADS) void l6_match_kernel(const uint8_t* __
                                                                const uint16_t* __restrict__ pd,
                                                                uint32_t* __restrict__ mlw, const L6Params prm,
                                                                uint32_t segs) {
-    extern __shared__ uint32_t smem[];
+    // static LDS: its address is known when the kernel is compiled, so the
+    // ring reads take it in their offset field instead of an add per read
+    __shared__ uint32_t smem[L6M_LDS / 4];
     for (;;) {
         const uint32_t k = next_item(&hdr->queue[Q_MATCH], hdr->nchunks * segs, smem);
         if (k == NO_ITEM) break;
@@ -1804,7 +1806,6 @@ hipError_t set_kernel_attributes_l6(size_t lds_max) {
     (void)lds_max;
     hipError_t e = l6_allow_lds(&l6_pred_kernel<true>, L6P_LDS);
     if (e == hipSuccess) e = l6_allow_lds(&l6_pred_kernel<false>, L6P_LDS);
-    if (e == hipSuccess) e = l6_allow_lds(&l6_match_kernel, L6M_LDS);
     if (e == hipSuccess) e = l6_allow_lds(&l6_parse_kernel, L6R_LDS);
     if (e == hipSuccess) e = l6_allow_lds(&l6_pseg_parse_kernel, L6R_LDS);
     return e;
@@ -1831,7 +1832,7 @@ hipError_t launch_l6_front(const Plan& p) {
     const uint32_t mcap = p.cus * l6_per_cu(p, L6M_LDS, L6M_THREADS);
     const uint32_t segs = l6_segments(p, mcap);
     const unsigned gm = grid_for(p.persistent_grid, p.nchunks * segs, mcap);
-    hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), L6M_LDS, p.stream, p.d_in, p.d_desc, p.d_bufs,
+    hipLaunchKernelGGL(l6_match_kernel, dim3(gm), dim3(L6M_THREADS), 0, p.stream, p.d_in, p.d_desc, p.d_bufs,
                        p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.l6, segs);
     const uint32_t dcap = p.cus * 2u;  // two 1024-thread workgroups per CU
     const uint32_t dsegs = l6_segments(p, dcap);
