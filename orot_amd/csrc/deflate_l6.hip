@@ -379,37 +379,36 @@ __device__ __forceinline__ void l6_match_body(uint32_t c, uint32_t seg, uint32_t
                     // than best (best < cap here, so these bytes are in the chunk)
                     if (!(a.live & (c0[i] == a.w) & ((a.best < 4) | (c1[i] == a.xbw)))) continue;
                     const uint32_t qq = q[i];
-                    uint32_t l;
-                    uint64_t df;
                     // bytes 0..3 are equal (c0): compare 4..7, then 8..15
                     const uint32_t d4 = ring_u32(DR, qq + 4) ^ (uint32_t)(a.xa >> 32);
-                    if (d4) {
-                        l = 4 + (uint32_t)(__builtin_ctz(d4) >> 3);
-                    } else if ((df = ring_u64(DR, qq + 8) ^ a.xb) != 0) {
-                        l = 8 + (uint32_t)(__builtin_ctzll(df) >> 3);
-                    } else {
-                        // 8-byte steps up to cap: bytes past the chunk's end
-                        // in the ring only lengthen a compare that the cap
-                        // below cuts back
-                        l = 16;
-                        while (l < a.cap) {
-                            const uint64_t d2 = ring_u64(DR, a.x + l) ^ ring_u64(DR, qq + l);
-                            if (d2) {
-                                l += (uint32_t)(__builtin_ctzll(d2) >> 3);
-                                break;
+                    // the length from d4 without a branch (bit 31 keeps ctz defined when
+                    // d4 == 0, which the branch below replaces)
+                    uint32_t l = 4 + (uint32_t)(__builtin_ctz(d4 | 0x80000000u) >> 3);
+                    if (d4 == 0) {
+                        const uint64_t df = ring_u64(DR, qq + 8) ^ a.xb;
+                        l = df ? 8 + (uint32_t)(__builtin_ctzll(df) >> 3) : 16u;
+                        if (df == 0) {
+                            // 8-byte steps up to cap: bytes past the chunk's end
+                            // in the ring only lengthen a compare that the cap
+                            // below cuts back
+                            while (l < a.cap) {
+                                const uint64_t d2 = ring_u64(DR, a.x + l) ^ ring_u64(DR, qq + l);
+                                if (d2) {
+                                    l += (uint32_t)(__builtin_ctzll(d2) >> 3);
+                                    break;
+                                }
+                                l += 8;
                             }
-                            l += 8;
                         }
                     }
                     l = min(l, a.cap);
-                    if (l > a.best) {
-                        // l >= 4: the first four bytes passed
-                        a.best = l;
-                        a.bd = a.x - qq;
-                        a.xbw = ring_u32(DR, a.x + l - 3);
-                        // nothing later can be longer than nice (stop rule) or cap
-                        if ((int)l >= nice || l >= a.cap) a.live = false;
-                    }
+                    // l >= 4: the first four bytes passed
+                    const bool better = l > a.best;
+                    a.bd = better ? a.x - qq : a.bd;
+                    // nothing later can be longer than nice (stop rule) or cap
+                    a.live = a.live & !(better & (((int)l >= nice) | (l >= a.cap)));
+                    if (better) a.xbw = ring_u32(DR, a.x + l - 3);
+                    a.best = better ? l : a.best;
                 }
             }
 #pragma unroll
