@@ -786,6 +786,9 @@ __device__ __forceinline__ uint32_t pseg_tokens(const uint32_t* G, const uint16_
         if (!x) continue;
         uint32_t cw[16];
         codes32_at(CODE, w << 5, cw);
+        asm volatile("" ::"v"(cw[0]), "v"(cw[1]), "v"(cw[2]), "v"(cw[3]), "v"(cw[4]), "v"(cw[5]), "v"(cw[6]),
+                     "v"(cw[7]), "v"(cw[8]), "v"(cw[9]), "v"(cw[10]), "v"(cw[11]), "v"(cw[12]), "v"(cw[13]),
+                     "v"(cw[14]), "v"(cw[15]));
         while (x) {
             const int k = __builtin_ctz(x);
             x &= x - 1;
@@ -1005,12 +1008,12 @@ __global__ __launch_bounds__(L6R_THREADS) void l6_pseg_write_kernel(const uint8_
                 }
             }
             // wait for the word's loads here, once: the token loop's stores
-        // share the load counter, so a first use inside the loop would
-        // wait for every store of the earlier tokens on each iteration
-        asm volatile("" ::"v"(cw[0]), "v"(cw[1]), "v"(cw[2]), "v"(cw[3]), "v"(cw[4]), "v"(cw[5]), "v"(cw[6]),
-                     "v"(cw[7]), "v"(cw[8]), "v"(cw[9]), "v"(cw[10]), "v"(cw[11]), "v"(cw[12]), "v"(cw[13]),
-                     "v"(cw[14]), "v"(cw[15]), "v"(sb[0]), "v"(sb[1]), "v"(sb[2]), "v"(sb[3]), "v"(sb[4]));
-        auto byte_at = [&](uint32_t o) -> uint32_t {  // o < 40
+            // share the load counter, so a first use inside the loop would
+            // wait for every store of the earlier tokens on each iteration
+            asm volatile("" ::"v"(cw[0]), "v"(cw[1]), "v"(cw[2]), "v"(cw[3]), "v"(cw[4]), "v"(cw[5]), "v"(cw[6]),
+                         "v"(cw[7]), "v"(cw[8]), "v"(cw[9]), "v"(cw[10]), "v"(cw[11]), "v"(cw[12]), "v"(cw[13]),
+                         "v"(cw[14]), "v"(cw[15]), "v"(sb[0]), "v"(sb[1]), "v"(sb[2]), "v"(sb[3]), "v"(sb[4]));
+            auto byte_at = [&](uint32_t o) -> uint32_t {  // o < 40
                 const uint64_t v = o < 8 ? sb[0] : (o < 16 ? sb[1] : (o < 24 ? sb[2] : (o < 32 ? sb[3] : sb[4])));
                 return (uint32_t)(v >> (8 * (o & 7))) & 0xFFu;
             };
