@@ -1723,6 +1723,11 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
     // Pack the slice in a 64-bit register; words wholly inside it are plain
     // stores, its first and last words (shared with a neighbour, the header
     // or the EOB) are atomicOr.
+    // the slice's token loads are complete here (the count used them); this
+    // says so to the compiler, which would otherwise wait for them at every
+    // token of the emission, behind the stores of the tokens before it
+#pragma unroll
+    for (int k = 0; k < QMAX; ++k) asm volatile("" ::"v"(q[k].x), "v"(q[k].y), "v"(q[k].z), "v"(q[k].w));
     if (mybits) {
         const uint64_t at = sym0 + wsum[wv] + incl - mybits;
         uint64_t wi = at >> 5;
