@@ -916,19 +916,36 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
 
     // ---- 0: stage the chunk in LDS, plus zeroed padding words (which may
     // reach into the bitmap; K clears it)
-    for (uint32_t i = (uint32_t)tid * 16u; i < n; i += LC_THREADS * 16u) {
-        if (i + 16 <= n) {
-            const uint64_t a = ld64u(s + i), b = ld64u(s + i + 8);
-            D[(i >> 2) + 0] = (uint32_t)a;
-            D[(i >> 2) + 1] = (uint32_t)(a >> 32);
-            D[(i >> 2) + 2] = (uint32_t)b;
-            D[(i >> 2) + 3] = (uint32_t)(b >> 32);
-        } else {
-            for (uint32_t w = i >> 2; w < ((n + 3) >> 2); ++w) {
-                uint32_t v = 0;
-                for (uint32_t k = 0; k < 4; ++k)
-                    if (4 * w + k < n) v |= (uint32_t)s[4 * w + k] << (8 * k);
-                D[w] = v;
+    // (a thread's 16-byte pieces are all loaded before any is stored, so
+    // their loads are in flight together)
+    {
+        constexpr int PIECES = LC_MAX_CHUNK / (LC_THREADS * 16u);
+        uint64_t va[PIECES], vb[PIECES];
+#pragma unroll
+        for (int k = 0; k < PIECES; ++k) {
+            const uint32_t i = ((uint32_t)tid + (uint32_t)k * LC_THREADS) * 16u;
+            va[k] = 0;
+            vb[k] = 0;
+            if (i + 16 <= n) {
+                va[k] = ld64u(s + i);
+                vb[k] = ld64u(s + i + 8);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PIECES; ++k) {
+            const uint32_t i = ((uint32_t)tid + (uint32_t)k * LC_THREADS) * 16u;
+            if (i + 16 <= n) {
+                D[(i >> 2) + 0] = (uint32_t)va[k];
+                D[(i >> 2) + 1] = (uint32_t)(va[k] >> 32);
+                D[(i >> 2) + 2] = (uint32_t)vb[k];
+                D[(i >> 2) + 3] = (uint32_t)(vb[k] >> 32);
+            } else if (i < n) {
+                for (uint32_t w = i >> 2; w < ((n + 3) >> 2); ++w) {
+                    uint32_t v = 0;
+                    for (uint32_t b = 0; b < 4; ++b)
+                        if (4 * w + b < n) v |= (uint32_t)s[4 * w + b] << (8 * b);
+                    D[w] = v;
+                }
             }
         }
     }
@@ -962,7 +979,11 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 if (q0 + k < nh) v[k >> 1] |= (uint32_t)g[k] << (16 * (k & 1));
             return make_uint4(v[0], v[1], v[2], v[3]);
         };
+        nx = make_uint4(0u, 0u, 0u, 0u);
         if (r0 < r1) nx = load_loc(r0);
+        // the first group's codes are waited for here, so the loop's own
+        // waits count only the loads and stores inside it
+        asm volatile("" ::"v"(nx.x), "v"(nx.y), "v"(nx.z), "v"(nx.w));
         for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
             const uint4 lv = nx;
             const uint32_t lw[4] = {lv.x, lv.y, lv.z, lv.w};
@@ -1043,8 +1064,20 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     // ---- K: the u8 codes A2 wrote per group, to LDS over the bytes, 8 at a
     // time in range order (codes of a last group's positions past n land
     // below LC_REGION0 and are never read); bitmap cleared.
-    for (uint32_t p0 = r0; p0 < r1; p0 += 8)
-        *reinterpret_cast<uint2*>(code + p0) = *reinterpret_cast<const uint2*>(LCB + gi(p0));
+    // (loaded in batches of six groups, each batch's loads in flight together)
+    for (uint32_t p0 = r0; p0 < r1; p0 += 48) {
+        uint2 v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const uint32_t q = p0 + 8u * (uint32_t)k;
+            v[k] = q < r1 ? *reinterpret_cast<const uint2*>(LCB + gi(q)) : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const uint32_t q = p0 + 8u * (uint32_t)k;
+            if (q < r1) *reinterpret_cast<uint2*>(code + q) = v[k];
+        }
+    }
     for (uint32_t w = tid; w < ((n + 31) >> 5); w += LC_THREADS) bm[w] = 0;
     uint32_t* run258 = misc + 32;  // bit t: every position of range t steps 258
     uint32_t* b0 = misc + 64;      // bit t: range t's first parse barrier is its first position
