@@ -596,11 +596,19 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
     {
         const uint32_t pstart = fb[tid];
         if (pstart != 0xFFFFFFFFu) {
-            uint32_t pos = pstart;
+            uint32_t pos = pstart, wb = 0xFFFFFFFFu;
+            uint32_t cw[16];
             while (pos < n) {
                 if (pos >= r1 && fb[pos / S] == pos) break;
                 atomicOr(&bm[pos >> 5], 1u << (pos & 31));
-                pos += CODE[pos] & 0xFFFu;
+                // the decisions of pos's 32-position window, loaded together
+                // when the walk enters it: one dependent load per window, not
+                // per token
+                if ((pos & ~31u) != wb) {
+                    wb = pos & ~31u;
+                    codes32(wb, cw);
+                }
+                pos += code_of(cw, (int)(pos & 31u)) & 0xFFFu;
             }
         }
     }
