@@ -979,24 +979,34 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
         uint32_t cd = 0, ce = 0;
         bool cx = false;
         uint4 nx;  // the next group's eight 16-bit candidate codes
-        auto load_loc = [&](uint32_t q0) -> uint4 {
-            const uint16_t* g = Q + L.group((uint32_t)tid, (q0 - r0) >> 3);
-            if (q0 + 8 <= nh) return *reinterpret_cast<const uint4*>(g);
-            uint32_t v[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (q0 + k < nh) v[k >> 1] |= (uint32_t)g[k] << (16 * (k & 1));
-            return make_uint4(v[0], v[1], v[2], v[3]);
+        // two groups ahead: every load reads a whole slot of the range
+        // (a group past the range's end reads its last group again), and
+        // the codes of positions at or past nh are cleared where they are used
+        const uint32_t glast = r0 < r1 ? r0 + ((r1 - r0 - 1) & ~7u) : r0;
+        auto slot = [&](uint32_t q0) -> uint4 {
+            return *reinterpret_cast<const uint4*>(Q + L.group((uint32_t)tid, (min(q0, glast) - r0) >> 3));
         };
         nx = make_uint4(0u, 0u, 0u, 0u);
-        if (r0 < r1) nx = load_loc(r0);
+        uint4 nx2 = nx;
+        if (r0 < r1) {
+            nx = slot(r0);
+            nx2 = slot(r0 + 8);
+        }
         // the first group's codes are waited for here, so the loop's own
         // waits count only the loads and stores inside it
         asm volatile("" ::"v"(nx.x), "v"(nx.y), "v"(nx.z), "v"(nx.w));
         for (uint32_t p0 = r0; p0 < r1; p0 += 8) {
-            const uint4 lv = nx;
+            uint4 lv = nx;
+            if (p0 + 8 > nh) {
+                uint32_t v[4] = {lv.x, lv.y, lv.z, lv.w};
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (p0 + k >= nh) v[k >> 1] &= ~(0xFFFFu << (16 * (k & 1)));
+                lv = make_uint4(v[0], v[1], v[2], v[3]);
+            }
             const uint32_t lw[4] = {lv.x, lv.y, lv.z, lv.w};
-            if (p0 + 8 < r1) nx = load_loc(p0 + 8);
+            nx = nx2;
+            nx2 = slot(p0 + 16);
             const uint64_t w0 = lds_u64(D, p0), w1 = lds_u64(D, p0 + 8);
             uint32_t cw[2] = {0, 0};  // the group's u8 codes
             uint32_t xl = 0;          // 2 bits per position: length - 255 where it is 255..257
