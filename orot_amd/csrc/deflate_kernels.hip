@@ -6,6 +6,8 @@
 //
 //   checksum_kernel   one workgroup per chunk: CRC-32 register or Adler-32
 //                     partial sums of the chunk (lane slices + GF(2) shifts).
+//                     Skipped for zlib on the barrier path, whose staging
+//                     pass sums the chunk's bytes as it puts them in LDS.
 //   l1_candidate_kernel + l1_barrier_kernel   levels 1-3, chunks up to
 //                     128 KiB (orot's defaults): see the section below.
 //   l1_lz77_kernel    level 1 with chunks of 128-256 KiB, one workgroup
@@ -907,7 +909,7 @@ template <int HB>
 __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, uint32_t chunk, uint16_t* __restrict__ cand,
     uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage, uint64_t stage_words,
-    ChunkInfo* __restrict__ info) {
+    ChunkInfo* __restrict__ info, int adler) {
     const uint32_t n = desc[c].len;
     const uint8_t* s = src + desc[c].in_off;
     // candidate codes from l1_candidate_kernel, the u8 codes written by A2
@@ -927,6 +929,11 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
     // reach into the bitmap; K clears it)
     // (a thread's 16-byte pieces are all loaded before any is stored, so
     // their loads are in flight together)
+    // With `adler`, the chunk's Adler-32 partials come from the staged bytes
+    // (zlib_wrapper.cpp's checksum, combined per buffer by finalize_kernel):
+    // s1 = sum of bytes, s2 = sum of (n - p) * byte = n * s1 - sum of p * byte.
+    uint32_t ad_s = 0;   // this thread's byte sum
+    uint64_t ad_p = 0;   // this thread's sum of p * byte
     {
         constexpr int PIECES = LC_MAX_CHUNK / (LC_THREADS * 16u);
         uint64_t va[PIECES], vb[PIECES];
@@ -952,14 +959,63 @@ __device__ __forceinline__ void l1_barrier_body(uint32_t c, uint32_t* smem, cons
                 for (uint32_t w = i >> 2; w < ((n + 3) >> 2); ++w) {
                     uint32_t v = 0;
                     for (uint32_t b = 0; b < 4; ++b)
-                        if (4 * w + b < n) v |= (uint32_t)s[4 * w + b] << (8 * b);
+                        if (4 * w + b < n) {
+                            const uint32_t byte = s[4 * w + b];
+                            v |= byte << (8 * b);
+                            ad_s += byte;
+                            ad_p += (uint64_t)(4 * w + b) * byte;
+                        }
                     D[w] = v;
+                }
+            }
+        }
+        if (adler) {
+            // the thread's own full pieces, read back from LDS: their bytes,
+            // and the bytes' offsets inside the piece, by dot products
+#pragma unroll
+            for (int k = 0; k < PIECES; ++k) {
+                const uint32_t i = ((uint32_t)tid + (uint32_t)k * LC_THREADS) * 16u;
+                if (i + 16 <= n) {
+                    const uint4 q4 = *reinterpret_cast<const uint4*>(D + (i >> 2));
+                    const uint32_t q[4] = {q4.x, q4.y, q4.z, q4.w};
+                    uint32_t ps = 0, pw = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t sj = __builtin_amdgcn_udot4(q[j], 0x01010101u, 0u, false);
+                        ps += sj;
+                        pw += 4u * (uint32_t)j * sj + __builtin_amdgcn_udot4(q[j], 0x03020100u, 0u, false);
+                    }
+                    ad_s += ps;
+                    ad_p += (uint64_t)i * ps + pw;
                 }
             }
         }
     }
     if (tid < 4) D[((n + 3) >> 2) + tid] = 0;  // past the staged words
+    if (adler) {
+        // wave sums, then the 16 waves' sums (in the first-barrier words,
+        // which P1 writes later) added by thread 0
+#pragma unroll
+        for (int dlt = 32; dlt; dlt >>= 1) {
+            ad_s += (uint32_t)__shfl_xor((int)ad_s, dlt, 64);
+            ad_p += (uint64_t)__shfl_xor((long long)ad_p, dlt, 64);
+        }
+        if (lane == 0) {
+            smem[LB_FB_OFF + 3 * wv] = ad_s;
+            smem[LB_FB_OFF + 3 * wv + 1] = (uint32_t)ad_p;
+            smem[LB_FB_OFF + 3 * wv + 2] = (uint32_t)(ad_p >> 32);
+        }
+    }
     __syncthreads();
+    if (adler && tid == 0) {
+        uint64_t a = 0, pp = 0;
+        for (int w = 0; w < LC_WAVES; ++w) {
+            a += smem[LB_FB_OFF + 3 * w];
+            pp += (uint64_t)smem[LB_FB_OFF + 3 * w + 1] | ((uint64_t)smem[LB_FB_OFF + 3 * w + 2] << 32);
+        }
+        info[c].adler_s1 = (uint32_t)(a % ADLER_MOD);
+        info[c].adler_s2 = (uint32_t)(((uint64_t)n * a - pp) % ADLER_MOD);
+    }
 
     // ---- A2: candidate distance from the candidate code (cand_code; for a
     // far candidate C is recovered by hash and its first byte compared,
@@ -1430,13 +1486,13 @@ template <int HB>
 __global__ __launch_bounds__(LC_THREADS) void l1_barrier_kernel(const uint8_t* __restrict__ src,
     const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
     uint32_t chunk, uint16_t* __restrict__ cand, uint32_t* __restrict__ tokw, uint32_t* __restrict__ stage,
-    uint64_t stage_words, ChunkInfo* __restrict__ info) {
+    uint64_t stage_words, ChunkInfo* __restrict__ info, int adler) {
     extern __shared__ uint32_t smem[];
     for (uint32_t i = threadIdx.x; i < 256u; i += LC_THREADS) smem[LB_LTAB_OFF + i] = len_entry(i);
     for (;;) {
         const uint32_t k = next_item(&hdr->queue[Q_LZ], hdr->nchunks, smem);
         if (k == NO_ITEM) break;
-        l1_barrier_body<HB>(order[k], smem, src, desc, chunk, cand, tokw, stage, stage_words, info);
+        l1_barrier_body<HB>(order[k], smem, src, desc, chunk, cand, tokw, stage, stage_words, info, adler);
         __syncthreads();
     }
 }
@@ -1896,7 +1952,8 @@ hipError_t launch_barrier(const Plan& p) {
     }
     const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, LB_LDS_BYTES, LC_THREADS));
     hipLaunchKernelGGL((l1_barrier_kernel<HB>), dim3(g), dim3(LC_THREADS), LB_LDS_BYTES, p.stream, p.d_in, p.d_desc,
-                       p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.d_stage, p.stage_words, p.d_info);
+                       p.d_order, p.d_hdr, p.chunk, p.d_pd, p.d_tok, p.d_stage, p.stage_words, p.d_info,
+                       p.format == 1 ? 1 : 0);
     return hipSuccess;
 }
 
@@ -1907,6 +1964,14 @@ hipError_t launch_wide(const Plan& p) {
     hipLaunchKernelGGL(l1_lz77_kernel<HB>, dim3(g), dim3(L1_THREADS), smem, p.stream, p.d_in, p.d_desc, p.d_order,
                        p.d_hdr, p.chunk, p.d_tok, p.d_stage, p.stage_words, p.d_info);
     return hipSuccess;
+}
+
+// Whether launch_l1_class runs the barrier kernel, which then also makes the
+// zlib checksum partials of its chunks.
+bool l1_uses_barrier(const Plan& p) {
+    if (p.level < 1 || p.level > 3) return false;
+    if (p.whole || p.l1_dict || l1_big_chunk(p.chunk, p.hash_bits)) return false;
+    return (p.hash_bits == 12 || p.hash_bits == 14) && p.chunk <= LC_MAX_CHUNK;
 }
 
 hipError_t launch_l1_class(const Plan& p) {
@@ -1971,7 +2036,9 @@ hipError_t launch_parallel_compress(const Plan& p) {
     const hipStream_t s = p.stream;
     const uint32_t light = p.cus * 8u;
     record(p.ev[0], s);
-    if (p.format != 0)
+    // zlib's Adler-32 partials of a barrier-path batch come from the barrier
+    // kernel's staging pass instead
+    if (p.format != 0 && !(p.format == 1 && l1_uses_barrier(p)))
         hipLaunchKernelGGL(checksum_kernel, dim3(grid_for(p.persistent_grid, p.nchunks, light)), dim3(256), 0, s,
                            p.d_in, p.d_desc, p.d_hdr, p.format == 2 ? 0 : 1, p.d_info);
     record(p.ev[1], s);
