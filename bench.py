@@ -45,6 +45,7 @@ HBM_PEAK = 8.0e12
 # default workload, scaled by the same counters on a plain copy kernel.
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "l1_lz_stage_256MiB_pmc.txt")
 BATCH_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "l1_lz_stage_batch_4096_pmc.txt")
+L6_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "l6_kernels_256x8MiB_pmc.txt")
 # the L1-class LZ stage at 12-bit hashes and chunks up to 128 KiB: candidate
 # scan, then match / parse / emission
 L1_STAGE = "l1_candidate_kernel + l1_barrier_kernel"
@@ -661,7 +662,7 @@ def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds, tra
         stages = bt.stage_times()
     front = stages[1] * 1e-3
     b_alg = n + clen
-    kernel = L1_STAGE if level <= 3 else "l6 front (pred + match + parse + block analysis)"
+    kernel = L1_STAGE if level <= 3 else "l6 front (pred + match + decide + parse + block analysis)"
     if level <= 3 and tuned["block_size"] > 131072:
         kernel = "l1_lz77_kernel"
     if level <= 3 and tuned["hash_bits"] == 14:
@@ -724,7 +725,7 @@ def bench_l6(args) -> dict:
     return bench_multi(args, specs, 6, 2,
                        "L6-class lazy parse, dynamic Huffman blocks (15-bit codes, <= 16384 symbols), gzip, "
                        "256 x 8 MiB buffers (128 Zipf text + 128 64-byte records), batched",
-                       lambda i: specs[i][0])
+                       lambda i: specs[i][0], L6_TRAFFIC_SUMMARY)
 
 
 def bench_batch(args) -> dict:

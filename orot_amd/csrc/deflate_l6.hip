@@ -533,8 +533,10 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
     // bitmap words owned by this range (S is a multiple of 32; an empty range
     // owns none, even when r0 falls inside the last range's final word)
     const uint32_t w0 = r0 >> 5, w1 = r0 < r1 ? (r1 + 31) >> 5 : w0;
-    // the decisions CODE[] come from l6_decide_kernel; clear the bitmap
+    // the decisions CODE[] come from l6_decide_kernel; clear the bitmap and
+    // the range's count of lazy literals (added by the exact parse below)
     for (uint32_t w = w0; w < w1; ++w) bm[w] = 0;
+    rcnt[tid] = 0;
     __syncthreads();
     // the range's codes 32 at a time (r0 is a multiple of 32, so the 16-byte
     // loads are aligned; codes past n read the chunk's slack and are masked)
@@ -608,27 +610,19 @@ __device__ __forceinline__ void l6_parse_body(uint32_t c, uint32_t* smem, const 
                     wb = pos & ~31u;
                     codes32(wb, cw);
                 }
-                pos += code_of(cw, (int)(pos & 31u)) & 0xFFFu;
+                const uint32_t cd = code_of(cw, (int)(pos & 31u));
+                // the token's lazy literals, counted for the range holding pos
+                if (cd >> 12) atomicAdd(&rcnt[pos / S], cd >> 12);
+                pos += cd & 0xFFFu;
             }
         }
     }
     __syncthreads();
-    // tokens per range (decision at p: lazy literals, then a match or one literal)
-    uint32_t cnt = 0;
-    for (uint32_t w = w0; w < w1; ++w) {
-        uint32_t x = bm[w];
-        if (!x) continue;
-        uint32_t cw[16];
-        codes32(w << 5, cw);
-        asm volatile("" ::"v"(cw[0]), "v"(cw[1]), "v"(cw[2]), "v"(cw[3]), "v"(cw[4]), "v"(cw[5]), "v"(cw[6]),
-                     "v"(cw[7]), "v"(cw[8]), "v"(cw[9]), "v"(cw[10]), "v"(cw[11]), "v"(cw[12]), "v"(cw[13]),
-                     "v"(cw[14]), "v"(cw[15]));
-        while (x) {
-            const int k = __builtin_ctz(x);
-            x &= x - 1;
-            cnt += 1u + (code_of(cw, k) >> 12);
-        }
-    }
+    // tokens per range (decision at p: lazy literals, then a match or one
+    // literal): the range's token starts plus their lazy literals, both in
+    // LDS, so no decisions are read again
+    uint32_t cnt = rcnt[tid];
+    for (uint32_t w = w0; w < w1; ++w) cnt += (uint32_t)__popc(bm[w]);
     const uint32_t incl = wave_incl_scan(cnt, lane);
     if (lane == 63) misc[wv] = incl;
     __syncthreads();
