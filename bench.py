@@ -25,6 +25,7 @@ import ctypes
 import hashlib
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -62,6 +63,34 @@ def measured_traffic(path: str = TRAFFIC_SUMMARY):
     except OSError:
         pass
     return None
+
+
+def measured_issue(path: str = TRAFFIC_SUMMARY):
+    """Issue-side ceilings of the dominant kernels from the committed counter
+    summary: per kernel, the fraction of the VALU issue slots and of the LDS
+    pipe its launch used (the summary's 'VALU x  LDS pipe y' lines, or its
+    VALU_frac / LDS_frac columns), or None when there are none."""
+    out = {}
+    try:
+        with open(path) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return None
+    cols = None
+    for line in lines:
+        m = re.match(r"\s+(\S+(?:, \S+)*)\s+t =\s+[\d.]+ us\s+VALU ([\d.]+)\s+LDS pipe ([\d.]+)", line)
+        if m:
+            out[m.group(1)] = {"valu": float(m.group(2)), "lds_pipe": float(m.group(3))}
+            continue
+        f = line.split()
+        if f and f[0] == "kernel" and "VALU_frac" in f:
+            cols = f
+            continue
+        if cols and len(f) == len(cols) and f[0].startswith("l6_match"):
+            out[f[0]] = {"valu": float(f[cols.index("VALU_frac")]), "lds_pipe": float(f[cols.index("LDS_frac")])}
+    if not out:
+        return None
+    return {"source": os.path.relpath(path, ROOT), "kernels": out}
 
 
 def calibrate_copy(nbytes: int = 256 << 20, reps: int = 3) -> int:
@@ -331,6 +360,7 @@ def bench_single(args, data: np.ndarray, level: int, fmt: int, chunk: int, confi
             "unit": "GB/s",
             "frac": round(achieved * 1e9 / HBM_PEAK, 5),
             "traffic": traffic,
+            "issue": measured_issue(),
         },
     })
     if not args.no_cpu:
@@ -697,6 +727,7 @@ def bench_multi(args, specs, level: int, fmt: int, workload: str, cpu_kinds, tra
             "unit": "GB/s",
             "frac": round(b_alg / front / HBM_PEAK, 5),
             "traffic": measured_traffic(traffic_summary) if traffic_summary else None,
+            "issue": measured_issue(traffic_summary) if traffic_summary else None,
         },
     })
     if not args.no_cpu:
