@@ -1932,7 +1932,11 @@ hipError_t allow_lds(K* kernel, size_t smem) {
 template <int HB, int W, bool EXCH>
 void launch_candidates(const Plan& p) {
     constexpr size_t smem = la_lds_bytes<HB, W>();
-    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, smem, 64 * W));
+    // the candidate scans hold about 180 VGPRs (two waves per SIMD), fewer
+    // workgroups than their LDS alone would allow
+    static const uint32_t resident =
+        resident_per_cu(l1_candidate_kernel<HB, W, EXCH>, 64 * W, smem, per_cu(p, smem, 64 * W));
+    const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * resident);
     hipLaunchKernelGGL((l1_candidate_kernel<HB, W, EXCH>), dim3(g), dim3(64 * W), smem, p.stream, p.d_in, p.d_desc,
                        p.d_hdr, p.chunk, p.d_pd);
 }
@@ -1944,11 +1948,12 @@ hipError_t launch_barrier(const Plan& p) {
         launch_candidates<14, 2, EXCH>(p);
     } else {
         // Few chunks: six waves per chunk, one chunk per CU, for the shortest
-        // scan per chunk.  Many: two waves per chunk, four chunks per CU,
-        // where the scan is bound by its memory traffic.  Both give the same
-        // codes.
+        // scan per chunk.  Many: one wave per chunk (16 KiB of LDS, no
+        // fix-up of the later sub-ranges' first positions), eight chunks per
+        // CU, which the scan's registers (two waves per SIMD) allow.  Both
+        // give the same codes.
         if (p.nchunks <= 2u * (uint32_t)p.cus) launch_candidates<12, 6, EXCH>(p);
-        else launch_candidates<12, 2, EXCH>(p);
+        else launch_candidates<12, 1, EXCH>(p);
     }
     const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * per_cu(p, LB_LDS_BYTES, LC_THREADS));
     hipLaunchKernelGGL((l1_barrier_kernel<HB>), dim3(g), dim3(LC_THREADS), LB_LDS_BYTES, p.stream, p.d_in, p.d_desc,
