@@ -703,16 +703,31 @@ constexpr size_t la_lds_bytes() { return (size_t)W * (4u << HB) + (size_t)(W - 1
 
 template <int HB, int W, bool EXCH>
 __global__ __launch_bounds__(64 * W) void l1_candidate_kernel(const uint8_t* __restrict__ src,
-    const ChunkDesc* __restrict__ desc, const BatchHdr* __restrict__ hdr, uint32_t chunk,
-    uint16_t* __restrict__ cand) {
+    const ChunkDesc* __restrict__ desc, const uint32_t* __restrict__ order, BatchHdr* __restrict__ hdr,
+    uint32_t chunk, uint16_t* __restrict__ cand) {
     constexpr uint32_t LA_TAB = 1u << HB;
     constexpr int NT = 64 * W;
     extern __shared__ uint32_t la_smem[];
     uint32_t* tab = la_smem;                                         // W x 4096
     uint16_t* first = reinterpret_cast<uint16_t*>(la_smem + W * LA_TAB);  // (W - 1) x LA_TAB
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    // A grid that covers the batch takes one chunk per workgroup.  Otherwise
+    // chunks come from the device queue in `order` (full chunks, then the
+    // partial ones longest first), so workgroups finish together on uneven
+    // batches; with one chunk each, the queue's only effect would be every
+    // workgroup's atomic on one counter at the start.
     const uint32_t nchunks = hdr->nchunks;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const bool queued = gridDim.x < nchunks;
+    for (uint32_t it = blockIdx.x;; it += gridDim.x) {
+        uint32_t c;
+        if (queued) {
+            const uint32_t k = next_item(&hdr->queue[Q_SCAN], nchunks, la_smem);
+            if (k == NO_ITEM) break;
+            c = order[k];
+        } else {
+            if (it >= nchunks) break;
+            c = it;
+        }
         const uint32_t n = desc[c].len;
         const uint8_t* s = src + desc[c].in_off;
         uint16_t* Q = cand + (uint64_t)c * lb_tok_stride(chunk);  // LbLayout order
@@ -1938,7 +1953,7 @@ void launch_candidates(const Plan& p) {
         resident_per_cu(l1_candidate_kernel<HB, W, EXCH>, 64 * W, smem, per_cu(p, smem, 64 * W));
     const unsigned g = grid_for(p.persistent_grid, p.nchunks, p.cus * resident);
     hipLaunchKernelGGL((l1_candidate_kernel<HB, W, EXCH>), dim3(g), dim3(64 * W), smem, p.stream, p.d_in, p.d_desc,
-                       p.d_hdr, p.chunk, p.d_pd);
+                       p.d_order, p.d_hdr, p.chunk, p.d_pd);
 }
 
 template <int HB, bool EXCH>
@@ -2020,8 +2035,8 @@ hipError_t check_exchange_order(hipStream_t stream, bool* ok) {
 hipError_t set_kernel_attributes_l1(size_t lds_max) {
     hipError_t e = allow_lds(&l1_barrier_kernel<12>, LB_LDS_BYTES);
     if (e == hipSuccess) e = allow_lds(&l1_barrier_kernel<14>, LB_LDS_BYTES);
-    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 2, true>, la_lds_bytes<12, 2>());
-    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 2, false>, la_lds_bytes<12, 2>());
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 1, true>, la_lds_bytes<12, 1>());
+    if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 1, false>, la_lds_bytes<12, 1>());
     if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 6, true>, la_lds_bytes<12, 6>());
     if (e == hipSuccess) e = allow_lds(&l1_candidate_kernel<12, 6, false>, la_lds_bytes<12, 6>());
     if (e == hipSuccess && lds_max >= la_lds_bytes<14, 2>())
