@@ -51,7 +51,7 @@ struct BufDesc {
 };
 
 // Work-queue heads of the persistent kernels.
-enum Queue { Q_LZ = 0, Q_PRED = 1, Q_MATCH = 2, Q_PARSE = 3, Q_SCAN = 4, Q_COUNT = 8 };
+enum Queue { Q_LZ = 0, Q_PRED = 1, Q_MATCH = 2, Q_PARSE = 3, Q_SCAN = 4, Q_ASM = 5, Q_BLOCK = 6, Q_EMIT = 7, Q_COUNT = 8 };
 
 // Batch header in device memory, rewritten (queue heads zero) at each submit.
 struct BatchHdr {

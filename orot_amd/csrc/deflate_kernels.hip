@@ -1808,13 +1808,25 @@ __device__ __forceinline__ ChunkView make_view(const uint8_t* src, const ChunkDe
 __global__ __launch_bounds__(256) void assemble_kernel(const uint8_t* __restrict__ src,
                                                        const ChunkDesc* __restrict__ desc,
                                                        const BufDesc* __restrict__ bufs,
-                                                       const BatchHdr* __restrict__ hdr,
+                                                       BatchHdr* __restrict__ hdr,
                                                        const uint32_t* __restrict__ stage, uint64_t stage_words,
                                                        const ChunkInfo* __restrict__ info,
                                                        const Result* __restrict__ res, uint8_t* __restrict__ out_arena,
                                                        uint32_t hdr_bytes, uint32_t slices) {
+    __shared__ uint32_t s_item;
     const uint32_t items = hdr->nchunks * slices;
-    for (uint32_t it = blockIdx.x; it < items; it += gridDim.x) {
+    // chunks' outputs differ widely (a stored chunk copies its whole input,
+    // a chunk of zeros a few hundred bytes): items come from a device queue
+    // unless the grid covers them, one item per workgroup
+    const bool queued = gridDim.x < items;
+    for (uint32_t nx = blockIdx.x;; nx += gridDim.x) {
+        uint32_t it = nx;
+        if (queued) {
+            it = next_item(&hdr->queue[Q_ASM], items, &s_item);
+            if (it == NO_ITEM) break;
+        } else if (it >= items) {
+            break;
+        }
         const uint32_t c = it / slices, sl = it % slices;
         const uint32_t bi = desc[c].buf & CHUNK_BUF;
         if (res[bi].total_bytes == 0) continue;

@@ -1431,14 +1431,25 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
 __global__ __launch_bounds__(L6B_THREADS) void l6_block_kernel(const uint32_t* __restrict__ toks,
                                                                const uint32_t* __restrict__ ntok_in,
                                                                const uint32_t* __restrict__ blkpos,
-                                                               const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                               BatchHdr* __restrict__ hdr, uint32_t chunk,
                                                                uint32_t maxblk, BlockRec* __restrict__ recs) {
     // items block-major (block b of every chunk, then b + 1): a chunk has
     // fewer blocks than maxblk, and a chunk-major order with a grid that is a
     // multiple of maxblk would hand every workgroup the same block index
+    // Blocks differ in tokens and a chunk's slots past its last block are
+    // empty: items come from a device queue unless the grid covers them.
+    __shared__ uint32_t s_item;
     const uint32_t nc = hdr->nchunks;
     const uint32_t items = nc * maxblk;
-    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
+    const bool queued = gridDim.x < items;
+    for (uint32_t nx = blockIdx.x;; nx += gridDim.x) {
+        uint32_t w = nx;
+        if (queued) {
+            w = next_item(&hdr->queue[Q_BLOCK], items, &s_item);
+            if (w == NO_ITEM) break;
+        } else if (w >= items) {
+            break;
+        }
         l6_block_body(w % nc, w / nc, toks, ntok_in, blkpos, chunk, maxblk, recs);
         __syncthreads();
     }
@@ -1774,17 +1785,27 @@ __device__ __forceinline__ void l6_emit_body(uint32_t c, uint32_t b, const uint8
 
 __global__ __launch_bounds__(L6E_THREADS) void l6_emit_kernel(const uint8_t* __restrict__ src,
                                                               const ChunkDesc* __restrict__ desc,
-                                                              const BatchHdr* __restrict__ hdr, uint32_t chunk,
+                                                              BatchHdr* __restrict__ hdr, uint32_t chunk,
                                                               const uint32_t* __restrict__ toks,
                                                               const uint32_t* __restrict__ ntok_in,
                                                               const BlockRec* __restrict__ recs, uint32_t maxblk,
                                                               const ChunkInfo* __restrict__ info,
                                                               const Result* __restrict__ res,
                                                               uint32_t* __restrict__ stage, uint64_t stage_words) {
-    // items block-major, as in l6_block_kernel
+    // items block-major, from a device queue unless the grid covers them, as
+    // in l6_block_kernel
+    __shared__ uint32_t s_item;
     const uint32_t nc = hdr->nchunks;
     const uint32_t items = nc * maxblk;
-    for (uint32_t w = blockIdx.x; w < items; w += gridDim.x) {
+    const bool queued = gridDim.x < items;
+    for (uint32_t nx = blockIdx.x;; nx += gridDim.x) {
+        uint32_t w = nx;
+        if (queued) {
+            w = next_item(&hdr->queue[Q_EMIT], items, &s_item);
+            if (w == NO_ITEM) break;
+        } else if (w >= items) {
+            break;
+        }
         l6_emit_body(w % nc, w / nc, src, desc, chunk, toks, ntok_in, recs, maxblk, info, res, stage,
                      stage_words);
         __syncthreads();
