@@ -1956,6 +1956,20 @@ hipError_t allow_lds(K* kernel, size_t smem) {
                                (int)smem);
 }
 
+// Workgroups of `kernel` that stay resident on one CU, registers and LDS both
+// counted; `fallback` when HIP cannot say.  A launch with fixed shares of
+// equal items sized above this leaves the later workgroups' whole shares to
+// run after the first ones finish.
+template <class K>
+uint32_t resident_per_cu(K* kernel, int threads, size_t smem, uint32_t fallback) {
+    int r = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, reinterpret_cast<const void*>(kernel), threads, smem) !=
+            hipSuccess ||
+        r <= 0)
+        return fallback;
+    return (uint32_t)r;
+}
+
 template <int HB, int W, bool EXCH>
 void launch_candidates(const Plan& p) {
     constexpr size_t smem = la_lds_bytes<HB, W>();

@@ -242,20 +242,6 @@ __device__ __forceinline__ uint32_t next_item(uint32_t* queue, uint32_t count, u
     return k < count ? k : NO_ITEM;
 }
 
-// Workgroups of `kernel` that stay resident on one CU, registers and LDS both
-// counted; `fallback` when HIP cannot say.  A grid-stride launch of items of
-// equal cost sized above this leaves the later workgroups' whole shares of
-// items to run after the first ones finish.
-template <class K>
-inline uint32_t resident_per_cu(K* kernel, int threads, size_t smem, uint32_t fallback) {
-    int r = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&r, reinterpret_cast<const void*>(kernel), threads, smem) !=
-            hipSuccess ||
-        r <= 0)
-        return fallback;
-    return (uint32_t)r;
-}
-
 // Grid size for a launch over `count` items with at most `cap` workgroups;
 // persistent launches (graph capture) always take `cap`.
 inline unsigned grid_for(bool persistent, uint32_t count, uint32_t cap) {
