@@ -744,10 +744,10 @@ __global__ __launch_bounds__(64 * W) void l1_candidate_kernel(const uint8_t* __r
         uint32_t* my = tab + (uint32_t)wv * LA_TAB;
         const uint32_t a = (uint32_t)wv * SR, b = min(a + SR, nh);
         const uint32_t spare = lb_tok_stride(chunk) - 1u;  // past every index of the layout
-        // the 4-byte windows of 512 positions come from global memory; five
-        // groups of loads (clamped into [a, b)) stay in flight ahead of the
-        // exchanges (loads complete in order behind the code stores issued
-        // before them, so a shallower queue waits on those stores)
+        // the 4-byte windows of 512 positions come from global memory; four
+        // or five groups of loads (clamped into [a, b)) stay in flight ahead
+        // of the exchanges (loads complete in order behind the code stores
+        // issued before them, so a shallower queue waits on those stores)
         auto load8 = [&](uint32_t q0, uint32_t* w) {
 #pragma unroll
             for (int j = 0; j < 8; ++j) w[j] = ld32u(s + min(q0 + 64u * j + (uint32_t)lane, b - 1));
@@ -821,32 +821,61 @@ __global__ __launch_bounds__(64 * W) void l1_candidate_kernel(const uint8_t* __r
                 Q[valid ? qi : spare] = (uint16_t)cand_code(p, res[j]);
             }
         };
-        uint32_t w0[8], w1[8], w2[8], w3[8], w4[8], w5[8];
-        if (a < b) {
-            load8(a, w0);
-            load8(a + 512u, w1);
-            load8(a + 1024u, w2);
-            load8(a + 1536u, w3);
-            load8(a + 2048u, w4);
-        }
-        for (uint32_t p0 = a; p0 < b; p0 += 3072u) {
-            load8(p0 + 2560u, w5);
-            step(p0, w0);
-            if (p0 + 512u >= b) break;
-            load8(p0 + 3072u, w0);
-            step(p0 + 512u, w1);
-            if (p0 + 1024u >= b) break;
-            load8(p0 + 3584u, w1);
-            step(p0 + 1024u, w2);
-            if (p0 + 1536u >= b) break;
-            load8(p0 + 4096u, w2);
-            step(p0 + 1536u, w3);
-            if (p0 + 2048u >= b) break;
-            load8(p0 + 4608u, w3);
-            step(p0 + 2048u, w4);
-            if (p0 + 2560u >= b) break;
-            load8(p0 + 5120u, w4);
-            step(p0 + 2560u, w5);
+        if constexpr (W == 1) {
+            // one wave per chunk, eight chunks per CU: five groups in flight
+            uint32_t w0[8], w1[8], w2[8], w3[8], w4[8], w5[8];
+            if (a < b) {
+                load8(a, w0);
+                load8(a + 512u, w1);
+                load8(a + 1024u, w2);
+                load8(a + 1536u, w3);
+                load8(a + 2048u, w4);
+            }
+            for (uint32_t p0 = a; p0 < b; p0 += 3072u) {
+                load8(p0 + 2560u, w5);
+                step(p0, w0);
+                if (p0 + 512u >= b) break;
+                load8(p0 + 3072u, w0);
+                step(p0 + 512u, w1);
+                if (p0 + 1024u >= b) break;
+                load8(p0 + 3584u, w1);
+                step(p0 + 1024u, w2);
+                if (p0 + 1536u >= b) break;
+                load8(p0 + 4096u, w2);
+                step(p0 + 1536u, w3);
+                if (p0 + 2048u >= b) break;
+                load8(p0 + 4608u, w3);
+                step(p0 + 2048u, w4);
+                if (p0 + 2560u >= b) break;
+                load8(p0 + 5120u, w4);
+                step(p0 + 2560u, w5);
+            }
+        } else {
+            // several waves per chunk: four groups in flight keep the scan
+            // under 168 VGPRs, three waves per SIMD
+            uint32_t w0[8], w1[8], w2[8], w3[8], w4[8];
+            if (a < b) {
+                load8(a, w0);
+                load8(a + 512u, w1);
+                load8(a + 1024u, w2);
+                load8(a + 1536u, w3);
+            }
+            for (uint32_t p0 = a; p0 < b; p0 += 2560u) {
+                load8(p0 + 2048u, w4);
+                step(p0, w0);
+                if (p0 + 512u >= b) break;
+                load8(p0 + 2560u, w0);
+                step(p0 + 512u, w1);
+                if (p0 + 1024u >= b) break;
+                load8(p0 + 3072u, w1);
+                step(p0 + 1024u, w2);
+                if (p0 + 1536u >= b) break;
+                load8(p0 + 3584u, w2);
+                step(p0 + 1536u, w3);
+                if (p0 + 2048u >= b) break;
+                load8(p0 + 4096u, w3);
+                step(p0 + 2048u, w4);
+            }
         }
         __syncthreads();
         // a hash's first position in sub-range j takes its latest position
