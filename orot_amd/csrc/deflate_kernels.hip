@@ -821,8 +821,10 @@ __global__ __launch_bounds__(64 * W) void l1_candidate_kernel(const uint8_t* __r
                 Q[valid ? qi : spare] = (uint16_t)cand_code(p, res[j]);
             }
         };
-        if constexpr (W == 1) {
-            // one wave per chunk, eight chunks per CU: five groups in flight
+        if constexpr (W <= 2) {
+            // one wave per chunk (eight chunks per CU by registers), or two
+            // with 14-bit tables (one chunk per CU by LDS): five groups in
+            // flight
             uint32_t w0[8], w1[8], w2[8], w3[8], w4[8], w5[8];
             if (a < b) {
                 load8(a, w0);
