@@ -1428,7 +1428,7 @@ __device__ __forceinline__ void l6_block_body(uint32_t c, uint32_t b, const uint
 
 // Per block (grid-stride over chunk x block slots; slots past a chunk's last
 // block are skipped).
-__global__ __launch_bounds__(L6B_THREADS) void l6_block_kernel(const uint32_t* __restrict__ toks,
+__global__ __launch_bounds__(L6B_THREADS, 5) void l6_block_kernel(const uint32_t* __restrict__ toks,
                                                                const uint32_t* __restrict__ ntok_in,
                                                                const uint32_t* __restrict__ blkpos,
                                                                BatchHdr* __restrict__ hdr, uint32_t chunk,
